@@ -1,0 +1,132 @@
+"""ctypes binding of libmimadv.so (the C ABI declared in include/mimadv.h).
+
+The shared library is built in-tree by ``__graft_entry__.build()`` (or
+``make -C fdd3260_amd``).  There is no fallback: if the library is missing
+or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmimadv.so")
+
+# status codes
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_UNSUPPORTED = -3
+ERR_NOMEM = -4
+
+BC_PER, BC_NNF, BC_OPE = 0, 1, 2
+WENO, PPM, MUSCL, WENO5 = 0, 1, 2, 3
+LIM_MINMOD, LIM_MC, LIM_VL, LIM_FROMM = 0, 1, 2, 3
+WENO_BOTTOM_X0_IS_X1, WENO_BOTTOM_CLOSURE = 0, 1
+INT_EULER, INT_HEUN, INT_SSPRK3 = 1, 2, 3
+
+DIR_X, DIR_Y, DIR_Z, DIR_ALL = 1, 2, 4, 7
+OVERWRITE = 8
+HOST_PTRS = 16
+
+EXPORTED = (
+    "mimadv_create",
+    "mimadv_destroy",
+    "mimadv_last_error",
+    "mimadv_set_stream",
+    "mimadv_synchronize",
+    "mimadv_set_opts",
+    "mimadv_scalar_tendency",
+    "mimadv_momentum_tendency",
+    "mimadv_fill_halo",
+    "mimadv_integ_scal",
+    "mimadv_totalav",
+    "mimadv_advance",
+    "mimadv_last_kernel_ms",
+)
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("nz", C.c_int),
+        ("dx", C.c_double),
+        ("dy", C.c_double),
+        ("dz", C.c_double),
+        ("fdz0", C.POINTER(C.c_double)),
+        ("dzw", C.POINTER(C.c_double)),
+        ("dz1w", C.POINTER(C.c_double)),
+        ("dz1", C.POINTER(C.c_double)),
+        ("bcl", C.c_int * 6),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("scheme", C.c_int),
+        ("muscl_limiter", C.c_int),
+        ("ppm_limiter", C.c_int),
+        ("lim_tol", C.c_double),
+        ("weno_bottom", C.c_int),
+        ("mom_ord", C.c_int),
+        ("lavisc", C.c_int),
+    ]
+
+
+_lib = None
+
+
+class MimadvError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load libmimadv.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MimadvError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(the HIP library has no CPU fallback)"
+        )
+    lib = C.CDLL(LIB_PATH)
+    vp, i, d, u = C.c_void_p, C.c_int, C.c_double, C.c_uint
+    pp = C.POINTER(C.c_void_p)
+    ip = C.POINTER(C.c_int)
+    lib.mimadv_create.argtypes = [C.POINTER(Grid), C.POINTER(Opts), C.POINTER(C.c_void_p)]
+    lib.mimadv_create.restype = i
+    lib.mimadv_destroy.argtypes = [vp]
+    lib.mimadv_destroy.restype = None
+    lib.mimadv_last_error.argtypes = [vp]
+    lib.mimadv_last_error.restype = C.c_char_p
+    lib.mimadv_set_stream.argtypes = [vp, vp]
+    lib.mimadv_set_stream.restype = i
+    lib.mimadv_synchronize.argtypes = [vp]
+    lib.mimadv_synchronize.restype = i
+    lib.mimadv_set_opts.argtypes = [vp, C.POINTER(Opts)]
+    lib.mimadv_set_opts.restype = i
+    lib.mimadv_scalar_tendency.argtypes = [vp, d, vp, vp, vp, i, pp, pp, ip, u]
+    lib.mimadv_scalar_tendency.restype = i
+    lib.mimadv_momentum_tendency.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp, vp, u]
+    lib.mimadv_momentum_tendency.restype = i
+    lib.mimadv_fill_halo.argtypes = [vp, i, pp, u]
+    lib.mimadv_fill_halo.restype = i
+    lib.mimadv_integ_scal.argtypes = [vp, i, i, d, i, pp, pp, pp, pp, u]
+    lib.mimadv_integ_scal.restype = i
+    lib.mimadv_totalav.argtypes = [vp, vp, C.POINTER(d), C.POINTER(d), u]
+    lib.mimadv_totalav.restype = i
+    lib.mimadv_advance.argtypes = [vp, i, i, d, vp, vp, vp, i, pp, ip, u]
+    lib.mimadv_advance.restype = i
+    lib.mimadv_last_kernel_ms.argtypes = [vp, C.POINTER(d)]
+    lib.mimadv_last_kernel_ms.restype = i
+    _lib = lib
+    return lib
+
+
+def ptr_array(ptrs):
+    arr = (C.c_void_p * max(1, len(ptrs)))()
+    for n, p in enumerate(ptrs):
+        arr[n] = p
+    return arr

@@ -1,0 +1,222 @@
+"""Host-side mirror of MIMICA's advection interface over libmimadv.
+
+The reference exposes (src/TOOLS/advection.f90:79)
+    advect, advection_x, advection_y, advection_z, advection_ac, ...
+and (src/INTEG/windadv.f90:41) qnadv_uvw.  ``Advection`` keeps those names
+and argument meanings for arrays held in Python:
+
+  * torch tensors on the GPU are passed as device pointers;
+  * numpy arrays (host memory) are staged by the library
+    (MIMADV_HOST_PTRS), like the Fortran shim does for a CPU caller.
+
+Arrays are float64 of shape (nz, ny, nx), i.e. the reference's Fortran
+(nx, ny, nz) layout.  Errors raise ``MimadvError`` carrying the library's
+message (the reference has no error returns; it stops later in checkall).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import MimadvError
+from .grid import Grid
+
+SCHEMES = {"weno": _lib.WENO, "ppm": _lib.PPM, "muscl": _lib.MUSCL, "weno5": _lib.WENO5}
+LIMITERS = {"minmod": _lib.LIM_MINMOD, "MC": _lib.LIM_MC, "VL": _lib.LIM_VL, "Fromm": _lib.LIM_FROMM}
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+class Advection:
+    """One library context: grid + options (scal_adv, muscllimiter,
+    ppmlimiter, lim_tol, mom_ord, lavisc) + a HIP stream."""
+
+    def __init__(self, grid: Grid, scal_adv="weno", muscllimiter="minmod", ppmlimiter="PPM",
+                 lim_tol=1.0e-9, mom_ord=4, lavisc=True, weno_bottom=_lib.WENO_BOTTOM_X0_IS_X1):
+        self.lib = _lib.load()
+        self.grid = grid
+        self._cgrid = grid.to_c()
+        self.opts = _lib.Opts()
+        self._set_opts(scal_adv, muscllimiter, ppmlimiter, lim_tol, mom_ord, lavisc, weno_bottom)
+        h = C.c_void_p()
+        rc = self.lib.mimadv_create(C.byref(self._cgrid), C.byref(self.opts), C.byref(h))
+        self._h = h
+        if rc != _lib.OK:
+            msg = self.lib.mimadv_last_error(h).decode() if h.value else "mimadv_create failed"
+            if h.value:
+                self.lib.mimadv_destroy(h)
+            self._h = C.c_void_p()
+            raise MimadvError(f"mimadv_create: {msg} (status {rc})")
+
+    # -- options -----------------------------------------------------------
+    def _set_opts(self, scal_adv, muscllimiter, ppmlimiter, lim_tol, mom_ord, lavisc, weno_bottom):
+        if scal_adv not in SCHEMES:
+            raise MimadvError(f"unknown scal_adv {scal_adv!r}")
+        if muscllimiter not in LIMITERS:
+            raise MimadvError(f"unknown muscllimiter {muscllimiter!r}")
+        o = self.opts
+        o.scheme = SCHEMES[scal_adv]
+        o.muscl_limiter = LIMITERS[muscllimiter]
+        o.ppm_limiter = 1 if ppmlimiter == "PPM" else 0
+        o.lim_tol = float(lim_tol)
+        o.weno_bottom = int(weno_bottom)
+        o.mom_ord = int(mom_ord)
+        o.lavisc = 1 if lavisc else 0
+        self.scal_adv = scal_adv
+
+    def set_options(self, scal_adv=None, muscllimiter=None, ppmlimiter=None, lim_tol=None, mom_ord=None,
+                    lavisc=None, weno_bottom=None):
+        inv_s = {v: k for k, v in SCHEMES.items()}
+        inv_l = {v: k for k, v in LIMITERS.items()}
+        o = self.opts
+        self._set_opts(
+            scal_adv if scal_adv is not None else inv_s[o.scheme],
+            muscllimiter if muscllimiter is not None else inv_l[o.muscl_limiter],
+            ppmlimiter if ppmlimiter is not None else ("PPM" if o.ppm_limiter else "none"),
+            lim_tol if lim_tol is not None else o.lim_tol,
+            mom_ord if mom_ord is not None else o.mom_ord,
+            lavisc if lavisc is not None else bool(o.lavisc),
+            weno_bottom if weno_bottom is not None else o.weno_bottom,
+        )
+        self._check(self.lib.mimadv_set_opts(self._h, C.byref(self.opts)), "mimadv_set_opts")
+
+    # -- plumbing ----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.mimadv_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != _lib.OK:
+            raise MimadvError(f"{what}: {self.lib.mimadv_last_error(self._h).decode()} (status {rc})")
+
+    def _ptrs(self, arrays, writable=()):
+        """Pointers and the HOST_PTRS flag for a homogeneous list of arrays."""
+        host = None
+        ptrs = []
+        shape = self.grid.shape
+        for a in arrays:
+            if a is None:
+                ptrs.append(None)
+                continue
+            if _is_torch(a):
+                import torch
+
+                if a.dtype != torch.float64 or not a.is_contiguous() or tuple(a.shape) != shape:
+                    raise MimadvError(f"expected a contiguous float64 tensor of shape {shape}")
+                if not a.is_cuda:
+                    raise MimadvError("torch tensors must live on the GPU (use numpy for host arrays)")
+                is_host = False
+                ptrs.append(a.data_ptr())
+            else:
+                if not isinstance(a, np.ndarray) or a.dtype != np.float64 or a.shape != shape \
+                        or not a.flags.c_contiguous:
+                    raise MimadvError(f"expected a C-contiguous float64 array of shape {shape}")
+                is_host = True
+                ptrs.append(a.ctypes.data)
+            if host is None:
+                host = is_host
+            elif host != is_host:
+                raise MimadvError("cannot mix host and device arrays in one call")
+        for a in writable:
+            if isinstance(a, np.ndarray) and not a.flags.writeable:
+                raise MimadvError("output array is read-only")
+        return ptrs, (_lib.HOST_PTRS if host else 0)
+
+    def stream_sync(self):
+        self._check(self.lib.mimadv_synchronize(self._h), "mimadv_synchronize")
+
+    def set_stream(self, stream_ptr):
+        self._check(self.lib.mimadv_set_stream(self._h, stream_ptr), "mimadv_set_stream")
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.mimadv_last_kernel_ms(self._h, C.byref(v)), "mimadv_last_kernel_ms")
+        return v.value
+
+    # -- scalar advection (advection.f90) -----------------------------------
+    def scalar_tendency(self, dt0, u, v, w, q_list, xs_list, adv=None, dirs=_lib.DIR_ALL, overwrite=False):
+        """xs_f = ((xs_f - X) - Y) - Z for every field (advection_x/y/z)."""
+        nf = len(q_list)
+        if len(xs_list) != nf:
+            raise MimadvError("q and xs lists differ in length")
+        ptrs, hflag = self._ptrs([u, v, w] + list(q_list) + list(xs_list), writable=xs_list)
+        pu, pv, pw = ptrs[:3]
+        pq = _lib.ptr_array(ptrs[3:3 + nf])
+        px = _lib.ptr_array(ptrs[3 + nf:])
+        ad = None
+        if adv is not None:
+            ad = (C.c_int * nf)(*[1 if a else 0 for a in adv])
+        flags = int(dirs) | hflag | (_lib.OVERWRITE if overwrite else 0)
+        rc = self.lib.mimadv_scalar_tendency(self._h, float(dt0), pu, pv, pw, nf, pq, px, ad, flags)
+        self._check(rc, "mimadv_scalar_tendency")
+
+    def advection_x(self, dtl, xtmp, xs, u, adv=False):
+        self.scalar_tendency(dtl, u, None, None, [xtmp], [xs], [adv], dirs=_lib.DIR_X)
+
+    def advection_y(self, dtl, xtmp, xs, v, adv=False):
+        self.scalar_tendency(dtl, None, v, None, [xtmp], [xs], [adv], dirs=_lib.DIR_Y)
+
+    def advection_z(self, dtl, xtmp, xs, w, adv=False):
+        self.scalar_tendency(dtl, None, None, w, [xtmp], [xs], [adv], dirs=_lib.DIR_Z)
+
+    def advection_ac(self, dt0, u, v, w, xx, xs, adv=False):
+        """Single tracer: xs += xsx + xsy + xsz (advection.f90:801-869)."""
+        self.scalar_tendency(dt0, u, v, w, [xx], [xs], [adv])
+
+    # -- momentum (windadv.f90) --------------------------------------------
+    def qnadv_uvw(self, ntime, mom, wind_adv, out):
+        """out (ua, va, wa) = +div of the momenta mom advected by wind_adv."""
+        mu, mv, mw = mom
+        ut, vt, wt = wind_adv
+        ua, va, wa = out
+        ptrs, hflag = self._ptrs([mu, mv, mw, ut, vt, wt, ua, va, wa], writable=out)
+        rc = self.lib.mimadv_momentum_tendency(self._h, int(ntime), *ptrs, hflag)
+        self._check(rc, "mimadv_momentum_tendency")
+
+    # -- boundaries, stage combine, budgets, stepping -----------------------
+    def fill_halo(self, fields):
+        ptrs, hflag = self._ptrs(list(fields), writable=fields)
+        rc = self.lib.mimadv_fill_halo(self._h, len(fields), _lib.ptr_array(ptrs), hflag)
+        self._check(rc, "mimadv_fill_halo")
+
+    def integ_scal(self, sorder, im, dt, xold, a, tend, x):
+        ptrs, hflag = self._ptrs([xold, a, tend, x], writable=[tend, x])
+        arr = [_lib.ptr_array([p]) for p in ptrs]
+        rc = self.lib.mimadv_integ_scal(self._h, int(sorder), int(im), float(dt), 1, *arr, hflag)
+        self._check(rc, "mimadv_integ_scal")
+
+    def totalav(self, x):
+        (p,), hflag = self._ptrs([x])
+        mean, wsum = C.c_double(), C.c_double()
+        rc = self.lib.mimadv_totalav(self._h, p, C.byref(mean), C.byref(wsum), hflag)
+        self._check(rc, "mimadv_totalav")
+        return mean.value, wsum.value
+
+    def advance(self, integrator, nsteps, dt, u, v, w, fields, adv=None):
+        ptrs, hflag = self._ptrs([u, v, w] + list(fields), writable=fields)
+        if hflag:
+            raise MimadvError("advance works on device tensors")
+        nf = len(fields)
+        ad = None
+        if adv is not None:
+            ad = (C.c_int * nf)(*[1 if a else 0 for a in adv])
+        rc = self.lib.mimadv_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1], ptrs[2],
+                                     nf, _lib.ptr_array(ptrs[3:]), ad, 0)
+        self._check(rc, "mimadv_advance")

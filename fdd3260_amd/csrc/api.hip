@@ -1,0 +1,503 @@
+// api.hip -- the C ABI of libmimadv (declared in include/mimadv.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/mimadv.h"
+#include "mimadv_internal.hpp"
+
+using namespace mimadv;
+
+struct mimadv_ctx {
+    GridDev g{};
+    SchemeParams sp{};
+    mimadv_opts opts{};
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double *d_metrics = nullptr;  // fdz0, dzw, dz1w, dz1, ppmc
+    double *d_red = nullptr;      // reduction scratch [2 + nz]
+    // staging buffers for MIMADV_HOST_PTRS calls
+    double *stage = nullptr;
+    size_t stage_elems = 0;
+    // advance scratch: two arrays per field
+    double *work = nullptr;
+    size_t work_elems = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    bool timing_pending = false;
+    size_t n3() const { return (size_t)g.nx * g.ny * g.nz; }
+};
+
+namespace {
+
+int fail(mimadv_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int hipfail(mimadv_ctx *c, hipError_t e, const char *what)
+{
+    return fail(c, MIMADV_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(ctx, expr)                                   \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hipfail(ctx, e_, #expr); \
+    } while (0)
+
+int check_opts(mimadv_ctx *c, const mimadv_opts *o)
+{
+    if (!o) return fail(c, MIMADV_ERR_ARG, "opts is NULL");
+    if (o->scheme < MIMADV_WENO || o->scheme > MIMADV_WENO5) return fail(c, MIMADV_ERR_ARG, "unknown scheme");
+    if (o->muscl_limiter < 0 || o->muscl_limiter > 3) return fail(c, MIMADV_ERR_ARG, "unknown MUSCL limiter");
+    if (o->weno_bottom != 0 && o->weno_bottom != 1) return fail(c, MIMADV_ERR_ARG, "unknown weno_bottom convention");
+    if (o->mom_ord != 2 && o->mom_ord != 4) return fail(c, MIMADV_ERR_ARG, "mom_ord must be 2 or 4");
+    if (!(o->lim_tol > 0.0)) return fail(c, MIMADV_ERR_ARG, "lim_tol must be positive");
+    return MIMADV_OK;
+}
+
+void apply_opts(mimadv_ctx *c, const mimadv_opts *o)
+{
+    c->opts = *o;
+    c->sp.scheme = o->scheme;
+    c->sp.muscl_limiter = o->muscl_limiter;
+    c->sp.ppm_limiter = o->ppm_limiter ? 1 : 0;
+    c->sp.weno_bottom = o->weno_bottom;
+    c->sp.lim_tol = o->lim_tol;
+    c->sp.dt0 = 0.0;
+}
+
+// Device staging for host-pointer calls: returns a device buffer of n3()*count.
+int ensure_stage(mimadv_ctx *c, size_t count)
+{
+    size_t need = c->n3() * count;
+    if (need <= c->stage_elems) return MIMADV_OK;
+    if (c->stage) hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_elems = 0;
+    hipError_t e = hipMalloc(&c->stage, need * sizeof(double));
+    if (e != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "staging allocation failed");
+    c->stage_elems = need;
+    return MIMADV_OK;
+}
+
+int ensure_work(mimadv_ctx *c, size_t count)
+{
+    size_t need = c->n3() * count;
+    if (need <= c->work_elems) return MIMADV_OK;
+    if (c->work) hipFree(c->work);
+    c->work = nullptr;
+    c->work_elems = 0;
+    hipError_t e = hipMalloc(&c->work, need * sizeof(double));
+    if (e != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "work allocation failed");
+    c->work_elems = need;
+    return MIMADV_OK;
+}
+
+int check_ptr_dev(mimadv_ctx *c, const void *p, const char *name)
+{
+    if (!p) return fail(c, MIMADV_ERR_ARG, std::string(name) + " is NULL");
+    return MIMADV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx **out)
+{
+    if (!out) return MIMADV_ERR_ARG;
+    *out = nullptr;
+    if (!grid) return MIMADV_ERR_ARG;
+    mimadv_ctx *c = new (std::nothrow) mimadv_ctx();
+    if (!c) return MIMADV_ERR_NOMEM;
+    auto bad = [&](int code) {
+        // keep the context so the caller can read the message, but mark it
+        *out = c;
+        return code;
+    };
+    if (grid->nx < 11) return bad(fail(c, MIMADV_ERR_ARG, "nx must be >= 11 (MAXX >= 6)"));
+    if (grid->ny != 1 && grid->ny < 11) return bad(fail(c, MIMADV_ERR_ARG, "ny must be 1 or >= 11"));
+    if (grid->nz < 8) return bad(fail(c, MIMADV_ERR_ARG, "nz must be >= 8"));
+    if (!(grid->dx > 0) || !(grid->dy > 0) || !(grid->dz > 0)) return bad(fail(c, MIMADV_ERR_ARG, "spacings must be positive"));
+    if (!grid->fdz0 || !grid->dzw || !grid->dz1w || !grid->dz1) return bad(fail(c, MIMADV_ERR_ARG, "metric arrays must be given"));
+    for (int b = 0; b < 6; ++b)
+        if (grid->bcl[b] < 0 || grid->bcl[b] > 2) return bad(fail(c, MIMADV_ERR_ARG, "unknown boundary code"));
+    int rc = check_opts(c, opts);
+    if (rc) return bad(rc);
+    apply_opts(c, opts);
+
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) return bad(hipfail(c, e, "hipGetDevice"));
+    c->g.nx = grid->nx;
+    c->g.ny = grid->ny;
+    c->g.nz = grid->nz;
+    c->g.dx = grid->dx;
+    c->g.dy = grid->dy;
+    c->g.dz = grid->dz;
+    for (int b = 0; b < 6; ++b) c->g.bcl[b] = grid->bcl[b];
+    int nz = grid->nz;
+    std::vector<double> ppmc = ppm_coefficients(nz, grid->dz1w);
+    size_t nmet = (size_t)4 * nz + ppmc.size();
+    e = hipMalloc(&c->d_metrics, nmet * sizeof(double));
+    if (e != hipSuccess) return bad(fail(c, MIMADV_ERR_NOMEM, "metric allocation failed"));
+    std::vector<double> h(nmet);
+    std::memcpy(h.data() + 0 * nz, grid->fdz0, nz * sizeof(double));
+    std::memcpy(h.data() + 1 * nz, grid->dzw, nz * sizeof(double));
+    std::memcpy(h.data() + 2 * nz, grid->dz1w, nz * sizeof(double));
+    std::memcpy(h.data() + 3 * nz, grid->dz1, nz * sizeof(double));
+    std::memcpy(h.data() + 4 * nz, ppmc.data(), ppmc.size() * sizeof(double));
+    e = hipMemcpy(c->d_metrics, h.data(), nmet * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bad(hipfail(c, e, "metric upload"));
+    c->g.zm.fdz0 = c->d_metrics;
+    c->g.zm.dzw = c->d_metrics + nz;
+    c->g.zm.dz1w = c->d_metrics + 2 * nz;
+    c->g.zm.dz1 = c->d_metrics + 3 * nz;
+    c->g.zm.ppmc = c->d_metrics + 4 * nz;
+    c->g.zm.dz = grid->dz;
+    e = hipMalloc(&c->d_red, (size_t)(2 + nz) * sizeof(double));
+    if (e != hipSuccess) return bad(fail(c, MIMADV_ERR_NOMEM, "reduction allocation failed"));
+    e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bad(hipfail(c, e, "hipStreamCreate"));
+    c->stream = c->own_stream;
+    e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) return bad(hipfail(c, e, "hipEventCreate"));
+    *out = c;
+    return MIMADV_OK;
+}
+
+void mimadv_destroy(mimadv_ctx *c)
+{
+    if (!c) return;
+    if (c->own_stream) hipStreamSynchronize(c->own_stream);
+    if (c->stream && c->stream != c->own_stream) hipStreamSynchronize(c->stream);
+    if (c->d_metrics) hipFree(c->d_metrics);
+    if (c->d_red) hipFree(c->d_red);
+    if (c->stage) hipFree(c->stage);
+    if (c->work) hipFree(c->work);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+const char *mimadv_last_error(const mimadv_ctx *c)
+{
+    if (!c) return "NULL context";
+    return c->err.c_str();
+}
+
+int mimadv_set_stream(mimadv_ctx *c, void *stream)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return MIMADV_OK;
+}
+
+int mimadv_synchronize(mimadv_ctx *c)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MIMADV_OK;
+}
+
+int mimadv_set_opts(mimadv_ctx *c, const mimadv_opts *o)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_opts(c, o);
+    if (rc) return rc;
+    apply_opts(c, o);
+    return MIMADV_OK;
+}
+
+int mimadv_scalar_tendency(mimadv_ctx *c, double dt0, const double *u, const double *v, const double *w, int nf,
+                           const double *const *q, double *const *xs, const int *adv_form, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0) return fail(c, MIMADV_ERR_ARG, "nf < 0");
+    if (nf == 0) return MIMADV_OK;
+    if (!q || !xs) return fail(c, MIMADV_ERR_ARG, "q/xs arrays are NULL");
+    unsigned dirs = flags & MIMADV_DIR_ALL;
+    if (dirs == 0) return fail(c, MIMADV_ERR_ARG, "no direction selected");
+    bool three_d = c->g.ny > 1;
+    if ((dirs & MIMADV_DIR_X) && !u) return fail(c, MIMADV_ERR_ARG, "u is NULL");
+    if ((dirs & MIMADV_DIR_Y) && three_d && !v) return fail(c, MIMADV_ERR_ARG, "v is NULL");
+    if ((dirs & MIMADV_DIR_Z) && !w) return fail(c, MIMADV_ERR_ARG, "w is NULL");
+    if (c->sp.scheme == MIMADV_PPM && !(std::isfinite(dt0))) return fail(c, MIMADV_ERR_ARG, "PPM needs a finite dt0");
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f] || !xs[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+    }
+    SchemeParams sp = c->sp;
+    sp.dt0 = dt0;
+    size_t n3 = c->n3();
+    size_t bytes = n3 * sizeof(double);
+    bool host = (flags & MIMADV_HOST_PTRS) != 0;
+    const double *du = u, *dv = v, *dw = w;
+    if (host) {
+        int rc = ensure_stage(c, 5);
+        if (rc) return rc;
+        double *su = c->stage, *sv = c->stage + n3, *sw = c->stage + 2 * n3;
+        if (u) HIPCHK(c, hipMemcpyAsync(su, u, bytes, hipMemcpyHostToDevice, c->stream));
+        if (v && three_d) HIPCHK(c, hipMemcpyAsync(sv, v, bytes, hipMemcpyHostToDevice, c->stream));
+        if (w) HIPCHK(c, hipMemcpyAsync(sw, w, bytes, hipMemcpyHostToDevice, c->stream));
+        du = su;
+        dv = sv;
+        dw = sw;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    float kms_total = 0.0f;
+    for (int f = 0; f < nf; ++f) {
+        ScalarArgs a{};
+        a.u = du;
+        a.v = dv;
+        a.w = dw;
+        a.adv = adv_form ? adv_form[f] : 0;
+        a.epi = (flags & MIMADV_OVERWRITE) ? EPI_OVERWRITE : EPI_ACCUM;
+        a.dirmask = (int)dirs;
+        a.fill = 0;
+        if (host) {
+            double *sq = c->stage + 3 * n3, *sx = c->stage + 4 * n3;
+            HIPCHK(c, hipMemcpyAsync(sq, q[f], bytes, hipMemcpyHostToDevice, c->stream));
+            // xs is staged in even for MIMADV_OVERWRITE: its ghost columns are
+            // not written by the kernel and must come back unchanged
+            HIPCHK(c, hipMemcpyAsync(sx, xs[f], bytes, hipMemcpyHostToDevice, c->stream));
+            a.q = sq;
+            a.xs = sx;
+            HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+            HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
+            HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+            HIPCHK(c, hipMemcpyAsync(xs[f], sx, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            kms_total += ms;
+        } else {
+            a.q = q[f];
+            a.xs = xs[f];
+            HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
+        }
+    }
+    if (host) {
+        c->last_ms = kms_total;
+        c->timing_pending = false;
+    } else {
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        c->timing_pending = true;
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_momentum_tendency(mimadv_ctx *c, int ntime, const double *mu, const double *mv, const double *mw,
+                             const double *ut, const double *vt, const double *wt, double *ua, double *va, double *wa,
+                             unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    bool three_d = c->g.ny > 1;
+    if (!mu || !mw || !ut || !wt || !ua || !wa) return fail(c, MIMADV_ERR_ARG, "NULL momentum/velocity array");
+    if (three_d && (!mv || !vt || !va)) return fail(c, MIMADV_ERR_ARG, "NULL v array in 3D");
+    size_t n3 = c->n3(), bytes = n3 * sizeof(double);
+    if (flags & MIMADV_HOST_PTRS) {
+        int rc = ensure_stage(c, 9);
+        if (rc) return rc;
+        double *s = c->stage;
+        const double *hin[6] = {mu, mv, mw, ut, vt, wt};
+        for (int a = 0; a < 6; ++a)
+            if (hin[a] && (three_d || (a != 1 && a != 4)))
+                HIPCHK(c, hipMemcpyAsync(s + a * n3, hin[a], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, launch_momentum_fp64(c->g, c->opts.mom_ord, c->opts.lavisc, ntime, s, s + n3, s + 2 * n3, s + 3 * n3,
+                                       s + 4 * n3, s + 5 * n3, s + 6 * n3, s + 7 * n3, s + 8 * n3, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+        HIPCHK(c, hipMemcpyAsync(ua, s + 6 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (three_d) HIPCHK(c, hipMemcpyAsync(va, s + 7 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(wa, s + 8 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        c->last_ms = ms;
+        return MIMADV_OK;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_momentum_fp64(c->g, c->opts.mom_ord, c->opts.lavisc, ntime, mu, mv, mw, ut, vt, wt, ua, va, wa,
+                                   c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+int mimadv_fill_halo(mimadv_ctx *c, int nf, double *const *x, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0 || (nf > 0 && !x)) return fail(c, MIMADV_ERR_ARG, "bad field list");
+    size_t bytes = c->n3() * sizeof(double);
+    for (int f = 0; f < nf; ++f) {
+        if (!x[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        if (flags & MIMADV_HOST_PTRS) {
+            int rc = ensure_stage(c, 1);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->stage, x[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_fill_scalar(c->g, c->stage, c->stream));
+            HIPCHK(c, hipMemcpyAsync(x[f], c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, launch_fill_scalar(c->g, x[f], c->stream));
+        }
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_integ_scal(mimadv_ctx *c, int sorder, int im, double dt, int nf, const double *const *xold,
+                      const double *const *a, double *const *tend, double *const *x, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0 || (nf > 0 && (!xold || !a || !x || (sorder > 1 && !tend))))
+        return fail(c, MIMADV_ERR_ARG, "bad field list");
+    if (im < 1) return fail(c, MIMADV_ERR_ARG, "stage index im is 1-based");
+    size_t n3 = c->n3(), bytes = n3 * sizeof(double);
+    for (int f = 0; f < nf; ++f) {
+        if (flags & MIMADV_HOST_PTRS) {
+            int rc = ensure_stage(c, 4);
+            if (rc) return rc;
+            double *s = c->stage;
+            HIPCHK(c, hipMemcpyAsync(s, xold[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(s + n3, a[f], bytes, hipMemcpyHostToDevice, c->stream));
+            if (sorder > 1) HIPCHK(c, hipMemcpyAsync(s + 2 * n3, tend[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_integ_scal((long)n3, sorder, im, dt, s, s + n3, s + 2 * n3, s + 3 * n3, c->stream));
+            if (sorder > 1) HIPCHK(c, hipMemcpyAsync(tend[f], s + 2 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipMemcpyAsync(x[f], s + 3 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, launch_integ_scal((long)n3, sorder, im, dt, xold[f], a[f], sorder > 1 ? tend[f] : nullptr, x[f],
+                                        c->stream));
+        }
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_totalav(mimadv_ctx *c, const double *x, double *mean, double *dzw_sum, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!x) return fail(c, MIMADV_ERR_ARG, "x is NULL");
+    const double *dx = x;
+    if (flags & MIMADV_HOST_PTRS) {
+        int rc = ensure_stage(c, 1);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->stage, x, c->n3() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        dx = c->stage;
+    }
+    HIPCHK(c, launch_totalav(c->g, dx, c->d_red, c->stream));
+    double h[2];
+    HIPCHK(c, hipMemcpyAsync(h, c->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mean) *mean = h[0];
+    if (dzw_sum) *dzw_sum = h[1];
+    return MIMADV_OK;
+}
+
+int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                   const double *w, int nf, double *const *q, const int *adv_form, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "mimadv_advance works on device arrays");
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED,
+                    "only Heun (sorder=2) and SSP-RK3 are supported; sorder=1 runs Lax-Wendroff in the reference");
+    for (int b = 0; b < 4; ++b)
+        if (c->g.bcl[b] != MIMADV_BC_PER && !(c->g.ny == 1 && b >= 2))
+            return fail(c, MIMADV_ERR_UNSUPPORTED, "mimadv_advance needs periodic lateral boundaries");
+    if (c->g.bcl[4] == MIMADV_BC_PER || c->g.bcl[5] == MIMADV_BC_PER)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "mimadv_advance needs non-periodic vertical boundaries");
+    if (nsteps < 0 || nf < 0) return fail(c, MIMADV_ERR_ARG, "negative count");
+    if (nf == 0 || nsteps == 0) return MIMADV_OK;
+    if (!u || !w || (c->g.ny > 1 && !v) || !q) return fail(c, MIMADV_ERR_ARG, "NULL array");
+    for (int f = 0; f < nf; ++f)
+        if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+    int rc = ensure_work(c, (size_t)2 * nf);
+    if (rc) return rc;
+    size_t n3 = c->n3();
+    SchemeParams sp = c->sp;
+    sp.dt0 = dt;
+    // the scratch arrays start from the state so their ghosts are defined
+    for (int f = 0; f < nf; ++f) {
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f + 1) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        int nst = integrator == MIMADV_INT_HEUN ? 2 : 3;
+        for (int st = 1; st <= nst; ++st) {
+            for (int f = 0; f < nf; ++f) {
+                double *w1 = c->work + (2 * f) * n3, *w2 = c->work + (2 * f + 1) * n3;
+                ScalarArgs a{};
+                a.u = u;
+                a.v = v;
+                a.w = w;
+                a.adv = adv_form ? adv_form[f] : 0;
+                a.dirmask = 7;
+                a.fill = 1;
+                if (integrator == MIMADV_INT_HEUN) {
+                    // w1: stage state x*, w2: accumulated tendency (integ.h)
+                    a.epi = EPI_HEUN;
+                    a.tend = w2;
+                    a.q0 = q[f];
+                    if (st == 1) {
+                        a.q = q[f];
+                        a.qn = w1;
+                        a.b_lm = 0.0;
+                        a.adt = 1.0 * dt;
+                    } else {
+                        a.q = w1;
+                        a.qn = q[f];
+                        a.b_lm = 1.0;
+                        a.adt = 0.5 * dt;
+                    }
+                } else {
+                    a.epi = EPI_SO;
+                    a.dt = dt;
+                    a.q0 = q[f];
+                    if (st == 1) {
+                        a.q = q[f]; a.qn = w1; a.c0 = 0.0; a.c1 = 1.0;
+                    } else if (st == 2) {
+                        a.q = w1; a.qn = w2; a.c0 = 0.75; a.c1 = 0.25;
+                    } else {
+                        a.q = w2; a.qn = q[f]; a.c0 = 1.0 / 3.0; a.c1 = 2.0 / 3.0;
+                    }
+                }
+                HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
+            }
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->timing_pending = false;
+    return MIMADV_OK;
+}
+
+int mimadv_last_kernel_ms(mimadv_ctx *c, double *ms)
+{
+    if (!c || !ms) return MIMADV_ERR_ARG;
+    if (c->timing_pending) {
+        HIPCHK(c, hipEventSynchronize(c->ev1));
+        float t = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+        c->last_ms = t;
+        c->timing_pending = false;
+    }
+    *ms = c->last_ms;
+    return MIMADV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// mimadv_internal.hpp -- shared host/device declarations of libmimadv.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "schemes_fp64.hpp"
+
+namespace mimadv {
+
+// Grid description held on the device side of a context.
+struct GridDev {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    int bcl[6];
+    ZMetrics zm;  // device pointers
+};
+
+// Epilogue of the scalar kernel.
+enum Epilogue : int {
+    EPI_ACCUM = 0,      // xs = ((xs - X) - Y) - Z
+    EPI_OVERWRITE = 1,  // xs = ((0 - X) - Y) - Z
+    EPI_HEUN = 2,       // a = ((0 - X) - Y) - Z; tend = a + b*tend; qn = q0 + adt*tend
+    EPI_SO = 3,         // a as above; qn = c0*q0 + c1*(q + dt*a)  (Shu-Osher stage)
+};
+
+struct ScalarArgs {
+    const double *q;  // stencil input
+    const double *u, *v, *w;
+    double *xs;        // ACCUM / OVERWRITE output
+    const double *q0;  // stage: state at the start of the step
+    double *tend;      // HEUN accumulator
+    double *qn;        // stage output (may alias q0, never q)
+    double b_lm, adt;  // HEUN coefficients
+    double c0, c1, dt; // SO coefficients
+    int adv;
+    int epi;
+    int dirmask;
+    int fill;  // write periodic ghost images of every interior value (stage epilogues)
+};
+
+// Launchers (scalar_fp64.hip, momentum_fp64.hip, util.hip).
+hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s);
+hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
+                                const double *mv, const double *mw, const double *ut, const double *vt,
+                                const double *wt, double *ua, double *va, double *wa, hipStream_t s);
+hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
+hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
+                             double *tend, double *x, hipStream_t s);
+hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out2, hipStream_t s);
+
+// Host-side PPM per-level coefficient table (12 doubles per level).
+std::vector<double> ppm_coefficients(int nz, const double *dz1w);
+
+}  // namespace mimadv

@@ -1,0 +1,262 @@
+// scalar_fp64.hip -- plain per-field FP64 scalar advection on gfx950.
+//
+// One launch computes the fused x+y+z flux divergence of one field
+// (advection_x/y/z of src/TOOLS/advection.f90:556-798 for all three
+// directions) and applies one of the epilogues of ScalarArgs: accumulate
+// into a tendency, overwrite it, or a full Heun / Shu-Osher stage update
+// with the periodic ghost images written in the same pass (so no separate
+// statebc/qnbc_p4 sweep is needed between stages).
+//
+// Work mapping: a workgroup of TX x TY threads owns a TX x TY tile of one
+// k-plane.  The plane tile plus a 3-cell halo in x and y is staged in LDS
+// with coalesced x-fastest loads; the x and y stencils read LDS, the z
+// stencil reads the thread's own column (coalesced across the wavefront,
+// which spans 64 consecutive x).  Tiles for neighbouring k of the same
+// (x,y) tile have linear block ids 256-apart-multiples, so they land on the
+// same XCD and share its L2 for the column reads.
+//
+// This file is compiled with -ffp-contract=off: the arithmetic follows the
+// reference operation order and matches the CPU restatement bit for bit.
+#include "mimadv_internal.hpp"
+
+namespace mimadv {
+
+__device__ __forceinline__ double weno5_left(double a, double b, double c, double d, double e)
+{
+    const double eps = 1e-6;
+    double q0 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
+    double q1 = (-b + 5.0 * c + 2.0 * d) / 6.0;
+    double q2 = (2.0 * c + 5.0 * d - e) / 6.0;
+    double b0 = 13.0 / 12.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) + 0.25 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
+    double b1 = 13.0 / 12.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 0.25 * (b - d) * (b - d);
+    double b2 = 13.0 / 12.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) + 0.25 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
+    double a0 = 0.1 / ((eps + b0) * (eps + b0));
+    double a1 = 0.6 / ((eps + b1) * (eps + b1));
+    double a2 = 0.3 / ((eps + b2) * (eps + b2));
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+template <class QF, class UF>
+__device__ __forceinline__ double weno5_hface(int f, QF Q, UF U)
+{
+    double uu = U(f + 1);
+    double fl = weno5_left(Q(f - 2), Q(f - 1), Q(f), Q(f + 1), Q(f + 2));
+    double fr = weno5_left(Q(f + 3), Q(f + 2), Q(f + 1), Q(f), Q(f - 1));
+    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
+}
+
+template <int SCHEME, class QF, class UF>
+__device__ __forceinline__ double h_contrib(int i, int n, double d, bool adv, int bc_hi, const SchemeParams &sp,
+                                            QF Q, UF U)
+{
+    double d1 = 1.0 / d;
+    if constexpr (SCHEME == 0) {
+        return weno3_h(i, d1, adv, Q, U);
+    } else if constexpr (SCHEME == 1) {
+        return ppm_h(i, n, d1, adv, bc_hi, sp, Q, U);
+    } else if constexpr (SCHEME == 2) {
+        return muscl_h(i, d, d1, adv, sp, Q, U);
+    } else {
+        double t = (weno5_hface(i, Q, U) - weno5_hface(i - 1, Q, U)) * d1;
+        if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
+        return t;
+    }
+}
+
+template <int SCHEME, class QF, class WF>
+__device__ __forceinline__ double z_contrib(int k, int nz, bool adv, const GridDev &g, const SchemeParams &sp,
+                                            QF Q, WF W)
+{
+    const ZMetrics &zm = g.zm;
+    int bc5 = g.bcl[4], bc6 = g.bcl[5];
+    double bc51 = (bc5 == 0) ? 1.0 : 0.0, bc61 = (bc6 == 0) ? 1.0 : 0.0;
+    if constexpr (SCHEME == 1) {
+        // cells k-1, k, k+1 (level nz carries only its Flux_l)
+        PPMRec cm{0.0, 0.0}, c0{0.0, 0.0}, cp{0.0, 0.0};
+        auto cell = [&](int c) -> PPMRec {
+            if (c == nz) {
+                PPMRec r;
+                r.fr = 0.0;
+                r.fl = ppm_ztop_fl(nz, zm, sp, Q, W);
+                return r;
+            }
+            return ppm_zcell(c, nz, zm, sp, bc51, bc61, Q, W);
+        };
+        bool need = (k >= 2 && k <= nz - 1) || (k == 1 && bc5 == 1) || (k == nz && bc6 == 1);
+        if (!need) return 0.0;
+        if (k >= 2) cm = cell(k - 1);
+        c0 = cell(k);
+        if (k <= nz - 1) cp = cell(k + 1);
+        auto face = [&](int f) -> double {
+            if (f == k - 1) return ppm_faceflux(cm.fr, c0.fl, W(k));
+            return ppm_faceflux(c0.fr, cp.fl, W(k + 1));
+        };
+        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
+    } else if constexpr (SCHEME == 2) {
+        auto face = [&](int f) -> double { return muscl_zface(f, nz, zm, sp, bc51, bc61, Q, W); };
+        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
+    } else {
+        auto face = [&](int f) -> double { return weno3_zface(f, nz, sp.weno_bottom, Q, W); };
+        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
+    }
+}
+
+template <int SCHEME, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void scalar_fp64_kernel(GridDev g, SchemeParams sp, ScalarArgs a)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    constexpr bool THREE_D = TY > 1;
+    constexpr int LW = TX + 6;
+    constexpr int LH = THREE_D ? TY + 6 : 1;
+    __shared__ double tile[LH][LW];
+
+    const int k = blockIdx.z + 1;
+    const int i0 = 4 + blockIdx.x * TX;
+    const int j0 = THREE_D ? 4 + blockIdx.y * TY : 1;
+    const size_t plane = (size_t)nx * ny;
+    const size_t kbase = (size_t)(k - 1) * plane;
+    const double *__restrict__ qk = a.q + kbase;
+    const int mx = nx - 5, my = ny - 5;  // interior extents (periodic images)
+
+    for (int t = threadIdx.y * TX + threadIdx.x; t < LH * LW; t += TX * TY) {
+        int r = t / LW, c = t - r * LW;
+        // positions outside the array are replaced by their periodic
+        // interior image; only WENO5 reads them (x(n+1)), other schemes'
+        // closures never do
+        int ii = i0 - 3 + c;
+        if (ii < 1 || ii > nx) {
+            int rr = (ii - 4) % mx;
+            ii = (rr < 0 ? rr + mx : rr) + 4;
+        }
+        int jj = 1;
+        if (THREE_D) {
+            jj = j0 - 3 + r;
+            if (jj < 1 || jj > ny) {
+                int rr = (jj - 4) % my;
+                jj = (rr < 0 ? rr + my : rr) + 4;
+            }
+        }
+        tile[r][c] = qk[(size_t)(ii - 1) + (size_t)nx * (jj - 1)];
+    }
+    __syncthreads();
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx, j = j0 + ty;
+    if (i > nx - 2 || (THREE_D && j > ny - 2)) return;
+    const size_t col = (size_t)(i - 1) + (size_t)nx * (j - 1);
+    const size_t id = col + kbase;
+    const bool adv = a.adv != 0;
+    const int lrow = THREE_D ? ty + 3 : 0;
+
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    if (a.dirmask & 1) {
+        const double *__restrict__ urow = a.u + kbase + (size_t)nx * (j - 1);
+        auto Q = [&](int ii) { return tile[lrow][ii - (i0 - 3)]; };
+        auto U = [&](int ii) { return urow[ii - 1]; };
+        X = h_contrib<SCHEME>(i, nx, g.dx, adv, g.bcl[1], sp, Q, U);
+    }
+    if (THREE_D && (a.dirmask & 2)) {
+        const double *__restrict__ vcol = a.v + kbase + (size_t)(i - 1);
+        auto Q = [&](int jj) { return tile[jj - (j0 - 3)][tx + 3]; };
+        auto V = [&](int jj) { return vcol[(size_t)nx * (jj - 1)]; };
+        Y = h_contrib<SCHEME>(j, ny, g.dy, adv, g.bcl[3], sp, Q, V);
+    }
+    if (a.dirmask & 4) {
+        const double *__restrict__ qc = a.q + col;
+        const double *__restrict__ wc = a.w + col;
+        auto Q = [&](int kk) { return qc[plane * (size_t)(kk - 1)]; };
+        auto W = [&](int kk) { return wc[plane * (size_t)(kk - 1)]; };
+        Z = z_contrib<SCHEME>(k, nz, adv, g, sp, Q, W);
+    }
+
+    if (a.epi == EPI_ACCUM || a.epi == EPI_OVERWRITE) {
+        double r = (a.epi == EPI_ACCUM) ? a.xs[id] : 0.0;
+        if (a.dirmask & 1) r = r - X;
+        if (THREE_D && (a.dirmask & 2)) r = r - Y;
+        if (a.dirmask & 4) r = r - Z;
+        a.xs[id] = r;
+        return;
+    }
+    double A = 0.0;
+    if (a.dirmask & 1) A = A - X;
+    if (THREE_D && (a.dirmask & 2)) A = A - Y;
+    if (a.dirmask & 4) A = A - Z;
+    double qn;
+    if (a.epi == EPI_HEUN) {
+        double T = (a.b_lm != 0.0) ? A + a.b_lm * a.tend[id] : A + 0.0;
+        a.tend[id] = T;
+        qn = a.q0[id] + a.adt * T;
+    } else {
+        qn = a.c0 * a.q0[id] + a.c1 * (a.q[id] + a.dt * A);
+    }
+    a.qn[id] = qn;
+    if (a.fill) {
+        // periodic images (qnbc_p4, bc.f90:409-414 and :431-436):
+        // x(1:3) = x(nx-4:nx-2), x(nx-1:nx) = x(4:5), same in y, corners
+        // from both
+        int ix = 0, jy = 0;
+        if (i >= nx - 4) ix = i - mx;
+        else if (i <= 5) ix = i + mx;
+        if (THREE_D) {
+            if (j >= ny - 4) jy = j - my;
+            else if (j <= 5) jy = j + my;
+        }
+        double *qk_out = a.qn + kbase;
+        if (ix) qk_out[(size_t)(ix - 1) + (size_t)nx * (j - 1)] = qn;
+        if (jy) {
+            qk_out[(size_t)(i - 1) + (size_t)nx * (jy - 1)] = qn;
+            if (ix) qk_out[(size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qn;
+        }
+    }
+}
+
+template <int SCHEME>
+static hipError_t launch_scheme(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s)
+{
+    int mxi = g.nx - 5;
+    if (g.ny > 1) {
+        constexpr int TX = 64, TY = 4;
+        dim3 grid((mxi + TX - 1) / TX, (g.ny - 5 + TY - 1) / TY, g.nz);
+        hipLaunchKernelGGL((scalar_fp64_kernel<SCHEME, TX, TY>), grid, dim3(TX, TY), 0, s, g, sp, a);
+    } else {
+        constexpr int TX = 64, TY = 1;
+        dim3 grid((mxi + TX - 1) / TX, 1, g.nz);
+        hipLaunchKernelGGL((scalar_fp64_kernel<SCHEME, TX, TY>), grid, dim3(TX, TY), 0, s, g, sp, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s)
+{
+    switch (sp.scheme) {
+    case 0: return launch_scheme<0>(g, sp, a, s);
+    case 1: return launch_scheme<1>(g, sp, a, s);
+    case 2: return launch_scheme<2>(g, sp, a, s);
+    case 3: return launch_scheme<3>(g, sp, a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+std::vector<double> ppm_coefficients(int nz, const double *dz1w)
+{
+    std::vector<double> c((size_t)nz * 12, 0.0);
+    for (int k = 3; k <= nz - 2; ++k) {
+        double delta = dz1w[k - 1], deltam1 = dz1w[k - 2], deltap1 = dz1w[k], deltap2 = dz1w[k + 1];
+        double *p = c.data() + 12 * (k - 1);
+        p[0] = delta / (deltam1 + delta + deltap1);
+        p[1] = (2.0 * deltam1 + delta) / (deltap1 + delta);
+        p[2] = (delta + 2.0 * deltap1) / (deltam1 + delta);
+        p[3] = deltap1 / (delta + deltap1 + deltap2);
+        p[4] = (2.0 * delta + deltap1) / (deltap2 + deltap1);
+        p[5] = (deltap1 + 2.0 * deltap2) / (delta + deltap1);
+        p[6] = delta / (delta + deltap1);
+        p[7] = 1.0 / (deltam1 + delta + deltap1 + deltap2);
+        p[8] = 2.0 * deltap1 * delta / (delta + deltap1) *
+               ((deltam1 + delta) / (2.0 * delta + deltap1) - (deltap2 + deltap1) / (2.0 * deltap1 + delta));
+        p[9] = delta * (deltam1 + delta) / (2.0 * delta + deltap1);
+        p[10] = deltap1 * (deltap1 + deltap2) / (2.0 * deltap1 + delta);
+    }
+    return c;
+}
+
+}  // namespace mimadv

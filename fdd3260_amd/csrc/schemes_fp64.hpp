@@ -1,0 +1,477 @@
+// schemes_fp64.hpp -- per-cell device formulas of the MIMICA advection
+// schemes, FP64, in the reference's floating-point operation order.
+//
+// The reference computes each direction as a sequential line sweep
+// (face fluxes into a line array, then differences).  Here every formula
+// is a pure function of a cell or face index and accessors Q(i), U(i)
+// (1-based line indices), so a thread can evaluate the contribution of its
+// own cell without the line sweep.  Position-dependent closures (tile
+// edges, bottom/top levels) are selected by index exactly where the
+// reference's loops switch formulas.  Built without FMA contraction these
+// reproduce the reference arithmetic bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mimadv {
+
+struct ZMetrics {
+    const double *fdz0, *dzw, *dz1w, *dz1;
+    const double *ppmc;  // [nz][12] PPM non-uniform coefficients per level
+    double dz;
+};
+
+struct SchemeParams {
+    int scheme;
+    int muscl_limiter;
+    int ppm_limiter;
+    int weno_bottom;
+    double lim_tol;
+    double dt0;
+};
+
+__device__ __forceinline__ double dmaxd(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmind(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double sgn1(double x) { return copysign(1.0, x); }
+
+// ---------------------------------------------------------------------------
+// Reference WENO (advq_weno.f90)
+// ---------------------------------------------------------------------------
+
+// weight{x,y,z} for one face (advq_weno.f90:329-350)
+__device__ __forceinline__ void w3_weights(double a, double b, double c, double d, double ww[4])
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15, thres = 20.0;
+    double eta1 = (b - a) * (b - a);
+    double eta2 = (c - b) * (c - b);
+    double eta3 = (d - c) * (d - c);
+    ww[0] = g1; ww[1] = g2; ww[2] = g2; ww[3] = g1;
+    if (dmaxd(eta1, eta2) / (dmind(eta1, eta2) + eps) > thres) {
+        double s1 = (eps + eta1) * (eps + eta1);
+        double s2 = (eps + eta2) * (eps + eta2);
+        double w1 = g1 / s1, w2 = g2 / s2;
+        double sumw = w1 + w2;
+        ww[0] = w1 / sumw;
+        ww[1] = w2 / sumw;
+    }
+    if (dmaxd(eta2, eta3) / (dmind(eta2, eta3) + eps) > thres) {
+        double s2 = (eps + eta2) * (eps + eta2);
+        double s3 = (eps + eta3) * (eps + eta3);
+        double w3 = g2 / s2, w4 = g1 / s3;
+        double sumw = w3 + w4;
+        ww[2] = w3 / sumw;
+        ww[3] = w4 / sumw;
+    }
+}
+
+// flux at face i+1/2 from a..d = x(i-1..i+2), uu = u(i+1) (advq_weno.f90:92-110)
+__device__ __forceinline__ double w3_face(double a, double b, double c, double d, double uu,
+                                          const double ww[4], bool limit)
+{
+    double ffim1 = -(0.5 * a) + 1.5 * b;
+    double ffip1p = 0.5 * b + 0.5 * c;
+    double ffip1m = ffip1p;
+    double ffip2 = 1.5 * c - 0.5 * d;
+    if (limit) {  // limiter_weno (advq_weno.f90:462-491)
+        const double eps = 1.e-15;
+        double minp = dmind(dmind(dmind(ffim1, ffip1m), ffip1p), ffip2);
+        double phi = fmin(fabs((0.0 - b) / (minp - b + eps)), 1.0);
+        ffim1 = phi * (ffim1 - b) + b;
+        ffip1m = phi * (ffip1m - b) + b;
+        ffip1p = phi * (ffip1p - b) + b;
+        ffip2 = phi * (ffip2 - b) + b;
+    }
+    double fl = ww[0] * ffim1 + ww[1] * ffip1p;
+    double fr = ww[2] * ffip1m + ww[3] * ffip2;
+    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
+}
+
+template <class QF, class UF>
+__device__ __forceinline__ double weno3_hface(int f, QF Q, UF U)
+{
+    double a = Q(f - 1), b = Q(f), c = Q(f + 1), d = Q(f + 2);
+    double ww[4];
+    w3_weights(a, b, c, d, ww);
+    return w3_face(a, b, c, d, U(f + 1), ww, true);
+}
+
+// x/y contribution of cell i (advsx_weno, advq_weno.f90:61-131)
+template <class QF, class UF>
+__device__ __forceinline__ double weno3_h(int i, double d1, bool adv, QF Q, UF U)
+{
+    double t = (weno3_hface(i, Q, U) - weno3_hface(i - 1, Q, U)) * d1;
+    if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
+    return t;
+}
+
+// z face flux k+1/2 (advsz_weno, advq_weno.f90:224-271)
+template <class QF, class WF>
+__device__ __forceinline__ double weno3_zface(int f, int nz, int bottom, QF Q, WF W)
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
+    double www = W(f + 1);
+    if (f == 1 && bottom == 1) {  // bottom closure kept
+        double ffip1p = 0.5 * Q(1) + 0.5 * Q(2);
+        double ffip2 = 1.5 * Q(2) - 0.5 * Q(3);
+        double fl = ffip1p;
+        double fr = g2 * ffip1p + g1 * ffip2;
+        return 0.5 * fl * (www + fabs(www)) + 0.5 * fr * (www - fabs(www));
+    }
+    if (f == nz - 1) {  // top closure, no limiter
+        double ffim1 = -(0.5 * Q(f - 1)) + 1.5 * Q(f);
+        double ffip1p = 0.5 * Q(f) + 0.5 * Q(f + 1);
+        double fl = g1 * ffim1 + g2 * ffip1p;
+        double fr = ffip1p;
+        return 0.5 * fl * (www + fabs(www)) + 0.5 * fr * (www - fabs(www));
+    }
+    // loop body; at f = 1 the reference reads x(0), taken as x(1)
+    double a = (f == 1) ? Q(1) : Q(f - 1);
+    double b = Q(f), c = Q(f + 1), d = Q(f + 2);
+    double ww[4] = {g1, g2, g2, g1};
+    if (f >= 3 && f <= nz - 3) w3_weights(a, b, c, d, ww);
+    return w3_face(a, b, c, d, www, ww, true);
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL (advq_muscl.f90)
+// ---------------------------------------------------------------------------
+
+// limiter_muscl (advq_muscl.f90:335-383)
+__device__ __forceinline__ void muscl_limiter(int lim, double lt, double xx, double ffip1, double ffip2,
+                                              double ffim1, double &phir, double &phil)
+{
+    double rp1 = 1.0, rp2 = 1.0;
+    if (fabs(ffim1) > lt * fabs(xx) && fabs(ffip1) < fabs(ffim1) / lt) rp1 = ffip1 / ffim1;
+    if (fabs(ffip2) > lt * fabs(xx) && fabs(ffip1) < fabs(ffip2) / lt) rp2 = ffip1 / ffip2;
+    switch (lim) {
+    case 2:  // VL
+        phil = (rp1 + fabs(rp1)) / (1.0 + fabs(rp1));
+        phir = (rp2 + fabs(rp2)) / (1.0 + fabs(rp2));
+        break;
+    case 1:  // MC
+        phil = dmaxd(0.0, dmind(dmind(2.0 * rp1, 0.5 * (rp1 + 1.0)), 2.0));
+        phir = dmaxd(0.0, dmind(dmind(2.0 * rp2, 0.5 * (rp2 + 1.0)), 2.0));
+        break;
+    case 0:  // minmod
+        phil = dmaxd(0.0, dmind(1.0, rp1));
+        phir = dmaxd(0.0, dmind(1.0, rp2));
+        break;
+    default:  // Fromm
+        phil = 0.5 * (rp1 + 1.0);
+        phir = 0.5 * (rp2 + 1.0);
+        break;
+    }
+}
+
+__device__ __forceinline__ double muscl_upwind(double uu, double fl, double fr)
+{
+    return 0.5 * uu * (fl * (1.0 + sgn1(uu)) + fr * (1.0 - sgn1(uu)));
+}
+
+// face flux i+1/2, x/y (advq_muscl.f90:81-100)
+template <class QF, class UF>
+__device__ __forceinline__ double muscl_hface(int i, double d, double d1, const SchemeParams &sp, QF Q, UF U)
+{
+    const double eps = 1.e-18;
+    double uu = U(i + 1), xx = Q(i);
+    double xp = Q(i + 1);
+    double slope_p = (xp - xx) * d1;
+    double slope_p2 = (Q(i + 2) - xp) * d1;
+    double slope_m = (xx - Q(i - 1)) * d1;
+    double lt = sp.lim_tol;
+    double fl, fr;
+    if (d * fabs(slope_m) > lt * fabs(xx) + eps || d * fabs(slope_p2) > lt * fabs(xx) + eps) {
+        double phir, phil;
+        muscl_limiter(sp.muscl_limiter, lt, xx, slope_p, slope_p2, slope_m, phir, phil);
+        fr = xp - 0.5 * phir * slope_p2 * d;
+        fl = xx + 0.5 * phil * slope_m * d;
+    } else {
+        fr = xp - 0.5 * slope_p * d;
+        fl = xx + 0.5 * slope_p * d;
+    }
+    return muscl_upwind(uu, fl, fr);
+}
+
+template <class QF, class UF>
+__device__ __forceinline__ double muscl_h(int i, double d, double d1, bool adv, const SchemeParams &sp, QF Q, UF U)
+{
+    double t = (muscl_hface(i, d, d1, sp, Q, U) - muscl_hface(i - 1, d, d1, sp, Q, U)) * d1;
+    if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
+    return t;
+}
+
+// z face flux k+1/2 (advsz_muscl, advq_muscl.f90:245-302)
+template <class QF, class WF>
+__device__ __forceinline__ double muscl_zface(int f, int nz, const ZMetrics &zm, const SchemeParams &sp,
+                                              double bc51, double bc61, QF Q, WF W)
+{
+    const double eps = 1.e-18;
+    double lt = sp.lim_tol, dz = zm.dz;
+    const double *dz1 = zm.dz1, *dz1w = zm.dz1w, *fdz0 = zm.fdz0;
+    double ww = W(f + 1);
+    double slope_p, slope_p2, slope_m, phir, phil, fl, fr;
+    double m = dz1w[f - 1];
+    if (f == 1) {
+        slope_p = (Q(2) - Q(1)) * dz1[0];
+        slope_p2 = (Q(3) - Q(2)) * dz1[1];
+        slope_m = bc51 * (Q(nz - 4) - Q(nz - 5)) * dz1[nz - 6] + (1.0 - bc51) * slope_p;
+        if (fdz0[1] * dz * fabs(slope_p2) > lt * fabs(Q(1)) + eps) {
+            muscl_limiter(sp.muscl_limiter, lt, Q(1), slope_p, slope_p2, slope_m, phir, phil);
+            fl = Q(1) + 0.5 * phil * slope_m / m;
+            fr = Q(2) - 0.5 * phir * slope_p2 / m;
+        } else {
+            fl = Q(1) + 0.5 * slope_p / m;
+            fr = Q(2) - 0.5 * slope_p / m;
+        }
+        return muscl_upwind(ww, fl, fr);
+    }
+    double xk = Q(f), xp = Q(f + 1);
+    slope_p = (xp - xk) * dz1[f - 1];
+    slope_m = (xk - Q(f - 1)) * dz1[f - 2];
+    bool limited;
+    if (f == nz - 1) {
+        slope_p2 = bc61 * (Q(6) - Q(5)) * dz1[4] + (1.0 - bc61) * slope_p;
+        limited = fdz0[f - 2] * dz * fabs(slope_m) > lt * fabs(xk) + eps;
+    } else {
+        slope_p2 = (Q(f + 2) - xp) * dz1[f];
+        limited = fdz0[f - 2] * dz * fabs(slope_m) > lt * fabs(xk) + eps ||
+                  fdz0[f] * dz * fabs(slope_p2) > lt * fabs(xk) + eps;
+    }
+    if (limited) {
+        muscl_limiter(sp.muscl_limiter, lt, xk, slope_p, slope_p2, slope_m, phir, phil);
+        fr = xp - 0.5 * phir * slope_p2 / m;
+        fl = xk + 0.5 * phil * slope_m / m;
+    } else {
+        fr = xp - 0.5 * slope_p / m;
+        fl = xk + 0.5 * slope_p / m;
+    }
+    return muscl_upwind(ww, fl, fr);
+}
+
+// ---------------------------------------------------------------------------
+// PPM (advq_ppm.f90)
+// ---------------------------------------------------------------------------
+
+// sqrt(spacing(x)) of Fortran for REAL(8): spacing = 2**max(ilogb(x)-52,-1022)
+// (2**-1022 for x == 0); the square root of a power of two is exact or
+// sqrt(2) times one.
+__device__ __forceinline__ double sqrt_spacing(double x)
+{
+    int e = (x == 0.0) ? -1022 : ilogb(x) - 52;
+    if (e < -1022) e = -1022;
+    if ((e & 1) == 0) return ldexp(1.0, e / 2);
+    return ldexp(1.4142135623730951, (e - 1) / 2);
+}
+
+struct PPMRec {
+    double fr, fl;
+};
+
+// limiter_ppm (advq_ppm.f90:449-530) followed by the CW84 face averages
+// Flux_r/Flux_l (advq_ppm.f90:93-94)
+__device__ __forceinline__ PPMRec ppm_cell(int use_lim, double xx, double xp, double xm, double xplus,
+                                           double xmin, double ffip1, double ffip2, double ffim1,
+                                           double ffim2, double mup, double muc)
+{
+    const double eps = 1.e-15, thres = 10.0;
+    double rp1 = 1.0;
+    double phir = xplus, phil = xmin;
+    double g1a = ffip1 - ffim1, g1b = ffip1 + ffim1;
+    double g2a = ffip2 - ffip1, g2b = ffip2 + ffip1;
+    double g3a = ffim1 - ffim2, g3b = ffim1 + ffim2;
+    double gamma1 = 0.5 * (g1a * g1a + g1b * g1b);
+    double gamma2 = 0.5 * (g2a * g2a + g2b * g2b);
+    double gamma3 = 0.5 * (g3a * g3a + g3b * g3b);
+    double lambda = dmaxd(dmaxd(gamma1, gamma2), gamma3) / (dmind(dmind(gamma1, gamma2), gamma3) + eps);
+    double phi1 = phir - phil;
+    double phi2 = 3.0 * (phil - 2.0 * xx + phir);
+    if (use_lim && lambda > thres) {
+        if (phir < 0.0 || ((phir - xx) * (xp - phir) < 0.0 &&
+                           (ffim1 * ffip2 >= 0.0 || ffim1 * ffip1 <= 0.0 || ffip2 * ffip1 <= 0.0))) {
+            phir = (fabs(phir - xx) < fabs(phir - xp)) ? xx : xp;
+        }
+        if (phil < 0.0 || ((xx - phil) * (phil - xm) < 0.0 &&
+                           (ffim2 * ffip1 >= 0.0 || ffim2 * ffim1 <= 0.0 || ffip1 * ffim1 <= 0.0))) {
+            phil = (fabs(phil - xx) < fabs(phil - xm)) ? xx : xm;
+        }
+        phi1 = phir - phil;
+        phi2 = 3.0 * (phil - 2.0 * xx + phir);
+        double slope = -0.5 * phi1 / (sgn1(phi2) * dmaxd(fabs(phi2), eps));
+        if (fabs(ffim1) > sqrt_spacing(ffim1)) rp1 = ffip1 / ffim1;
+        // (slope > -1/2 .or. slope < 1/2) holds for every non-NaN slope
+        if (fabs(phi2) > eps && (slope > -0.5 || slope < 0.5)) {
+            double lim = dmaxd(0.0, dmind(dmind(2.0, 0.5 * (rp1 + 1.0)), 2.0 * rp1));
+            phir = xx + 0.5 * ffim1 * lim;
+            phil = xx - 0.5 * ffim1 * lim;
+            if ((xx - phil) * (phir - xx) < 0.0) {
+                phil = xx;
+                phir = xx;
+            }
+            phi1 = phir - phil;
+            phi2 = 0.0;
+        }
+    }
+    PPMRec r;
+    r.fr = phir - 0.5 * mup * (phi1 + (1.0 - (2.0 / 3.0) * mup) * phi2);
+    r.fl = phil + 0.5 * muc * (phi1 - (1.0 - (2.0 / 3.0) * muc) * phi2);
+    return r;
+}
+
+// right-face value xplus of cell c on an x/y line (left boundary c = 3,
+// core c = 4..n-2), advq_ppm.f90:89,110
+template <class QF>
+__device__ __forceinline__ double ppm_hxplus(int c, QF Q)
+{
+    if (c == 3) return Q(3) + 0.5 * (Q(4) - Q(3));
+    double ffip1 = Q(c + 1) - Q(c), ffip2 = Q(c + 2) - Q(c + 1), ffim1 = Q(c) - Q(c - 1);
+    double sl = 0.5 * (ffip1 + ffim1);
+    double slp1 = 0.5 * (ffip2 + ffip1);
+    return Q(c) + 0.5 * ffip1 - (1.0 / 6.0) * (slp1 - sl);
+}
+
+// Flux_r/Flux_l of cell c in 3..n-1 on an x/y line (advq_ppm.f90:79-134)
+template <class QF, class UF>
+__device__ __forceinline__ PPMRec ppm_hcell(int c, int n, double d1, const SchemeParams &sp, QF Q, UF U)
+{
+    double dt0 = sp.dt0;
+    double mup = fabs(U(c + 1)) * dt0 * d1;
+    double muc = fabs(U(c)) * dt0 * d1;
+    double xx = Q(c), xp = Q(c + 1), xm = Q(c - 1);
+    if (c == 3) {
+        double ffip1 = xp - xx, ffip2 = Q(5) - xp, ffim1 = xx - xm;
+        double xmin = 0.5 * (xp + xx);
+        double xplus = xx + 0.5 * ffip1;
+        return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip2, ffim1, ffim1, mup, muc);
+    }
+    double xmin = ppm_hxplus(c - 1, Q);
+    if (c == n - 1) {
+        double ffip1 = xp - xx, ffim1 = xx - xm, ffim2 = xm - Q(c - 2);
+        double xplus = xx + 0.5 * ffip1;
+        return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip1, ffim1, ffim2, mup, muc);
+    }
+    double ffip1 = xp - xx, ffip2 = Q(c + 2) - xp, ffim1 = xx - xm, ffim2 = xm - Q(c - 2);
+    double sl = 0.5 * (ffip1 + ffim1);
+    double slp1 = 0.5 * (ffip2 + ffip1);
+    double xplus = xx + 0.5 * ffip1 - (1.0 / 6.0) * (slp1 - sl);
+    return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip2, ffim1, ffim2, mup, muc);
+}
+
+__device__ __forceinline__ double ppm_faceflux(double fr, double fl, double uf)
+{
+    return 0.5 * fr * (uf + fabs(uf)) + 0.5 * fl * (uf - fabs(uf));
+}
+
+// x/y contribution of cell i in 4..n-2 (advq_ppm.f90:136-163); bc_hi is the
+// east/north boundary code (non-periodic edge fix-up :143-146)
+template <class QF, class UF>
+__device__ __forceinline__ double ppm_h(int i, int n, double d1, bool adv, int bc_hi, const SchemeParams &sp,
+                                        QF Q, UF U)
+{
+    PPMRec cm = ppm_hcell(i - 1, n, d1, sp, Q, U);
+    PPMRec c0 = ppm_hcell(i, n, d1, sp, Q, U);
+    PPMRec cp = ppm_hcell(i + 1, n, d1, sp, Q, U);
+    double fr_i = c0.fr, fl_ip = cp.fl;
+    if (bc_hi != 0 && i == n - 2) {
+        fr_i = c0.fl;      // Flux_r(ii2) = Flux_l(ii2)
+        fl_ip = cm.fr;     // Flux_l(ii2+1) = Flux_r(ii2-1)
+    }
+    double flux_old = ppm_faceflux(cm.fr, c0.fl, U(i));
+    double flux = ppm_faceflux(fr_i, fl_ip, U(i + 1));
+    double t = (flux - flux_old) * d1;
+    if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
+    return t;
+}
+
+// Per-level PPM coefficients (metric-only subexpressions of the non-uniform
+// face formula, advq_ppm.f90:379-383, evaluated in the same order):
+//  [0] delta/(dm1+delta+dp1)   [1] (2dm1+delta)/(dp1+delta)  [2] (delta+2dp1)/(dm1+delta)
+//  [3] dp1/(delta+dp1+dp2)     [4] (2delta+dp1)/(dp2+dp1)    [5] (dp1+2dp2)/(delta+dp1)
+//  [6] delta/(delta+dp1)       [7] 1/(dm1+delta+dp1+dp2)
+//  [8] 2dp1*delta/(delta+dp1)*((dm1+delta)/(2delta+dp1)-(dp2+dp1)/(2dp1+delta))
+//  [9] delta*(dm1+delta)/(2delta+dp1)   [10] dp1*(dp1+dp2)/(2dp1+delta)
+template <class QF>
+__device__ __forceinline__ double ppm_zxplus(int c, const ZMetrics &zm, QF Q)
+{
+    if (c <= 2) return Q(c) + 0.5 * (Q(c + 1) - Q(c));
+    const double *p = zm.ppmc + 12 * (c - 1);
+    double ffip1 = Q(c + 1) - Q(c), ffip2 = Q(c + 2) - Q(c + 1), ffim1 = Q(c) - Q(c - 1);
+    double sl = p[0] * (p[1] * ffip1 + p[2] * ffim1);
+    double slp1 = p[3] * (p[4] * ffip2 + p[5] * ffip1);
+    return Q(c) + p[6] * ffip1 + p[7] * (p[8] * ffip1 - p[9] * slp1 + p[10] * sl);
+}
+
+// Flux_r/Flux_l of level c in 1..nz-1, advsz_ppm (advq_ppm.f90:326-409)
+template <class QF, class WF>
+__device__ __forceinline__ PPMRec ppm_zcell(int c, int nz, const ZMetrics &zm, const SchemeParams &sp,
+                                            double bc51, double bc61, QF Q, WF W)
+{
+    double dt0 = sp.dt0;
+    double dz1c = zm.dz1[c - 1];
+    double mup = fabs(W(c + 1)) * dt0 * dz1c;
+    double muc = fabs(W(c)) * dt0 * dz1c;
+    double xx = Q(c), xp = Q(c + 1);
+    double ffip1 = xp - xx;
+    if (c == 1) {
+        double ffip2 = Q(3) - xp;
+        double ffim1 = bc51 * (Q(nz - 4) - Q(nz - 5));
+        double ffim2 = bc51 * (Q(nz - 5) - Q(nz - 6));
+        double xmin = xx - 0.5 * ffip1;
+        double xplus = xx + 0.5 * ffip1;
+        return ppm_cell(sp.ppm_limiter, xx, xp, xp, xplus, xmin, ffip1, ffip2, ffim1, ffim2, mup, muc);
+    }
+    double xm = Q(c - 1);
+    double xmin = ppm_zxplus(c - 1, zm, Q);
+    if (c == 2) {
+        double ffip2 = Q(4) - xp;
+        double ffim1 = xx - xm;
+        double ffim2 = bc51 * (Q(nz - 4) - Q(nz - 5));
+        double xplus = xx + 0.5 * ffip1;
+        return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip2, ffim1, ffim2, mup, muc);
+    }
+    double ffim1 = xx - xm, ffim2 = xm - Q(c - 2);
+    if (c == nz - 1) {
+        double ffip2 = bc61 * (Q(6) - Q(5)) + (1.0 - bc61) * ffip1;
+        double xplus = xx + 0.5 * ffip1;
+        return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip2, ffim1, ffim2, mup, muc);
+    }
+    double ffip2 = Q(c + 2) - xp;
+    double xplus = ppm_zxplus(c, zm, Q);
+    return ppm_cell(sp.ppm_limiter, xx, xp, xm, xplus, xmin, ffip1, ffip2, ffim1, ffim2, mup, muc);
+}
+
+// Flux_l of the top level (advq_ppm.f90:411-414)
+template <class QF, class WF>
+__device__ __forceinline__ double ppm_ztop_fl(int nz, const ZMetrics &zm, const SchemeParams &sp, QF Q, WF W)
+{
+    double muc = fabs(W(nz)) * sp.dt0 * zm.dz1[nz - 1];
+    double ffim1 = Q(nz) - Q(nz - 1);
+    return Q(nz) + 0.5 * ffim1 * (1.0 - muc);
+}
+
+// ---------------------------------------------------------------------------
+// z contributions: the tendency loop of advs z_* (face differences, adv
+// term, nnf closures), same for all three schemes
+// ---------------------------------------------------------------------------
+
+template <class FF, class QF, class WF>
+__device__ __forceinline__ double z_assemble(int k, int nz, bool adv, int bc5, int bc6, const ZMetrics &zm,
+                                             FF face, QF Q, WF W)
+{
+    const double *dz1w = zm.dz1w;
+    if (k >= 2 && k <= nz - 1) {
+        double t = (face(k) - face(k - 1)) * dz1w[k - 1];
+        if (adv) t = t - Q(k) * (W(k + 1) - W(k)) * dz1w[k - 1];
+        return t;
+    }
+    double t = 0.0;
+    if (k == 1 && bc5 == 1) {
+        t = t + (face(1) - 0.0) * dz1w[0];
+        if (adv) t = t - Q(1) * (W(2) - 0.0) * dz1w[0];
+    }
+    if (k == nz && bc6 == 1) {
+        t = t + (0.0 - face(nz - 1)) * dz1w[nz - 1];
+        if (adv) t = t - Q(nz) * (0.0 - W(nz)) * dz1w[nz - 1];
+    }
+    return t;
+}
+
+}  // namespace mimadv

@@ -1,0 +1,187 @@
+// util.hip -- ghost fill, stage combine and interior reductions.
+#include "mimadv_internal.hpp"
+
+namespace mimadv {
+
+namespace {
+
+// qnbc_p4 (bc.f90:378-503), serial branch.  The reference runs the x copy,
+// then the y copy over full rows, then the z copy, then the mirror fills;
+// each pass is one launch here so later passes see earlier results.
+__global__ void fill_x_kernel(int nx, int ny, int nz, int bc_w, int bc_e, double *x)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (j > ny) return;
+    double *r = x + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k - 1));
+    auto X = [&](int i) -> double & { return r[i - 1]; };
+    if (bc_w == 0 || bc_e == 0) {
+        X(1) = X(nx - 4);
+        X(2) = X(nx - 3);
+        X(3) = X(nx - 2);
+        X(nx) = X(5);
+        X(nx - 1) = X(4);
+    }
+}
+
+__global__ void fill_y_kernel(int nx, int ny, int nz, double *x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (i > nx) return;
+    size_t sy = (size_t)nx;
+    double *c = x + (size_t)(i - 1) + (size_t)nx * ny * (k - 1);
+    auto X = [&](int j) -> double & { return c[sy * (j - 1)]; };
+    X(1) = X(ny - 4);
+    X(2) = X(ny - 3);
+    X(3) = X(ny - 2);
+    X(ny) = X(5);
+    X(ny - 1) = X(4);
+}
+
+__global__ void fill_z_kernel(int nx, int ny, int nz, double *x)
+{
+    size_t n = (size_t)nx * ny;
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    auto X = [&](int k) -> double & { return x[p + n * (k - 1)]; };
+    X(1) = X(nz - 4);
+    X(2) = X(nz - 3);
+    X(3) = X(nz - 2);
+    X(nz) = X(5);
+    X(nz - 1) = X(4);
+}
+
+// no-flux / open mirror fills (bc.f90:467-497)
+__global__ void mirror_x_kernel(int nx, int ny, int nz, int bc_w, int bc_e, double *x)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (j > ny) return;
+    double *r = x + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (k - 1));
+    if (bc_w == 1 || bc_w == 2) {
+        r[1] = r[3];
+        r[0] = r[4];
+    }
+    if (bc_e == 1 || bc_e == 2) {
+        r[nx - 1] = r[nx - 5];
+        r[nx - 2] = r[nx - 4];
+    }
+}
+
+__global__ void mirror_y_kernel(int nx, int ny, int nz, int bc_s, int bc_n, double *x)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    int k = blockIdx.y + 1;
+    if (i > nx) return;
+    size_t sy = (size_t)nx;
+    double *c = x + (size_t)(i - 1) + (size_t)nx * ny * (k - 1);
+    if (bc_s == 1 || bc_s == 2) {
+        c[sy * 1] = c[sy * 3];
+        c[0] = c[sy * 4];
+    }
+    if (bc_n == 1 || bc_n == 2) {
+        c[sy * (ny - 1)] = c[sy * (ny - 5)];
+        c[sy * (ny - 2)] = c[sy * (ny - 4)];
+    }
+}
+
+__global__ void integ_kernel(long n, int sorder, double a_lm, double b_lm, double dt, const double *__restrict__ xold,
+                             const double *__restrict__ a, double *__restrict__ tend, double *__restrict__ x)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (sorder > 1) {
+        double t = a[i] + b_lm * tend[i];
+        tend[i] = t;
+        x[i] = xold[i] + a_lm * dt * t;
+    } else {
+        x[i] = xold[i] + dt * a[i];
+    }
+}
+
+// Interior sums per level: plain and dzw-weighted, tree reduction within a
+// block (deterministic), one block per level.
+__global__ __launch_bounds__(256) void level_sums_kernel(int nx, int ny, int nz, const double *__restrict__ x,
+                                                         double *__restrict__ part)
+{
+    const bool three_d = ny > 1;
+    int k = blockIdx.x + 1;
+    int j0 = three_d ? 4 : 1, j1 = three_d ? ny - 2 : 1;
+    int mx = nx - 5, my = j1 - j0 + 1;
+    long cnt = (long)mx * my;
+    const double *p = x + (size_t)nx * ny * (k - 1);
+    double s = 0.0;
+    for (long t = threadIdx.x; t < cnt; t += blockDim.x) {
+        int jj = (int)(t / mx), ii = (int)(t - (long)jj * mx);
+        s += p[(size_t)(ii + 3) + (size_t)nx * (jj + j0 - 1)];
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k - 1] = sh[0];
+}
+
+__global__ void finish_sums_kernel(int nz, long ncells, const double *__restrict__ part, const double *__restrict__ dzw,
+                                   double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0, sw = 0.0;
+    for (int k = 0; k < nz; ++k) {
+        s += part[k];
+        sw += dzw[k] * part[k];
+    }
+    out[0] = s / (double)ncells;
+    out[1] = sw;
+}
+
+}  // namespace
+
+hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s)
+{
+    int nx = g.nx, ny = g.ny, nz = g.nz;
+    bool three_d = ny > 1;
+    if (g.bcl[0] == 0 || g.bcl[1] == 0)
+        hipLaunchKernelGGL(fill_x_kernel, dim3((ny + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, g.bcl[0], g.bcl[1], x);
+    if (three_d && (g.bcl[2] == 0 || g.bcl[3] == 0))
+        hipLaunchKernelGGL(fill_y_kernel, dim3((nx + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, x);
+    if (g.bcl[4] == 0 || g.bcl[5] == 0) {
+        size_t n = (size_t)nx * ny;
+        hipLaunchKernelGGL(fill_z_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, nz, x);
+    }
+    if (g.bcl[0] != 0 || g.bcl[1] != 0)
+        hipLaunchKernelGGL(mirror_x_kernel, dim3((ny + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, g.bcl[0], g.bcl[1], x);
+    if (three_d && (g.bcl[2] != 0 || g.bcl[3] != 0))
+        hipLaunchKernelGGL(mirror_y_kernel, dim3((nx + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, g.bcl[2], g.bcl[3], x);
+    return hipGetLastError();
+}
+
+hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
+                             double *tend, double *x, hipStream_t s)
+{
+    double a_lm = 1.0, b_lm = 0.0;
+    if (sorder > 1 && im != 1) {
+        a_lm = 0.5;
+        b_lm = 1.0;
+    }
+    unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(integ_kernel, dim3(nb), dim3(256), 0, s, n, sorder, a_lm, b_lm, dt, xold, a, tend, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out, hipStream_t s)
+{
+    // d_out: [0] mean, [1] dzw-weighted sum, [2..] per-level partials (nz)
+    double *part = d_out + 2;
+    hipLaunchKernelGGL(level_sums_kernel, dim3(g.nz), dim3(256), 0, s, g.nx, g.ny, g.nz, x, part);
+    long my = g.ny > 1 ? g.ny - 5 : 1;
+    long ncells = (long)(g.nx - 5) * my * g.nz;
+    hipLaunchKernelGGL(finish_sums_kernel, dim3(1), dim3(64), 0, s, g.nz, ncells, part, g.zm.dzw, d_out);
+    return hipGetLastError();
+}
+
+}  // namespace mimadv

@@ -1,0 +1,107 @@
+/*
+ * mimadv_oracle.h -- CPU restatement of MIMICA's advection operator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP library is
+ * compared against (tests, the smoke check and the CPU-baseline leg of the
+ * benchmark).  Nothing in the product path (fdd3260_amd/) links, loads or
+ * calls it.
+ *
+ * It restates, in plain C with the reference's loop structure and operation
+ * order, the Fortran routines of the reference model:
+ *   src/TOOLS/advq_weno.f90   advs{x,y,z}_weno, weight{x,y,z}, limiter_weno
+ *   src/TOOLS/advq_ppm.f90    advs{x,y,z}_ppm, limiter_ppm
+ *   src/TOOLS/advq_muscl.f90  advs{x,y,z}_muscl, limiter_muscl
+ *   src/TOOLS/advection.f90   advection_{x,y,z} (:556-798), advection_ac (:801-869)
+ *   src/INTEG/windadv.f90     qnadv_uvw and adv{u,v,w}{x,y,z}_fd
+ *   src/TOOLS/bc.f90          qnbc_p4 (:378-503), lbcu_p4 (:81-144)
+ *   src/INTEG/integpack.f90 + INCLUDE/integ.h   Heun / Euler stage combine
+ *   src/TOOLS/averages.f90    totalav (:507-551)
+ * plus two schemes the reference does not have (WENO5-JS reconstruction and
+ * SSP-RK3 stage coefficients), which are pinned only by closed forms.
+ *
+ * Arrays are Fortran column-major with x fastest: element (i,j,k), 1-based,
+ * lives at [(i-1) + nx*((j-1) + ny*(k-1))].  nx = MAXX+5, ny = MAXY+5 (or 1
+ * for a 2D run), nz = MAXZ; interior columns are i = 4..nx-2, j = 4..ny-2.
+ */
+#ifndef MIMADV_ORACLE_H
+#define MIMADV_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_BC_PER = 0, ORC_BC_NNF = 1, ORC_BC_OPE = 2 };
+enum { ORC_WENO = 0, ORC_PPM = 1, ORC_MUSCL = 2, ORC_WENO5 = 3 };
+enum { ORC_LIM_MINMOD = 0, ORC_LIM_MC = 1, ORC_LIM_VL = 2, ORC_LIM_FROMM = 3 };
+/* weno_bottom: what advsz_weno reads for x(0) in its first loop pass. */
+enum { ORC_WENO_BOTTOM_X0_IS_X1 = 0, ORC_WENO_BOTTOM_CLOSURE = 1 };
+
+typedef struct {
+    int nx, ny, nz;          /* full extents including ghosts */
+    double dx, dy, dz;
+    const double *fdz0;      /* [nz] cell size fraction */
+    const double *dzw;       /* [nz] dz*fdz0 */
+    const double *dz1w;      /* [nz] 1/dzw */
+    const double *dz1;       /* [nz] 1/(centre distance k..k+1) */
+    int bcl[6];              /* ORC_BC_* for west, east, south, north, bottom, top */
+} orc_grid;
+
+typedef struct {
+    int scheme;              /* ORC_WENO / ORC_PPM / ORC_MUSCL / ORC_WENO5 */
+    int muscl_limiter;       /* ORC_LIM_* (reference default: minmod) */
+    int ppm_limiter;         /* 1: ppmlimiter='PPM' (reference default), 0: none */
+    double lim_tol;          /* reference default 1e-9 */
+    double dt0;              /* full step, used by PPM Courant numbers */
+    int weno_bottom;         /* ORC_WENO_BOTTOM_* */
+    int mom_ord;             /* 2 or 4 */
+    int lavisc;              /* artificial viscosity on momentum */
+} orc_opts;
+
+/* ---- 1D line kernels (pointers to element 1 of the line, stride 1) ---- */
+void orc_advs_h(const orc_opts *o, int n, double d, int bc_lo, int bc_hi,
+                const double *x, const double *u, double *xa, int adv);
+/* x0 is the value read for x(0) (reference out-of-bounds read, see
+ * advq_weno.f90:245-247); ignored by PPM/MUSCL. */
+void orc_advs_z(const orc_grid *g, const orc_opts *o, const double *x, double x0,
+                const double *w, double *xa, int adv);
+
+/* ---- 3D operator ---- */
+/* xs = xs - div_dir(q) for dir = 0 (x), 1 (y), 2 (z); like advection_{x,y,z}. */
+void orc_advection_dir(const orc_grid *g, const orc_opts *o, int dir,
+                       const double *q, const double *vel, double *xs, int adv);
+/* xs = ((xs - X) - Y) - Z over the directions in dirmask (bit0 x, bit1 y,
+ * bit2 z); y is skipped when ny == 1 (2D model). */
+void orc_scalar_tendency(const orc_grid *g, const orc_opts *o, const double *q,
+                         const double *u, const double *v, const double *w,
+                         double *xs, int adv, int dirmask);
+
+/* qnadv_uvw: ua/va/wa are OVERWRITTEN with +div(rho0 u u) (windadv.f90:46-79).
+ * mu/mv/mw: momentum (rho0 u); ut/vt/wt: advecting velocities. */
+void orc_qnadv_uvw(const orc_grid *g, const orc_opts *o, int ntime,
+                   const double *mu, const double *mv, const double *mw,
+                   const double *ut, const double *vt, const double *wt,
+                   double *ua, double *va, double *wa);
+
+/* Ghost fill for one scalar (qnbc_p4, serial branch). */
+void orc_fill_scalar(const orc_grid *g, double *x);
+/* Ghost fill for winds, periodic lateral part of lbcu_p4 (serial branch). */
+void orc_fill_winds(const orc_grid *g, double *u, double *v, double *w);
+
+/* Stage combine of integ_scal for one field:
+ *   sorder > 1:  tend = a + b_lm*tend ; x = xold + a_lm*dt*tend
+ *   sorder == 1: x = xold + dt*a
+ * with (a_lm, b_lm) from integ.h for stage im (1-based). n elements. */
+void orc_integ_scal(long n, int sorder, int im, double dt, const double *xold,
+                    const double *a, double *tend, double *x);
+
+/* totalav: unweighted mean over interior columns, all levels. */
+double orc_totalav(const orc_grid *g, const double *x);
+
+/* OpenMP thread count used by the 3D drivers (1 = serial).  Returns the
+ * previous value. */
+int orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

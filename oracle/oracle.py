@@ -1,0 +1,184 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the CPU-baseline leg of bench.py, always as the checker.  The product path
+(fdd3260_amd/) never imports it.
+
+Arrays are float64 numpy arrays of shape (nz, ny, nx) (the reference's
+Fortran (nx, ny, nz) layout); the grid object is fdd3260_amd.grid.Grid or
+anything with the same attributes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+BC = {"per": 0, "nnf": 1, "ope": 2}
+SCHEMES = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
+LIMITERS = {"minmod": 0, "MC": 1, "VL": 2, "Fromm": 3}
+
+
+class OGrid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("fdz0", C.POINTER(C.c_double)), ("dzw", C.POINTER(C.c_double)),
+        ("dz1w", C.POINTER(C.c_double)), ("dz1", C.POINTER(C.c_double)),
+        ("bcl", C.c_int * 6),
+    ]
+
+
+class OOpts(C.Structure):
+    _fields_ = [
+        ("scheme", C.c_int), ("muscl_limiter", C.c_int), ("ppm_limiter", C.c_int),
+        ("lim_tol", C.c_double), ("dt0", C.c_double), ("weno_bottom", C.c_int),
+        ("mom_ord", C.c_int), ("lavisc", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, i, d, dp = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)
+    G, O = C.POINTER(OGrid), C.POINTER(OOpts)
+    lib.orc_advs_h.argtypes = [O, i, d, i, i, dp, dp, dp, i]
+    lib.orc_advs_z.argtypes = [G, O, dp, d, dp, dp, i]
+    lib.orc_advection_dir.argtypes = [G, O, i, dp, dp, dp, i]
+    lib.orc_scalar_tendency.argtypes = [G, O, dp, dp, dp, dp, dp, i, i]
+    lib.orc_qnadv_uvw.argtypes = [G, O, i, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    lib.orc_fill_scalar.argtypes = [G, dp]
+    lib.orc_fill_winds.argtypes = [G, dp, dp, dp]
+    lib.orc_integ_scal.argtypes = [C.c_long, i, i, d, dp, dp, dp, dp]
+    lib.orc_totalav.argtypes = [G, dp]
+    lib.orc_totalav.restype = d
+    lib.orc_set_threads.argtypes = [i]
+    lib.orc_set_threads.restype = i
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Oracle:
+    """Oracle bound to one grid and option set (same option names as
+    fdd3260_amd.advection.Advection)."""
+
+    def __init__(self, grid, scal_adv="weno", muscllimiter="minmod", ppmlimiter="PPM", lim_tol=1.0e-9,
+                 mom_ord=4, lavisc=True, weno_bottom=0, dt0=0.0):
+        self.lib = load()
+        self.grid = grid
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (grid.fdz, grid.dzw, grid.dz1w, grid.dz1)]
+        g = OGrid()
+        g.nx, g.ny, g.nz = grid.nx, grid.ny, grid.nz
+        g.dx, g.dy, g.dz = grid.dx, grid.dy, grid.dz
+        g.fdz0, g.dzw, g.dz1w, g.dz1 = (_dp(x) for x in self._keep)
+        for n, b in enumerate(grid.bcl):
+            g.bcl[n] = BC[b]
+        self.g = g
+        o = OOpts()
+        o.scheme = SCHEMES[scal_adv]
+        o.muscl_limiter = LIMITERS[muscllimiter]
+        o.ppm_limiter = 1 if ppmlimiter == "PPM" else 0
+        o.lim_tol = lim_tol
+        o.dt0 = dt0
+        o.weno_bottom = weno_bottom
+        o.mom_ord = mom_ord
+        o.lavisc = 1 if lavisc else 0
+        self.o = o
+
+    def set_threads(self, n):
+        return self.lib.orc_set_threads(int(n))
+
+    def scalar_tendency(self, dt0, u, v, w, q, xs, adv=False, dirmask=7):
+        self.o.dt0 = dt0
+        self.lib.orc_scalar_tendency(C.byref(self.g), C.byref(self.o), _dp(q), _dp(u), _dp(v), _dp(w), _dp(xs),
+                                     1 if adv else 0, int(dirmask))
+
+    def advection_dir(self, dt0, direction, q, vel, xs, adv=False):
+        self.o.dt0 = dt0
+        self.lib.orc_advection_dir(C.byref(self.g), C.byref(self.o), int(direction), _dp(q), _dp(vel), _dp(xs),
+                                   1 if adv else 0)
+
+    def line_h(self, dt0, x, u, d, bc_lo="per", bc_hi="per", adv=False):
+        """One x/y line kernel; returns the line tendency xa (starting at 0)."""
+        self.o.dt0 = dt0
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        xa = np.zeros_like(x)
+        self.lib.orc_advs_h(C.byref(self.o), x.size, float(d), BC[bc_lo], BC[bc_hi], _dp(x), _dp(u), _dp(xa),
+                            1 if adv else 0)
+        return xa
+
+    def line_z(self, dt0, x, w, adv=False, x0=None):
+        self.o.dt0 = dt0
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        xa = np.zeros_like(x)
+        self.lib.orc_advs_z(C.byref(self.g), C.byref(self.o), _dp(x), float(x[0] if x0 is None else x0), _dp(w),
+                            _dp(xa), 1 if adv else 0)
+        return xa
+
+    def qnadv_uvw(self, ntime, mu, mv, mw, ut, vt, wt):
+        ua, va, wa = (np.zeros_like(mu) for _ in range(3))
+        self.lib.orc_qnadv_uvw(C.byref(self.g), C.byref(self.o), int(ntime), _dp(mu), _dp(mv), _dp(mw), _dp(ut),
+                               _dp(vt), _dp(wt), _dp(ua), _dp(va), _dp(wa))
+        return ua, va, wa
+
+    def fill_scalar(self, x):
+        self.lib.orc_fill_scalar(C.byref(self.g), _dp(x))
+
+    def fill_winds(self, u, v, w):
+        self.lib.orc_fill_winds(C.byref(self.g), _dp(u), _dp(v), _dp(w))
+
+    def integ_scal(self, sorder, im, dt, xold, a, tend, x):
+        self.lib.orc_integ_scal(xold.size, int(sorder), int(im), float(dt), _dp(xold), _dp(a), _dp(tend), _dp(x))
+
+    def totalav(self, x):
+        return self.lib.orc_totalav(C.byref(self.g), _dp(x))
+
+    # -- the advection-only dynamics loop used as a reference trajectory --
+    def heun_step(self, dt, u, v, w, q, adv=False):
+        """One Heun step (integ.h, sorder=2) of one field, ghosts refreshed
+        after every stage (statebc); q updated in place."""
+        q0 = q.copy()
+        tend = np.zeros_like(q)
+        for im in (1, 2):
+            a = np.zeros_like(q)
+            self.scalar_tendency(dt, u, v, w, q, a, adv)
+            self.integ_scal(2, im, dt, q0, a, tend, q)
+            self.fill_scalar(q)
+
+    def ssprk3_step(self, dt, u, v, w, q, adv=False):
+        """Shu-Osher SSP-RK3 step: q1 = q + dt L(q); q2 = 3/4 q + 1/4 (q1 +
+        dt L(q1)); q3 = 1/3 q + 2/3 (q2 + dt L(q2))."""
+        q0 = q.copy()
+        cur = q0
+        for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
+            a = np.zeros_like(q)
+            self.scalar_tendency(dt, u, v, w, cur, a, adv)
+            nxt = c0 * q0 + c1 * (cur + dt * a)
+            self.fill_scalar(nxt)
+            cur = nxt
+        q[...] = cur
