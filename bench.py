@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X advection operator (one JSON line on stdout).
+
+Workload (default): the 3D plain-method forward run on one GPU --
+256x256x128 interior cells, dx = dy = 50 m, dz stretched linearly from 20 m
+to 80 m, periodic laterally and no-flux at floor and lid, divergence-free
+turbulent wind (curl of a seeded vector potential, k^-5/3, 2 m/s RMS), two
+advected fields (a log-normal positive-definite scalar, mu = -9, sigma = 1,
+and theta = 300 K + 3 K/km + 0.1 K noise), reference 'weno' scheme, Heun
+stages (sorder = 2), FP64, one plain per-field launch per field and stage,
+all state resident in HBM.  A "step" is one full time step (two stages of
+both fields); K steps are timed.
+
+value: cell.species-updates/s = interior cells x fields x stages x K / time.
+roofline: the dominant kernel (the fused scalar tendency + stage kernel)
+against the 8 TB/s HBM peak, with its algorithmic bytes per launch and its
+average duration from HIP events recorded by the library on its stream.
+cpu_baseline: the C restatement of the reference (the oracle) on the host
+cores, on a bounded number of Heun steps of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_case(maxx, maxy, nz, seed, rms):
+    from fdd3260_amd import cases, grid as gm
+
+    g = gm.stretched(maxx, maxy, nz, dx=50.0, dy=50.0, dz_bottom=20.0, dz_top=80.0)
+    u, v, w = cases.turbulent_wind(g, seed=seed, rms=rms)
+    q = cases.lognormal(g, seed=seed + 1, mu=-9.0, sigma=1.0)
+    th = cases.theta_profile(g, seed=seed + 2, noise=0.1)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    return g, u, v, w, [q, th], dt
+
+
+def stage_bytes(integrator, ncell):
+    """Algorithmic HBM bytes of each stage launch of one FP64 field: every
+    array the stage touches, read or written once, plus the 3 velocity
+    components."""
+    vel = 3 * 8
+    if integrator == "heun":
+        # stage 1: read q (also q0), write tend, write q*; stage 2: read q*,
+        # q0, tend, write q
+        per = [(1 + 2) * 8 + vel, (3 + 1) * 8 + vel]
+    else:
+        # Shu-Osher stages: stage 1 reads q (= q0) writes q1; later stages
+        # read the stage state and q0, write the next state
+        per = [(1 + 1) * 8 + vel, (2 + 1) * 8 + vel, (2 + 1) * 8 + vel]
+    return [p * ncell for p in per]
+
+
+def cpu_baseline(g, u, v, w, fields, dt, scheme, limiter, integrator, steps, threads):
+    from oracle.oracle import Oracle
+
+    ora = Oracle(g, scal_adv=scheme, muscllimiter=limiter)
+    ora.set_threads(threads)
+    qs = [f.copy() for f in fields]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for q in qs:
+            (ora.heun_step if integrator == "heun" else ora.ssprk3_step)(dt, u, v, w, q)
+    el = time.perf_counter() - t0
+    nst = 2 if integrator == "heun" else 3
+    ncell = g.maxx * max(g.maxy, 1) * g.nz
+    return ncell * len(fields) * nst * steps / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--scheme", default="weno", choices=["weno", "ppm", "muscl", "weno5"])
+    ap.add_argument("--limiter", default="MC")
+    ap.add_argument("--integrator", default="heun", choices=["heun", "ssprk3"])
+    ap.add_argument("--size", default="256,256,128")
+    ap.add_argument("--cpu-steps", type=int, default=6, help="Heun steps of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (at most 16)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the advection operator is built and measured on a single "
+              "MI355X; runs over several GPUs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from fdd3260_amd import Advection, _lib
+
+    maxx, maxy, nz = (int(s) for s in args.size.split(","))
+    t0 = time.time()
+    g, u, v, w, fields, dt = build_case(maxx, maxy, nz, seed=2, rms=2.0)
+    log(f"case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
+    dev = torch.device("cuda:0")
+    du, dv, dw = (torch.from_numpy(a).to(dev) for a in (u, v, w))
+    dq = [torch.from_numpy(f).to(dev) for f in fields]
+    adv = Advection(g, scal_adv=args.scheme, muscllimiter=args.limiter)
+    code = _lib.INT_HEUN if args.integrator == "heun" else _lib.INT_SSPRK3
+    nst = 2 if args.integrator == "heun" else 3
+
+    if args.warmup > 0:
+        adv.advance(code, args.warmup, dt, du, dv, dw, dq)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    adv.advance(code, args.steps, dt, du, dv, dw, dq)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    kern_ms = adv.last_kernel_ms()
+    for f in dq:
+        if not bool(torch.isfinite(f).all()):
+            raise RuntimeError("non-finite state after the timed steps")
+
+    ncell = maxx * max(maxy, 1) * nz
+    nfield = len(fields)
+    updates = ncell * nfield * nst * args.steps
+    value = updates / wall
+    launches = nfield * nst * args.steps
+    avg_launch_s = kern_ms * 1e-3 / launches
+    sb = stage_bytes(args.integrator, ncell)
+    alg_bytes = sum(sb) / len(sb)
+    achieved = alg_bytes / avg_launch_s / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {args.cpu_steps} steps with {threads} threads")
+        rate, el = cpu_baseline(g, u, v, w, fields, dt, args.scheme, args.limiter, args.integrator,
+                                args.cpu_steps, threads)
+        cpu = {
+            "value": rate,
+            "unit": "cell*species-updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} {args.integrator} steps of the same {maxx}x{max(maxy, 1)}x{nz} "
+                      f"{nfield}-field workload, C restatement (gcc -O2, no FMA contraction, OpenMP over "
+                      f"lines), {el:.1f}s",
+        }
+
+    out = {
+        "metric": "cell*species-updates/s and % HBM roofline at 1 GPU",
+        "value": value,
+        "unit": "cell*species-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded turbulent wind, log-normal scalar, theta profile)",
+        "config": {
+            "workload": f"3D plain per-field forward {maxx}x{max(maxy, 1)}x{nz}, stretched dz 20-80 m, "
+                        f"{args.scheme}{'/' + args.limiter if args.scheme == 'muscl' else ''}, "
+                        f"{args.integrator}, {nfield} fields, FP64",
+            "cells": ncell,
+            "fields": nfield,
+            "stages_per_step": nst,
+            "dt_s": dt,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"scalar_fp64_kernel<{SCHEME_ID[args.scheme]},64,4>",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_launch": alg_bytes,
+            "avg_launch_us": avg_launch_s * 1e6,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
