@@ -28,6 +28,8 @@ INT_EULER, INT_HEUN, INT_SSPRK3 = 1, 2, 3
 DIR_X, DIR_Y, DIR_Z, DIR_ALL = 1, 2, 4, 7
 OVERWRITE = 8
 HOST_PTRS = 16
+GRAPH = 32
+F64, F32 = 0, 1
 
 EXPORTED = (
     "mimadv_create",
@@ -43,6 +45,10 @@ EXPORTED = (
     "mimadv_totalav",
     "mimadv_advance",
     "mimadv_last_kernel_ms",
+    "mimadv_set_tuning",
+    "mimadv_batch_tendency",
+    "mimadv_batch_advance",
+    "mimadv_batch_budget",
 )
 
 
@@ -72,6 +78,10 @@ class Opts(C.Structure):
         ("mom_ord", C.c_int),
         ("lavisc", C.c_int),
     ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("group32", C.c_int), ("kchunk", C.c_int), ("blocks_per_cu", C.c_int)]
 
 
 _lib = None
@@ -121,6 +131,14 @@ def load() -> C.CDLL:
     lib.mimadv_advance.restype = i
     lib.mimadv_last_kernel_ms.argtypes = [vp, C.POINTER(d)]
     lib.mimadv_last_kernel_ms.restype = i
+    lib.mimadv_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.mimadv_set_tuning.restype = i
+    lib.mimadv_batch_tendency.argtypes = [vp, vp, vp, vp, i, ip, pp, pp, u]
+    lib.mimadv_batch_tendency.restype = i
+    lib.mimadv_batch_advance.argtypes = [vp, i, i, d, vp, vp, vp, i, ip, pp, u]
+    lib.mimadv_batch_advance.restype = i
+    lib.mimadv_batch_budget.argtypes = [vp, i, ip, pp, C.POINTER(d)]
+    lib.mimadv_batch_budget.restype = i
     _lib = lib
     return lib
 

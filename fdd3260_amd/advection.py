@@ -220,3 +220,63 @@ class Advection:
         rc = self.lib.mimadv_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1], ptrs[2],
                                      nf, _lib.ptr_array(ptrs[3:]), ad, 0)
         self._check(rc, "mimadv_advance")
+
+    # -- batched mixed-precision path ---------------------------------------
+    def set_tuning(self, group32=0, kchunk=0, blocks_per_cu=0):
+        t = _lib.Tuning(int(group32), int(kchunk), int(blocks_per_cu))
+        self._check(self.lib.mimadv_set_tuning(self._h, C.byref(t)), "mimadv_set_tuning")
+
+    def _batch_fields(self, fields):
+        import torch
+
+        shape = self.grid.shape
+        prec, ptrs = [], []
+        for f in fields:
+            if not _is_torch(f) or not f.is_cuda or not f.is_contiguous() or tuple(f.shape) != shape:
+                raise MimadvError(f"batched fields must be contiguous GPU tensors of shape {shape}")
+            if f.dtype == torch.float32:
+                prec.append(_lib.F32)
+            elif f.dtype == torch.float64:
+                prec.append(_lib.F64)
+            else:
+                raise MimadvError("batched fields must be float32 or float64")
+            ptrs.append(f.data_ptr())
+        return (C.c_int * max(1, len(prec)))(*prec), _lib.ptr_array(ptrs)
+
+    def _vel32(self, u, v, w):
+        import torch
+
+        shape = self.grid.shape
+        out = []
+        for a in (u, v, w):
+            if a is None:
+                out.append(None)
+                continue
+            if not _is_torch(a) or a.dtype != torch.float32 or not a.is_cuda or not a.is_contiguous() \
+                    or tuple(a.shape) != shape:
+                raise MimadvError(f"batched velocities must be contiguous float32 GPU tensors of shape {shape}")
+            out.append(a.data_ptr())
+        return out
+
+    def batch_tendency(self, u, v, w, fields, outs, dirs=_lib.DIR_ALL):
+        pr, pq = self._batch_fields(fields)
+        pr2, px = self._batch_fields(outs)
+        if list(pr) != list(pr2):
+            raise MimadvError("tendencies must have the precision of their fields")
+        pu, pv, pw = self._vel32(u, v, w)
+        rc = self.lib.mimadv_batch_tendency(self._h, pu, pv, pw, len(fields), pr, pq, px, int(dirs))
+        self._check(rc, "mimadv_batch_tendency")
+
+    def batch_advance(self, integrator, nsteps, dt, u, v, w, fields, graph=False):
+        pr, pq = self._batch_fields(fields)
+        pu, pv, pw = self._vel32(u, v, w)
+        flags = _lib.GRAPH if graph else 0
+        rc = self.lib.mimadv_batch_advance(self._h, int(integrator), int(nsteps), float(dt), pu, pv, pw,
+                                           len(fields), pr, pq, flags)
+        self._check(rc, "mimadv_batch_advance")
+
+    def batch_budget(self, fields):
+        pr, pq = self._batch_fields(fields)
+        out = (C.c_double * max(1, len(fields)))()
+        self._check(self.lib.mimadv_batch_budget(self._h, len(fields), pr, pq, out), "mimadv_batch_budget")
+        return [out[n] for n in range(len(fields))]

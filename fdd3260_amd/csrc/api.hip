@@ -10,6 +10,7 @@
 
 #include "../../include/mimadv.h"
 #include "mimadv_internal.hpp"
+#include "fused.hpp"
 
 using namespace mimadv;
 
@@ -29,6 +30,12 @@ struct mimadv_ctx {
     // advance scratch: two arrays per field
     double *work = nullptr;
     size_t work_elems = 0;
+    // batched path: scratch bytes, tuning, captured step graph
+    char *bwork = nullptr;
+    size_t bwork_bytes = 0;
+    mimadv_tuning tuning{0, 0, 0};
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<const void *> graph_key;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
@@ -185,6 +192,8 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->d_red) hipFree(c->d_red);
     if (c->stage) hipFree(c->stage);
     if (c->work) hipFree(c->work);
+    if (c->bwork) hipFree(c->bwork);
+    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
@@ -483,6 +492,278 @@ int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const d
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->timing_pending = false;
+    return MIMADV_OK;
+}
+
+int mimadv_set_tuning(mimadv_ctx *c, const mimadv_tuning *t)
+{
+    if (!c || !t) return MIMADV_ERR_ARG;
+    if (t->group32 < 0 || t->group32 > FUSED_MAX32 || t->kchunk < 0 || t->blocks_per_cu < 0)
+        return fail(c, MIMADV_ERR_ARG, "tuning values out of range");
+    if (t->group32 > 0 && fused_padded_count(t->group32) != t->group32)
+        return fail(c, MIMADV_ERR_ARG, "group32 must be one of 1, 2, 4, 8, 16, 18");
+    c->tuning = *t;
+    return MIMADV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+struct BatchGroup {
+    std::vector<int> f64, f32;  // field indices
+};
+
+// FP64 fields ride with the first group; FP32 fields are cut into groups of
+// at most group32 (register pressure of the kernel grows with the count).
+int plan_groups(mimadv_ctx *c, int nf, const int *prec, std::vector<BatchGroup> &groups)
+{
+    std::vector<int> i64, i32;
+    for (int f = 0; f < nf; ++f) {
+        if (prec[f] == MIMADV_F64) i64.push_back(f);
+        else if (prec[f] == MIMADV_F32) i32.push_back(f);
+        else return fail(c, MIMADV_ERR_ARG, "precision must be MIMADV_F32 or MIMADV_F64");
+    }
+    if ((int)i64.size() > FUSED_MAX64) return fail(c, MIMADV_ERR_UNSUPPORTED, "at most 2 FP64 fields per batch");
+    int g32 = c->tuning.group32 > 0 ? c->tuning.group32 : 8;
+    groups.clear();
+    size_t p = 0;
+    do {
+        BatchGroup b;
+        if (groups.empty()) b.f64 = i64;
+        for (int n = 0; n < g32 && p < i32.size(); ++n) b.f32.push_back(i32[p++]);
+        groups.push_back(b);
+    } while (p < i32.size());
+    return MIMADV_OK;
+}
+
+int check_batch(mimadv_ctx *c, const float *u, const float *v, const float *w)
+{
+    if (c->sp.scheme != MIMADV_WENO && c->sp.scheme != MIMADV_MUSCL)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path implements the WENO and MUSCL schemes");
+    for (int b = 0; b < 4; ++b)
+        if (c->g.bcl[b] != MIMADV_BC_PER && !(c->g.ny == 1 && b >= 2))
+            return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path needs periodic lateral boundaries");
+    if (c->g.bcl[4] != MIMADV_BC_NNF || c->g.bcl[5] != MIMADV_BC_NNF)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path needs no-flux ('nnf') floor and lid");
+    if (!u || !w || (c->g.ny > 1 && !v)) return fail(c, MIMADV_ERR_ARG, "NULL velocity");
+    return MIMADV_OK;
+}
+
+FusedArgs base_args(mimadv_ctx *c, const float *u, const float *v, const float *w)
+{
+    FusedArgs A{};
+    A.nx = c->g.nx;
+    A.ny = c->g.ny;
+    A.nz = c->g.nz;
+    A.mx = c->g.nx - 5;
+    A.my = c->g.ny - 5;
+    A.three_d = c->g.ny > 1;
+    A.dx1 = 1.0 / c->g.dx;
+    A.dy1 = 1.0 / c->g.dy;
+    A.dz = c->g.dz;
+    A.dz1w = c->g.zm.dz1w;
+    A.dzw = c->g.zm.dzw;
+    A.dz1 = c->g.zm.dz1;
+    A.fdz0 = c->g.zm.fdz0;
+    A.u = u;
+    A.v = v;
+    A.w = w;
+    A.muscl_limiter = c->sp.muscl_limiter;
+    A.lim_tol = c->sp.lim_tol;
+    A.weno_bottom = c->sp.weno_bottom;
+    A.kchunk = c->tuning.kchunk > 0 ? c->tuning.kchunk : c->g.nz;
+    A.dirmask = 7;
+    return A;
+}
+
+// One launch per group.  in/in0/out give, per field index, the stencil
+// input, the step-start state and the output.
+hipError_t launch_groups(mimadv_ctx *c, const FusedArgs &base, const std::vector<BatchGroup> &groups,
+                         const std::vector<const void *> &in, const std::vector<const void *> &in0,
+                         const std::vector<void *> &out)
+{
+    for (const BatchGroup &b : groups) {
+        FusedArgs A = base;
+        A.n64_active = (int)b.f64.size();
+        A.n32_active = (int)b.f32.size();
+        for (size_t n = 0; n < b.f64.size(); ++n) {
+            int f = b.f64[n];
+            A.q64[n] = (const double *)in[f];
+            A.q0_64[n] = (const double *)in0[f];
+            A.out64[n] = (double *)out[f];
+        }
+        for (size_t n = 0; n < b.f32.size(); ++n) {
+            int f = b.f32[n];
+            A.q32[n] = (const float *)in[f];
+            A.q0_32[n] = (const float *)in0[f];
+            A.out32[n] = (float *)out[f];
+        }
+        int n32 = fused_padded_count(A.n32_active);
+        hipError_t e = launch_fused(c->sp.scheme, A.n64_active, n32, A, c->stream, c->tuning.blocks_per_cu);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mimadv_batch_tendency(mimadv_ctx *c, const float *u, const float *v, const float *w, int nf,
+                          const int *precision, const void *const *q, void *const *xs, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path works on device arrays");
+    int rc = check_batch(c, u, v, w);
+    if (rc) return rc;
+    if (nf <= 0) return nf == 0 ? MIMADV_OK : fail(c, MIMADV_ERR_ARG, "nf < 0");
+    if (!precision || !q || !xs) return fail(c, MIMADV_ERR_ARG, "NULL field list");
+    std::vector<BatchGroup> groups;
+    rc = plan_groups(c, nf, precision, groups);
+    if (rc) return rc;
+    std::vector<const void *> in(nf), in0(nf);
+    std::vector<void *> out(nf);
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f] || !xs[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        in[f] = q[f];
+        in0[f] = q[f];
+        out[f] = xs[f];
+    }
+    FusedArgs A = base_args(c, u, v, w);
+    A.epi = 0;
+    A.fill = 0;
+    A.dirmask = (flags & MIMADV_DIR_ALL) ? (int)(flags & MIMADV_DIR_ALL) : 7;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_groups(c, A, groups, in, in0, out));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const float *u, const float *v,
+                         const float *w, int nf, const int *precision, void *const *q, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path works on device arrays");
+    int rc = check_batch(c, u, v, w);
+    if (rc) return rc;
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "batched integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
+    if (nsteps < 0 || nf < 0) return fail(c, MIMADV_ERR_ARG, "negative count");
+    if (nsteps == 0 || nf == 0) return MIMADV_OK;
+    if (!precision || !q) return fail(c, MIMADV_ERR_ARG, "NULL field list");
+    std::vector<BatchGroup> groups;
+    rc = plan_groups(c, nf, precision, groups);
+    if (rc) return rc;
+    size_t n3 = c->n3();
+    // two stage buffers per field, in the field's precision
+    std::vector<size_t> off(nf);
+    size_t bytes = 0;
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        size_t es = precision[f] == MIMADV_F64 ? 8 : 4;
+        bytes = (bytes + 255) & ~(size_t)255;
+        off[f] = bytes;
+        bytes += 2 * n3 * es;
+    }
+    if (bytes > c->bwork_bytes) {
+        if (c->bwork) hipFree(c->bwork);
+        c->bwork = nullptr;
+        c->bwork_bytes = 0;
+        if (c->graph_exec) {
+            hipGraphExecDestroy(c->graph_exec);
+            c->graph_exec = nullptr;
+        }
+        if (hipMalloc(&c->bwork, bytes) != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "batch scratch allocation failed");
+        c->bwork_bytes = bytes;
+    }
+    std::vector<void *> s1(nf), s2(nf);
+    for (int f = 0; f < nf; ++f) {
+        size_t es = precision[f] == MIMADV_F64 ? 8 : 4;
+        s1[f] = c->bwork + off[f];
+        s2[f] = c->bwork + off[f] + n3 * es;
+        // stage buffers start as copies so their ghost columns are defined
+        HIPCHK(c, hipMemcpyAsync(s1[f], q[f], n3 * es, hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s2[f], q[f], n3 * es, hipMemcpyDeviceToDevice, c->stream));
+    }
+    FusedArgs A = base_args(c, u, v, w);
+    A.epi = 1;
+    A.fill = 1;
+    A.dt = dt;
+    struct Stage {
+        double c0, c1;
+        int in, out;  // 0: q, 1: s1, 2: s2
+    };
+    std::vector<Stage> st;
+    if (integrator == MIMADV_INT_HEUN) st = {{0.0, 1.0, 0, 1}, {0.5, 0.5, 1, 0}};
+    else st = {{0.0, 1.0, 0, 1}, {0.75, 0.25, 1, 2}, {1.0 / 3.0, 2.0 / 3.0, 2, 0}};
+    auto pick = [&](int which, int f) -> void * { return which == 0 ? q[f] : (which == 1 ? s1[f] : s2[f]); };
+    auto one_step = [&]() -> hipError_t {
+        for (const Stage &s : st) {
+            FusedArgs B = A;
+            B.c0 = s.c0;
+            B.c1 = s.c1;
+            std::vector<const void *> in(nf), in0(nf);
+            std::vector<void *> out(nf);
+            for (int f = 0; f < nf; ++f) {
+                in[f] = pick(s.in, f);
+                in0[f] = q[f];
+                out[f] = pick(s.out, f);
+            }
+            hipError_t e = launch_groups(c, B, groups, in, in0, out);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    bool use_graph = (flags & MIMADV_GRAPH) != 0;
+    if (use_graph) {
+        // the captured step depends on every pointer and scalar it bakes in
+        std::vector<const void *> key = {u, v, w, (const void *)c->bwork};
+        for (int f = 0; f < nf; ++f) key.push_back(q[f]);
+        key.push_back((const void *)(size_t)integrator);
+        key.push_back((const void *)(size_t)nf);
+        double dtk = dt;
+        size_t dbits;
+        std::memcpy(&dbits, &dtk, sizeof dbits);
+        key.push_back((const void *)dbits);
+        if (!c->graph_exec || key != c->graph_key) {
+            if (c->graph_exec) {
+                hipGraphExecDestroy(c->graph_exec);
+                c->graph_exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t le = one_step();
+            hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+            if (le != hipSuccess) return hipfail(c, le, "launch during capture");
+            if (ce != hipSuccess) return hipfail(c, ce, "hipStreamEndCapture");
+            hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (ie != hipSuccess) return hipfail(c, ie, "hipGraphInstantiate");
+            c->graph_key = key;
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+        else HIPCHK(c, one_step());
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+int mimadv_batch_budget(mimadv_ctx *c, int nf, const int *precision, const void *const *q, double *out)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0 || (nf > 0 && (!precision || !q || !out))) return fail(c, MIMADV_ERR_ARG, "bad field list");
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        HIPCHK(c, launch_wsum(c->g, q[f], precision[f] == MIMADV_F32, c->d_red + 2, c->d_red, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out + f, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return MIMADV_OK;
 }
 

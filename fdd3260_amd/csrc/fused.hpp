@@ -1,0 +1,36 @@
+// fused.hpp -- arguments of the batched mixed-precision advection kernel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace mimadv {
+
+constexpr int FUSED_MAX64 = 2;
+constexpr int FUSED_MAX32 = 18;
+
+struct FusedArgs {
+    int nx, ny, nz, mx, my, three_d;
+    double dx1, dy1, dz;
+    const double *dz1w, *dzw, *dz1, *fdz0;  // device metrics
+    const float *u, *v, *w;                 // advecting velocity (FP32)
+    const double *q64[FUSED_MAX64];         // stencil input
+    const double *q0_64[FUSED_MAX64];       // state at the start of the step
+    double *out64[FUSED_MAX64];             // tendency or next stage state
+    const float *q32[FUSED_MAX32];
+    const float *q0_32[FUSED_MAX32];
+    float *out32[FUSED_MAX32];
+    int n64_active, n32_active;  // fields beyond these are padding
+    int epi;                     // 0: out = -div; 1: Shu-Osher stage
+    double c0, c1, dt;           // out = c0*q0 + c1*(q + dt*(-div))
+    int fill;                    // write periodic ghost images
+    int muscl_limiter;
+    double lim_tol;
+    int weno_bottom;
+    int ntx, nty, kchunk, nwork, xcd_remap;
+    int dirmask;  // directions included in the divergence (bit0 x, bit1 y, bit2 z)
+};
+
+// Padded FP32 field count with a compiled kernel (-1 if too many).
+int fused_padded_count(int n32);
+hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStream_t s, int blocks_per_cu);
+
+}  // namespace mimadv

@@ -49,6 +49,7 @@ hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
 hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
                              double *tend, double *x, hipStream_t s);
 hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out2, hipStream_t s);
+hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_part, double *d_out, hipStream_t s);
 
 // Host-side PPM per-level coefficient table (12 doubles per level).
 std::vector<double> ppm_coefficients(int nz, const double *dz1w);

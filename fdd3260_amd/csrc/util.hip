@@ -139,7 +139,54 @@ __global__ void finish_sums_kernel(int nz, long ncells, const double *__restrict
     out[1] = sw;
 }
 
+// dzw-weighted interior sum per level of an FP32 or FP64 field (FP64
+// accumulation, deterministic tree)
+template <class T>
+__global__ __launch_bounds__(256) void level_wsums_kernel(int nx, int ny, int nz, const T *__restrict__ x,
+                                                          const double *__restrict__ dzw, double *__restrict__ part)
+{
+    const bool three_d = ny > 1;
+    int k = blockIdx.x + 1;
+    int j0 = three_d ? 4 : 1, j1 = three_d ? ny - 2 : 1;
+    int mx = nx - 5, my = j1 - j0 + 1;
+    long cnt = (long)mx * my;
+    const T *p = x + (size_t)nx * ny * (k - 1);
+    double s = 0.0;
+    for (long t = threadIdx.x; t < cnt; t += blockDim.x) {
+        int jj = (int)(t / mx), ii = (int)(t - (long)jj * mx);
+        s += double(p[(size_t)(ii + 3) + (size_t)nx * (jj + j0 - 1)]);
+    }
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[k - 1] = dzw[k - 1] * sh[0];
+}
+
+__global__ void sum_levels_kernel(int nz, const double *__restrict__ part, double *__restrict__ out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < nz; ++k) s += part[k];
+    out[0] = s;
+}
+
 }  // namespace
+
+hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_part, double *d_out, hipStream_t s)
+{
+    if (fp32)
+        hipLaunchKernelGGL(level_wsums_kernel<float>, dim3(g.nz), dim3(256), 0, s, g.nx, g.ny, g.nz, (const float *)x,
+                           g.zm.dzw, d_part);
+    else
+        hipLaunchKernelGGL(level_wsums_kernel<double>, dim3(g.nz), dim3(256), 0, s, g.nx, g.ny, g.nz,
+                           (const double *)x, g.zm.dzw, d_part);
+    hipLaunchKernelGGL(sum_levels_kernel, dim3(1), dim3(64), 0, s, g.nz, d_part, d_out);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s)
 {

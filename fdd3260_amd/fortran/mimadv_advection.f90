@@ -1,0 +1,271 @@
+!
+! mimadv_advection.f90 -- Fortran drop-in for MIMICA's advection interface.
+!
+! The reference module `advection` (src/TOOLS/advection.f90:79) exports
+!   advect, advection_x, advection_y, advection_z, advection_ac, ...
+! and module `windadv` (src/INTEG/windadv.f90:41) exports qnadv_uvw.  This
+! module keeps advection_x/y/z with the reference's argument list and
+! meaning (explicit-shape (ip_start:ip_end, jp_start:jp_end, 1:nz) REAL(8)
+! arrays, xs accumulated with -div) and forwards them through iso_c_binding
+! to libmimadv.so (include/mimadv.h).  The derived-type entry points of the
+! reference (advect, advection_ac, qnadv_uvw on atm_state / atm_winds /
+! hydrometeor) are thin loops over these array routines, given in
+! INTEGRATION.md; the array forms below are what they call.
+!
+! mimadv_init replaces advection_init (advection.f90:1045-1067) and takes
+! explicitly the module globals the reference reads (shared_data.f90:
+! scal_adv, lim_tol, mom_ord, lavisc, bcl, dx, dy, dz, fdz0, dzw, dz1w,
+! dz1; muscllimiter of advq_muscl.f90:49; ppmlimiter of advq_ppm.f90:49).
+! A non-zero status from the library stops the run, as the reference stops
+! in checkall (check.f90:59-101) through stop_mimica(1).
+!
+module mimadv_advection
+  use iso_c_binding
+  implicit none
+  private
+
+  public :: mimadv_init, mimadv_finalize
+  public :: advection_x, advection_y, advection_z, advection_ac_arrays, qnadv_uvw_arrays
+
+  integer(c_int), parameter :: MIMADV_DIR_X = 1, MIMADV_DIR_Y = 2, MIMADV_DIR_Z = 4
+  integer(c_int), parameter :: MIMADV_HOST_PTRS = 16
+
+  type, bind(c) :: mimadv_grid_t
+    integer(c_int) :: nx, ny, nz
+    real(c_double) :: dx, dy, dz
+    type(c_ptr) :: fdz0, dzw, dz1w, dz1
+    integer(c_int) :: bcl(6)
+  end type
+
+  type, bind(c) :: mimadv_opts_t
+    integer(c_int) :: scheme, muscl_limiter, ppm_limiter
+    real(c_double) :: lim_tol
+    integer(c_int) :: weno_bottom, mom_ord, lavisc
+  end type
+
+  interface
+    integer(c_int) function c_create(grid, opts, ctx) bind(c, name='mimadv_create')
+      import :: c_int, c_ptr, mimadv_grid_t, mimadv_opts_t
+      type(mimadv_grid_t), intent(in) :: grid
+      type(mimadv_opts_t), intent(in) :: opts
+      type(c_ptr), intent(out) :: ctx
+    end function
+    subroutine c_destroy(ctx) bind(c, name='mimadv_destroy')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    type(c_ptr) function c_last_error(ctx) bind(c, name='mimadv_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function c_scalar_tendency(ctx, dt0, u, v, w, nf, q, xs, adv_form, flags) &
+        bind(c, name='mimadv_scalar_tendency')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dt0
+      type(c_ptr), value :: u, v, w
+      integer(c_int), value :: nf
+      type(c_ptr), value :: q, xs, adv_form
+      integer(c_int), value :: flags
+    end function
+    integer(c_int) function c_momentum_tendency(ctx, ntime, mu, mv, mw, ut, vt, wt, ua, va, wa, flags) &
+        bind(c, name='mimadv_momentum_tendency')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ntime
+      type(c_ptr), value :: mu, mv, mw, ut, vt, wt, ua, va, wa
+      integer(c_int), value :: flags
+    end function
+  end interface
+
+  type(c_ptr), save :: ctx = c_null_ptr
+  integer, save :: g_nx = 0, g_ny = 0, g_nz = 0
+  logical, save :: g_conservative = .false.
+  real(c_double), allocatable, target, save :: m_fdz0(:), m_dzw(:), m_dz1w(:), m_dz1(:)
+
+contains
+
+  subroutine check(status, what)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (status == 0) return
+    call c_f_pointer(c_last_error(ctx), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    enddo
+    write(*, '(a,a,a,i0,a,512a)') 'mimadv: ', what, ' failed (status ', status, '): ', msg(1:n)
+    error stop 1
+  end subroutine
+
+  integer(c_int) function bc_code(s)
+    character(len=*), intent(in) :: s
+    select case (trim(s))
+    case ('per')
+      bc_code = 0
+    case ('nnf')
+      bc_code = 1
+    case default
+      bc_code = 2
+    end select
+  end function
+
+  ! One-time set-up (advection_init plus the module globals it relies on).
+  subroutine mimadv_init(nx, ny, nz, dx, dy, dz, fdz0, dzw, dz1w, dz1, bcl, scal_adv, muscllimiter, &
+                         ppmlimiter, lim_tol, mom_ord, lavisc, conservative)
+    integer, intent(in) :: nx, ny, nz, mom_ord
+    real(8), intent(in) :: dx, dy, dz, lim_tol
+    real(8), intent(in) :: fdz0(nz), dzw(nz), dz1w(nz), dz1(nz)
+    character(len=3), intent(in) :: bcl(6)
+    character(len=*), intent(in) :: scal_adv, muscllimiter, ppmlimiter
+    logical, intent(in) :: lavisc, conservative
+    type(mimadv_grid_t) :: g
+    type(mimadv_opts_t) :: o
+    integer :: b
+    if (c_associated(ctx)) call mimadv_finalize()
+    allocate(m_fdz0(nz), m_dzw(nz), m_dz1w(nz), m_dz1(nz))
+    m_fdz0 = fdz0; m_dzw = dzw; m_dz1w = dz1w; m_dz1 = dz1
+    g%nx = nx; g%ny = ny; g%nz = nz
+    g%dx = dx; g%dy = dy; g%dz = dz
+    g%fdz0 = c_loc(m_fdz0); g%dzw = c_loc(m_dzw); g%dz1w = c_loc(m_dz1w); g%dz1 = c_loc(m_dz1)
+    do b = 1, 6
+      g%bcl(b) = bc_code(bcl(b))
+    enddo
+    select case (trim(scal_adv))
+    case ('weno')
+      o%scheme = 0
+    case ('ppm')
+      o%scheme = 1
+    case ('muscl')
+      o%scheme = 2
+    case default
+      write(*, '(a,a)') 'mimadv: scal_adv not provided by the GPU operator: ', trim(scal_adv)
+      error stop 1
+    end select
+    select case (trim(muscllimiter))
+    case ('MC')
+      o%muscl_limiter = 1
+    case ('VL')
+      o%muscl_limiter = 2
+    case ('minmod')
+      o%muscl_limiter = 0
+    case default
+      o%muscl_limiter = 3
+    end select
+    o%ppm_limiter = merge(1, 0, trim(ppmlimiter) == 'PPM')
+    o%lim_tol = lim_tol
+    o%weno_bottom = 0
+    o%mom_ord = mom_ord
+    o%lavisc = merge(1, 0, lavisc)
+    g_nx = nx; g_ny = ny; g_nz = nz
+    g_conservative = conservative
+    call check(c_create(g, o, ctx), 'mimadv_create')
+  end subroutine
+
+  subroutine mimadv_finalize()
+    if (c_associated(ctx)) call c_destroy(ctx)
+    ctx = c_null_ptr
+    if (allocated(m_fdz0)) deallocate(m_fdz0, m_dzw, m_dz1w, m_dz1)
+  end subroutine
+
+  subroutine scalar_dir(dirflag, dtl, xtmp, xs, vel, lim, adv)
+    integer(c_int), intent(in) :: dirflag
+    real(8), intent(in) :: dtl
+    real(8), target, intent(in) :: xtmp(g_nx, g_ny, g_nz), vel(g_nx, g_ny, g_nz)
+    real(8), target, intent(inout) :: xs(g_nx, g_ny, g_nz)
+    logical, intent(in) :: lim
+    logical, optional, intent(in) :: adv
+    type(c_ptr), target :: qp(1), xp(1)
+    integer(c_int), target :: af(1)
+    type(c_ptr) :: pu, pv, pw
+    if (.not. lim) then
+      write(*, '(a)') 'mimadv: unlimited advection (LW/QUICK without limiter) is not provided by the GPU operator'
+      error stop 1
+    endif
+    af(1) = merge(1, 0, .not. g_conservative)
+    if (present(adv)) af(1) = merge(1, 0, adv)
+    qp(1) = c_loc(xtmp)
+    xp(1) = c_loc(xs)
+    pu = c_null_ptr; pv = c_null_ptr; pw = c_null_ptr
+    if (dirflag == MIMADV_DIR_X) pu = c_loc(vel)
+    if (dirflag == MIMADV_DIR_Y) pv = c_loc(vel)
+    if (dirflag == MIMADV_DIR_Z) pw = c_loc(vel)
+    call check(c_scalar_tendency(ctx, dtl, pu, pv, pw, 1_c_int, c_loc(qp), c_loc(xp), c_loc(af), &
+                                 ior(dirflag, MIMADV_HOST_PTRS)), 'advection')
+  end subroutine
+
+  ! advection_x (advection.f90:556): xs = xs - (x flux divergence of xtmp)
+  subroutine advection_x(name, dtl, xrho, xtmp, xs, u, lim, adv)
+    character(len=2), intent(in) :: name
+    real(8), intent(in) :: dtl
+    real(8), intent(inout) :: xrho(g_nx, g_ny, g_nz)
+    real(8), intent(in) :: xtmp(g_nx, g_ny, g_nz), u(g_nx, g_ny, g_nz)
+    real(8), intent(inout) :: xs(g_nx, g_ny, g_nz)
+    logical, intent(in) :: lim
+    logical, optional, intent(in) :: adv
+    call scalar_dir(MIMADV_DIR_X, dtl, xtmp, xs, u, lim, adv)
+  end subroutine
+
+  ! advection_y (advection.f90:637)
+  subroutine advection_y(name, dtl, xrho, xtmp, xs, v, lim, adv)
+    character(len=2), intent(in) :: name
+    real(8), intent(in) :: dtl
+    real(8), intent(inout) :: xrho(g_nx, g_ny, g_nz)
+    real(8), intent(in) :: xtmp(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz)
+    real(8), intent(inout) :: xs(g_nx, g_ny, g_nz)
+    logical, intent(in) :: lim
+    logical, optional, intent(in) :: adv
+    call scalar_dir(MIMADV_DIR_Y, dtl, xtmp, xs, v, lim, adv)
+  end subroutine
+
+  ! advection_z (advection.f90:719)
+  subroutine advection_z(name, dtl, xrho, xtmp, xs, w, lim, adv)
+    character(len=2), intent(in) :: name
+    real(8), intent(in) :: dtl
+    real(8), intent(inout) :: xrho(g_nx, g_ny, g_nz)
+    real(8), intent(in) :: xtmp(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    real(8), intent(inout) :: xs(g_nx, g_ny, g_nz)
+    logical, intent(in) :: lim
+    logical, optional, intent(in) :: adv
+    call scalar_dir(MIMADV_DIR_Z, dtl, xtmp, xs, w, lim, adv)
+  end subroutine
+
+  ! advection_ac (advection.f90:801) on the wind components: xs = xs + xsx +
+  ! xsy + xsz with dt0, all three directions in one call
+  subroutine advection_ac_arrays(dt0, u, v, w, xx, xs, ilim)
+    real(8), intent(in) :: dt0
+    real(8), target, intent(in) :: u(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    real(8), target, intent(in) :: xx(g_nx, g_ny, g_nz)
+    real(8), target, intent(inout) :: xs(g_nx, g_ny, g_nz)
+    logical, optional, intent(in) :: ilim
+    type(c_ptr), target :: qp(1), xp(1)
+    integer(c_int), target :: af(1)
+    if (present(ilim)) then
+      if (.not. ilim) then
+        write(*, '(a)') 'mimadv: unlimited advection is not provided by the GPU operator'
+        error stop 1
+      endif
+    endif
+    af(1) = merge(1, 0, .not. g_conservative)
+    qp(1) = c_loc(xx)
+    xp(1) = c_loc(xs)
+    call check(c_scalar_tendency(ctx, dt0, c_loc(u), c_loc(v), c_loc(w), 1_c_int, c_loc(qp), c_loc(xp), &
+                                 c_loc(af), ior(7_c_int, MIMADV_HOST_PTRS)), 'advection_ac')
+  end subroutine
+
+  ! qnadv_uvw (windadv.f90:46) on arrays: (ua, va, wa) OVERWRITTEN with
+  ! +div of the momenta (mu, mv, mw) advected by (ut, vt, wt)
+  subroutine qnadv_uvw_arrays(ntime, ua, va, wa, mu, mv, mw, ut, vt, wt)
+    integer, intent(in) :: ntime
+    real(8), target, intent(inout) :: ua(g_nx, g_ny, g_nz), va(g_nx, g_ny, g_nz), wa(g_nx, g_ny, g_nz)
+    real(8), target, intent(in) :: mu(g_nx, g_ny, g_nz), mv(g_nx, g_ny, g_nz), mw(g_nx, g_ny, g_nz)
+    real(8), target, intent(in) :: ut(g_nx, g_ny, g_nz), vt(g_nx, g_ny, g_nz), wt(g_nx, g_ny, g_nz)
+    call check(c_momentum_tendency(ctx, int(ntime, c_int), c_loc(mu), c_loc(mv), c_loc(mw), c_loc(ut), &
+                                   c_loc(vt), c_loc(wt), c_loc(ua), c_loc(va), c_loc(wa), MIMADV_HOST_PTRS), &
+               'qnadv_uvw')
+  end subroutine
+
+end module mimadv_advection

@@ -159,6 +159,43 @@ int mimadv_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const
                    const double *v, const double *w, int nf, double *const *q, const int *adv_form,
                    unsigned flags);
 
+/* ---- batched mixed-precision path (no reference counterpart: the
+ * reference advects field by field in FP64) ---- */
+#define MIMADV_F64 0
+#define MIMADV_F32 1
+#define MIMADV_GRAPH 32u /* capture one step in a hipGraph and replay it */
+
+/* Tuning of the batched launches; 0 selects the default of each field. */
+typedef struct mimadv_tuning {
+    int group32;        /* FP32 fields per launch (default 8) */
+    int kchunk;         /* levels per work item (default: whole column) */
+    int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
+} mimadv_tuning;
+int mimadv_set_tuning(mimadv_ctx *ctx, const mimadv_tuning *t);
+
+/* Tendency xs_f = -div(q_f) of nf fields sharing one FP32 velocity, with
+ * the context's scheme (MIMADV_WENO or MIMADV_MUSCL) in the batched kernel:
+ * FP32 reconstruction for MIMADV_F32 fields, FP64 for MIMADV_F64 ones (at
+ * most 2), flux divergence accumulated in FP64.  Device arrays; xs_f has the
+ * precision of q_f.  Lateral boundaries periodic, vertical no-flux.  flags
+ * may restrict the directions (MIMADV_DIR_*; none given = all three).
+ * Replaces, for a whole batch, the per-field loops of advect /
+ * qnadv_scal_{x,y,z} (advection.f90:84-553) with flux-form fields. */
+int mimadv_batch_tendency(mimadv_ctx *ctx, const float *u, const float *v, const float *w, int nf,
+                          const int *precision, const void *const *q, void *const *xs, unsigned flags);
+
+/* Advance nf fields by nsteps Shu-Osher steps (MIMADV_INT_SSPRK3, or
+ * MIMADV_INT_HEUN as its two-stage SSP form q1 = q + dt L(q),
+ * q+ = q/2 + (q1 + dt L(q1))/2), every stage one batched launch per field
+ * group with the periodic ghost refresh fused in.  flags may contain
+ * MIMADV_GRAPH. */
+int mimadv_batch_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const float *u, const float *v,
+                         const float *w, int nf, const int *precision, void *const *q, unsigned flags);
+
+/* Domain budgets: dzw-weighted interior sums of nf fields, accumulated in
+ * FP64 (out[f]). */
+int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const void *const *q, double *out);
+
 /* Kernel time of the last mimadv_advance / mimadv_scalar_tendency call,
  * measured with HIP events on the context stream (ms). */
 int mimadv_last_kernel_ms(mimadv_ctx *ctx, double *ms);

@@ -1,0 +1,147 @@
+"""The batched mixed-precision path against the FP64 oracle.
+
+FP64 members use reciprocals and a reduced operation count, so they agree
+with the oracle to round-off (1e-10 norm-wise); FP32 members are held to the
+1e-5 relative RMS bound of the mixed-precision design.  Inputs of the oracle
+are the same FP32-rounded velocities and fields.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fdd3260_amd import Advection, _lib, cases
+from fdd3260_amd import grid as gm
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(maxx=40, maxy=36, nz=24, nspec=5, seed=31):
+    g = gm.stretched(maxx, maxy, nz)
+    u, v, w = cases.turbulent_wind(g, seed=seed, rms=3.0)
+    u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
+    th = cases.theta_profile(g, seed=seed + 1)
+    qv = cases.lognormal(g, seed=seed + 2, mu=-5.0, sigma=0.3)
+    specs = []
+    for s in range(nspec):
+        q = cases.lognormal(g, seed=seed + 10 + s, mu=-20 + 14 * s / max(1, nspec - 1), sigma=1.5, zero_frac=0.3)
+        specs.append(q.astype(np.float32).astype(np.float64))
+    return g, u, v, w, [th, qv], specs
+
+
+def _rel_rms(got, ref, g):
+    I = g.interior()
+    d = (got - ref)[I]
+    return float(np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(ref[I] ** 2)), 1e-300))
+
+
+def _relmax(got, ref, g):
+    I = g.interior()
+    return float(np.abs(got - ref)[I].max() / max(np.abs(ref[I]).max(), 1e-300))
+
+
+@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+@pytest.mark.parametrize("group32", [8, 2])
+def test_batch_tendency(cuda, scheme, group32):
+    print(f"case batch_tendency {scheme} group32={group32}", flush=True)
+    g, u, v, w, f64, f32 = _case()
+    ora = Oracle(g, scal_adv=scheme, muscllimiter="MC")
+    refs = []
+    for q in f64 + f32:
+        xs = np.zeros(g.shape)
+        ora.scalar_tendency(0.0, u, v, w, q, xs)
+        refs.append(xs)
+    with Advection(g, scal_adv=scheme, muscllimiter="MC") as a:
+        a.set_tuning(group32=group32)
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
+                                                               for q in f32]
+        outs = [torch.zeros_like(f) for f in fields]
+        a.batch_tendency(du, dv, dw, fields, outs)
+        a.stream_sync()
+        for n, (o, r) in enumerate(zip(outs, refs)):
+            got = o.double().cpu().numpy()
+            if n < len(f64):
+                assert _relmax(got, r, g) <= 1e-10, f"fp64 field {n}"
+            else:
+                assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
+
+
+@pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
+def test_batch_advance_and_graph(cuda, integrator):
+    print(f"case batch_advance {integrator}", flush=True)
+    g, u, v, w, f64, f32 = _case(nspec=4)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    nsteps = 3
+    ora = Oracle(g, scal_adv="weno")
+    refs = [q.copy() for q in f64 + f32]
+    for _ in range(nsteps):
+        for r in refs:
+            if integrator == "ssprk3":
+                ora.ssprk3_step(dt, u, v, w, r)
+            else:
+                # two-stage Shu-Osher form of Heun (equal to integ.h's in exact arithmetic)
+                q0 = r.copy()
+                a = np.zeros(g.shape)
+                ora.scalar_tendency(dt, u, v, w, r, a)
+                q1 = q0 + dt * a
+                ora.fill_scalar(q1)
+                a = np.zeros(g.shape)
+                ora.scalar_tendency(dt, u, v, w, q1, a)
+                r[...] = 0.5 * q0 + 0.5 * (q1 + dt * a)
+                ora.fill_scalar(r)
+    code = _lib.INT_SSPRK3 if integrator == "ssprk3" else _lib.INT_HEUN
+    results = []
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        for graph in (False, True):
+            fields = [torch.from_numpy(q).to(cuda) for q in f64] + [
+                torch.from_numpy(q.astype(np.float32)).to(cuda) for q in f32]
+            b0 = a.batch_budget(fields)
+            a.batch_advance(code, nsteps, dt, du, dv, dw, fields, graph=graph)
+            a.stream_sync()
+            b1 = a.batch_budget(fields)
+            results.append([f.double().cpu().numpy() for f in fields])
+            for n, (f, r) in enumerate(zip(results[-1], refs)):
+                if n < len(f64):
+                    assert _relmax(f, r, g) <= 1e-9, f"fp64 field {n}"
+                    assert abs(b1[n] - b0[n]) <= 1e-12 * abs(b0[n])
+                else:
+                    assert _rel_rms(f, r, g) <= 1e-5, f"fp32 field {n}"
+                    assert abs(b1[n] - b0[n]) <= 1e-5 * abs(b0[n])
+    for x, y in zip(*results):
+        assert np.array_equal(x, y), "graph replay differs from eager launches"
+
+
+def test_batch_rejects_unsupported(cuda):
+    g = gm.stretched(20, 16, 12, bcl=("per", "per", "per", "per", "ope", "nnf"))
+    with Advection(g, scal_adv="weno") as a:
+        t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
+        with pytest.raises(Exception, match="nnf"):
+            a.batch_tendency(t, t, t, [t], [t.clone()])
+    g = gm.stretched(20, 16, 12)
+    with Advection(g, scal_adv="ppm") as a:
+        t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
+        with pytest.raises(Exception, match="WENO and MUSCL"):
+            a.batch_tendency(t, t, t, [t], [t.clone()])
+
+
+@pytest.mark.parametrize("dirs", [1, 2, 4, 7])
+def test_batch_tendency_per_direction(cuda, dirs):
+    g, u, v, w, f64, f32 = _case(nspec=4)
+    ora = Oracle(g, scal_adv="weno")
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
+                                                               for q in f32]
+        outs = [torch.zeros_like(f) for f in fields]
+        a.batch_tendency(du, dv, dw, fields, outs, dirs=dirs)
+        a.stream_sync()
+        for n, (q, o) in enumerate(zip(f64 + f32, outs)):
+            ref = np.zeros(g.shape)
+            ora.scalar_tendency(0.0, u, v, w, q, ref, False, dirs)
+            got = o.double().cpu().numpy()
+            rr, rm = _rel_rms(got, ref, g), _relmax(got, ref, g)
+            print(f"dirs={dirs} field {n} ({'fp64' if n < 2 else 'fp32'}): rel rms {rr:.3e} rel max {rm:.3e}",
+                  flush=True)
+            assert rr <= (1e-10 if n < 2 else 1e-5)
