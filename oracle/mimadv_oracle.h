@@ -19,6 +19,13 @@
  * plus two schemes the reference does not have (WENO5-JS reconstruction and
  * SSP-RK3 stage coefficients), which are pinned only by closed forms.
  *
+ * Pins: every routine is checked against closed forms, invariants or
+ * hand-derived known answers in tests/test_oracle_pins.py, except
+ *   orc_fill_winds               -- parity unpinned (not on the measured path);
+ *   advsz_ppm on a stretched grid -- parity partly unpinned (its face
+ *                                   formula is pinned only in the uniform
+ *                                   limit and by conservation).
+ *
  * Arrays are Fortran column-major with x fastest: element (i,j,k), 1-based,
  * lives at [(i-1) + nx*((j-1) + ny*(k-1))].  nx = MAXX+5, ny = MAXY+5 (or 1
  * for a 2D run), nz = MAXZ; interior columns are i = 4..nx-2, j = 4..ny-2.

@@ -497,3 +497,80 @@ def test_weno5_convergence():
         exact = (np.sin(2 * np.pi * (t + d / 2)) - np.sin(2 * np.pi * (t - d / 2))) / d
         errs.append(np.abs(xa - exact).mean())
     assert np.log2(errs[0] / errs[1]) >= 4.5
+
+
+# ------------------------------------------------------ closures (extra) -----
+
+def test_ppm_vertical_uniform_grid_quadratic_exact():
+    """On a uniform grid the non-uniform face formula of advsz_ppm
+    (advq_ppm.f90:379-383) reduces to the 4th-order one, so for cell
+    averages of a quadratic the core vertical fluxes are the exact swept
+    averages, as in x."""
+    g = gm.uniform(10, 0, 40, dz=1.0)
+    al, be, ga = 5.0, 1.0, 0.01
+    k = np.arange(1, g.nz + 1, dtype=float)
+    x = al + be * k + ga * (k ** 2 + 1.0 / 12.0)
+    Q = lambda t: al * t + be * t ** 2 / 2 + ga * t ** 3 / 3
+    rng = np.random.default_rng(12)
+    w = rng.uniform(-1, 1, g.nz)
+    dt0 = 0.3
+    ora = Oracle(g, scal_adv="ppm")
+    xa = ora.line_z(dt0, x, w)
+
+    def F(f):  # face f+1/2, velocity w(f+1)
+        wf = w[f]
+        mu = abs(wf) * dt0  # dz1 = 1
+        tf = f + 0.5
+        return wf * ((Q(tf) - Q(tf - mu)) / mu if wf > 0 else (Q(tf + mu) - Q(tf)) / mu)
+
+    for c in range(6, g.nz - 4):
+        assert xa[c - 1] == pytest.approx(F(c) - F(c - 1), rel=1e-10, abs=1e-12)
+
+
+def test_ppm_nonperiodic_right_edge_fixup():
+    """bcl(2) /= 'per' (advq_ppm.f90:143-146): Flux_r(n-2) := Flux_l(n-2), so
+    for linear data (limiter off, dt0 = 0) the last interior face carries
+    u * (x(n-2) - 1/2 slope) instead of the face value."""
+    n = 30
+    x = 2.0 + 0.1 * np.arange(1, n + 1)
+    ora = _line_oracle("ppm")
+    u = np.zeros(n)
+    f = n - 2
+    u[f] = 1.0  # u(n-1): face (n-2)+1/2
+    per = ora.line_h(0.0, x, u, 1.0, "per", "per")[f - 1]
+    nnf = ora.line_h(0.0, x, u, 1.0, "per", "nnf")[f - 1]
+    assert per == pytest.approx(x[f - 1] + 0.05, rel=1e-14)
+    assert nnf == pytest.approx(x[f - 1] - 0.05, rel=1e-14)
+
+
+@pytest.mark.parametrize("bc", ["nnf", "ope"])
+def test_momentum_vertical_closures_constant_fields(bc):
+    """Constant momentum and velocity: every interior flux equals F = c m, so
+    the closures give closed forms: u at k = 1 (nnf) F dz1w(1), at k = 2
+    -c2 F / (dzw(1)+dzw(2)+dzw(3)) (nnf) or 0 (ope), at k = nz -F dz1w(nz)
+    (nnf) or 0 (ope), at k = nz-1 +c2 F / (dzw(nz-2)+dzw(nz-1)+dzw(nz))
+    (nnf: the wide stencil reaches the zero lid flux) or 0 (ope); w at
+    k = nz -F dz1(nz-1) (nnf)."""
+    g = gm.stretched(12, 10, 14, bcl=("per", "per", "per", "per", bc, bc))
+    cm, cu = 1.3, 0.7
+    F = cm * cu
+    mu = np.full(g.shape, cm)
+    ut = np.full(g.shape, cu)
+    zero = np.zeros(g.shape)
+    ua, _, _ = Oracle(g, mom_ord=4, lavisc=True).qnadv_uvw(2, mu, zero, zero, zero, zero, ut)
+    col = ua[:, 5, 5]
+    if bc == "nnf":
+        assert col[0] == pytest.approx(F * g.dz1w[0], rel=1e-14)
+        assert col[1] == pytest.approx(-(1.0 / 8.0) * F / (g.dzw[0] + g.dzw[1] + g.dzw[2]), rel=1e-13)
+        assert col[-1] == pytest.approx(-F * g.dz1w[-1], rel=1e-14)
+    else:
+        assert np.abs(col[[0, 1, -2, -1]]).max() <= 1e-14 * F
+    if bc == "nnf":  # the core line at k = nz-1 reaches the zero lid flux
+        assert col[-2] == pytest.approx((1.0 / 8.0) * F / (g.dzw[-3] + g.dzw[-2] + g.dzw[-1]), rel=1e-13)
+    assert np.abs(col[2:-2]).max() <= 1e-14 * F
+    mw = np.full(g.shape, cm)
+    _, _, wa = Oracle(g, mom_ord=4, lavisc=True).qnadv_uvw(2, zero, zero, mw, zero, zero, ut)
+    wcol = wa[:, 5, 5]
+    assert wcol[0] == 0.0
+    if bc == "nnf":
+        assert wcol[-1] == pytest.approx(-F * g.dz1[-2], rel=1e-14)
