@@ -1196,3 +1196,136 @@ double orc_totalav(const orc_grid *g, const double *x)
             }
     return s / (double)n;
 }
+
+/* ======================================================================
+ * Tangent-linear model of the reference WENO operator (flux form),
+ * forward mode with the branch decisions of the value computation.
+ * One-sided choices at the kinks (documented conventions shared with the
+ * GPU adjoint):
+ *   - min() of the limiter candidates: derivative of the candidate the
+ *     value computation selects (ties: the later argument, as dmin);
+ *   - phi = min(|r|, 1): the clamped branch (|r| >= 1) has zero derivative;
+ *     |r| uses sign(r) with sign(0) = +1;
+ *   - the ratio-20 weight switch: derivative of the branch taken;
+ *   - the face velocity kink at u = 0: the u >= 0 side (d|u| = du).
+ * ====================================================================== */
+
+typedef struct { double v, d; } dual;
+
+static dual dd_(double v, double d) { dual r = {v, d}; return r; }
+static dual dadd(dual a, dual b) { return dd_(a.v + b.v, a.d + b.d); }
+static dual dsub(dual a, dual b) { return dd_(a.v - b.v, a.d - b.d); }
+static dual dmul(dual a, dual b) { return dd_(a.v * b.v, a.d * b.v + a.v * b.d); }
+static dual dscale(double s, dual a) { return dd_(s * a.v, s * a.d); }
+static dual ddiv(dual a, dual b) { double q = a.v / b.v; return dd_(q, (a.d - q * b.d) / b.v); }
+static dual dabs_(dual a) { return a.v < 0.0 ? dd_(-a.v, -a.d) : a; }
+static dual dminl(dual a, dual b) { return a.v < b.v ? a : b; }
+static dual dsq(dual a) { return dmul(a, a); }
+
+static void tl_weights(dual a, dual b, dual c, dual d, dual ww[4])
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15, thres = 20.0;
+    dual e1 = dsq(dsub(b, a)), e2 = dsq(dsub(c, b)), e3 = dsq(dsub(d, c));
+    dual epsd = dd_(eps, 0.0);
+    ww[0] = dd_(g1, 0); ww[1] = dd_(g2, 0); ww[2] = dd_(g2, 0); ww[3] = dd_(g1, 0);
+    if (dmax(e1.v, e2.v) / (dmin(e1.v, e2.v) + eps) > thres) {
+        dual w1 = ddiv(dd_(g1, 0), dsq(dadd(epsd, e1)));
+        dual w2 = ddiv(dd_(g2, 0), dsq(dadd(epsd, e2)));
+        dual s = dadd(w1, w2);
+        ww[0] = ddiv(w1, s);
+        ww[1] = ddiv(w2, s);
+    }
+    if (dmax(e2.v, e3.v) / (dmin(e2.v, e3.v) + eps) > thres) {
+        dual w3 = ddiv(dd_(g2, 0), dsq(dadd(epsd, e2)));
+        dual w4 = ddiv(dd_(g1, 0), dsq(dadd(epsd, e3)));
+        dual s = dadd(w3, w4);
+        ww[2] = ddiv(w3, s);
+        ww[3] = ddiv(w4, s);
+    }
+}
+
+/* face flux and its derivative; kind: 0 loop body, 1 bottom closure,
+ * 2 top closure; nonlin: weights switch allowed */
+static dual tl_face(dual a, dual b, dual c, dual d, dual u, int kind, int nonlin)
+{
+    const double eps = 1.e-15, g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
+    dual ffim1 = dadd(dscale(-0.5, a), dscale(1.5, b));
+    dual ffip1 = dadd(dscale(0.5, b), dscale(0.5, c));
+    dual ffip2 = dsub(dscale(1.5, c), dscale(0.5, d));
+    dual fl, fr;
+    if (kind == 2) {
+        fl = dadd(dscale(g1, ffim1), dscale(g2, ffip1));
+        fr = ffip1;
+    } else if (kind == 1) {
+        fl = ffip1;
+        fr = dadd(dscale(g2, ffip1), dscale(g1, ffip2));
+    } else {
+        dual ww[4] = {dd_(g1, 0), dd_(g2, 0), dd_(g2, 0), dd_(g1, 0)};
+        if (nonlin) tl_weights(a, b, c, d, ww);
+        dual minp = dminl(dminl(dminl(ffim1, ffip1), ffip1), ffip2);
+        dual r = ddiv(dsub(dd_(0, 0), b), dadd(dsub(minp, b), dd_(eps, 0)));
+        dual phi = dabs_(r);
+        if (!(phi.v < 1.0)) phi = dd_(1.0, 0.0);
+        dual f1 = dadd(dmul(phi, dsub(ffim1, b)), b);
+        dual f2 = dadd(dmul(phi, dsub(ffip1, b)), b);
+        dual f4 = dadd(dmul(phi, dsub(ffip2, b)), b);
+        fl = dadd(dmul(ww[0], f1), dmul(ww[1], f2));
+        fr = dadd(dmul(ww[2], f2), dmul(ww[3], f4));
+    }
+    /* 0.5 fl (u + |u|) + 0.5 fr (u - |u|) with d|u| = du at u = 0 */
+    dual au = u.v < 0.0 ? dd_(-u.v, -u.d) : u;
+    return dadd(dmul(dscale(0.5, fl), dadd(u, au)), dmul(dscale(0.5, fr), dsub(u, au)));
+}
+
+void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
+                           const double *v, const double *w, const double *du, const double *dv,
+                           const double *dw, double *xs, double *dxs)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    int three_d = ny > 1;
+    int jt0 = three_d ? 4 : 1, jt1 = three_d ? ny - 2 : 1;
+    double dx1 = 1.0 / g->dx, dy1 = 1.0 / g->dy;
+#define DQ(i, j, k) dd_(q[IDX(g, i, j, k)], dq[IDX(g, i, j, k)])
+#pragma omp parallel for collapse(2) num_threads(g_threads) schedule(static)
+    for (int k = 1; k <= nz; ++k)
+        for (int j = jt0; j <= jt1; ++j)
+            for (int i = 4; i <= nx - 2; ++i) {
+                size_t id = IDX(g, i, j, k);
+                dual t = dd_(0.0, 0.0);
+                /* x */
+                {
+                    dual fp = tl_face(DQ(i - 1, j, k), DQ(i, j, k), DQ(i + 1, j, k), DQ(i + 2, j, k),
+                                      dd_(u[IDX(g, i + 1, j, k)], du[IDX(g, i + 1, j, k)]), 0, 1);
+                    dual fm = tl_face(DQ(i - 2, j, k), DQ(i - 1, j, k), DQ(i, j, k), DQ(i + 1, j, k),
+                                      dd_(u[id], du[id]), 0, 1);
+                    t = dsub(t, dscale(dx1, dsub(fp, fm)));
+                }
+                if (three_d) {
+                    dual fp = tl_face(DQ(i, j - 1, k), DQ(i, j, k), DQ(i, j + 1, k), DQ(i, j + 2, k),
+                                      dd_(v[IDX(g, i, j + 1, k)], dv[IDX(g, i, j + 1, k)]), 0, 1);
+                    dual fm = tl_face(DQ(i, j - 2, k), DQ(i, j - 1, k), DQ(i, j, k), DQ(i, j + 1, k),
+                                      dd_(v[id], dv[id]), 0, 1);
+                    t = dsub(t, dscale(dy1, dsub(fp, fm)));
+                }
+                /* z: faces f = 1..nz-1 with the closures; F(0) = F(nz) = 0 (nnf) */
+                {
+                    dual fz[2];
+                    for (int s = 0; s < 2; ++s) {
+                        int f = k - 1 + s; /* face f+1/2 */
+                        if (f < 1 || f > nz - 1) { fz[s] = dd_(0.0, 0.0); continue; }
+                        dual wf = dd_(w[IDX(g, i, j, f + 1)], dw[IDX(g, i, j, f + 1)]);
+                        if (f == nz - 1) {
+                            fz[s] = tl_face(DQ(i, j, f - 1), DQ(i, j, f), DQ(i, j, f + 1), dd_(0, 0), wf, 2, 0);
+                        } else {
+                            dual a = f == 1 ? DQ(i, j, 1) : DQ(i, j, f - 1);
+                            fz[s] = tl_face(a, DQ(i, j, f), DQ(i, j, f + 1), DQ(i, j, f + 2), wf, 0,
+                                            f >= 3 && f <= nz - 3);
+                        }
+                    }
+                    t = dsub(t, dscale(L(g->dz1w, k), dsub(fz[1], fz[0])));
+                }
+                xs[id] = t.v;
+                dxs[id] = t.d;
+            }
+#undef DQ
+}

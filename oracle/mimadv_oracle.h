@@ -104,6 +104,15 @@ void orc_integ_scal(long n, int sorder, int im, double dt, const double *xold,
 /* totalav: unweighted mean over interior columns, all levels. */
 double orc_totalav(const orc_grid *g, const double *x);
 
+/* Tangent-linear of the reference 'weno' tendency in flux form with
+ * periodic sides (ghosts filled as periodic images) and 'nnf' floor/lid:
+ * xs = -div q, dxs = its derivative along (dq, du, dv, dw); interior cells
+ * only (x(0) := x(1) convention at the floor).  Kink conventions are listed
+ * in mimadv_oracle.c. */
+void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
+                           const double *v, const double *w, const double *du, const double *dv,
+                           const double *dw, double *xs, double *dxs);
+
 /* OpenMP thread count used by the 3D drivers (1 = serial).  Returns the
  * previous value. */
 int orc_set_threads(int n);

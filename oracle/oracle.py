@@ -68,6 +68,7 @@ def load():
     lib.orc_integ_scal.argtypes = [C.c_long, i, i, d, dp, dp, dp, dp]
     lib.orc_totalav.argtypes = [G, dp]
     lib.orc_totalav.restype = d
+    lib.orc_tlm_weno_tendency.argtypes = [G, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.orc_set_threads.argtypes = [i]
     lib.orc_set_threads.restype = i
     _lib = lib
@@ -145,6 +146,13 @@ class Oracle:
         self.lib.orc_qnadv_uvw(C.byref(self.g), C.byref(self.o), int(ntime), _dp(mu), _dp(mv), _dp(mw), _dp(ut),
                                _dp(vt), _dp(wt), _dp(ua), _dp(va), _dp(wa))
         return ua, va, wa
+
+    def tlm_weno_tendency(self, q, dq, u, v, w, du, dv, dw):
+        """Tangent-linear of the 'weno' flux-form tendency: (xs, dxs)."""
+        xs, dxs = np.zeros_like(q), np.zeros_like(q)
+        self.lib.orc_tlm_weno_tendency(C.byref(self.g), _dp(q), _dp(dq), _dp(u), _dp(v), _dp(w), _dp(du), _dp(dv),
+                                       _dp(dw), _dp(xs), _dp(dxs))
+        return xs, dxs
 
     def fill_scalar(self, x):
         self.lib.orc_fill_scalar(C.byref(self.g), _dp(x))
