@@ -151,6 +151,137 @@ def fused(args):
     print(json.dumps(out), flush=True)
 
 
+def adjoint(args):
+    """Trajectory adjoint: J = sum over the top half of rho0*q(T)^2 after
+    `steps` SSP-RK3 'weno' steps of the log-normal scalar of the plain
+    workload; gradient with respect to (q0, u, v, w) in FP64, checkpoints
+    every `checkpoint` steps.  value = wall(adjoint incl. recomputation) /
+    wall(forward).  Central differences along seeded directions check the
+    gradient at full size (the step is cut by decades while it straddles a
+    jump of the scheme)."""
+    import numpy as np
+    import torch
+
+    from fdd3260_amd import Advection, _lib, cases
+
+    maxx, maxy, nz = (int(s) for s in args.size.split(","))
+    t0 = time.time()
+    g, u, v, w, fields, dt = build_case(maxx, maxy, nz, seed=2, rms=2.0)
+    q0 = fields[0]
+    dev = torch.device("cuda:0")
+    U, V, W, Q0 = (torch.from_numpy(a).to(dev) for a in (u, v, w, q0))
+    rho0 = 1.2 * np.exp(-g.zc() / 8000.0)
+    top = np.zeros(g.shape)
+    I = g.interior()
+    top[nz // 2:] = rho0[nz // 2:, None, None]
+    mask = np.zeros(g.shape)
+    mask[I] = 1.0
+    TOP = torch.from_numpy(top * mask).to(dev)
+    log(f"adjoint case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
+    adv = Advection(g, scal_adv="weno")
+    adv.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    code, nsteps, ck = _lib.INT_SSPRK3, args.steps, args.checkpoint
+
+    def forward(q0_, u_, v_, w_):
+        q = q0_.clone()
+        adv.advance(code, nsteps, dt, u_, v_, w_, [q])
+        return q
+
+    def J(*a):
+        q = forward(*a)
+        return float((TOP * q * q).sum())
+
+    def gradient():
+        qT = forward(Q0, U, V, W)
+        qb = (2.0 * TOP * qT).contiguous()
+        ub, vb, wb = (torch.zeros_like(Q0) for _ in range(3))
+        adv.adjoint_advance(code, nsteps, dt, U, V, W, Q0, qb, ub, vb, wb, checkpoint=ck)
+        return qb, ub, vb, wb
+
+    for _ in range(max(1, args.warmup // 5)):
+        forward(Q0, U, V, W)
+        gradient()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    forward(Q0, U, V, W)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - ts
+    # the adjoint run alone (the forward trajectory is recomputed inside it)
+    qT = forward(Q0, U, V, W)
+    qb = (2.0 * TOP * qT).contiguous()
+    ub, vb, wb = (torch.zeros_like(Q0) for _ in range(3))
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    adv.adjoint_advance(code, nsteps, dt, U, V, W, Q0, qb, ub, vb, wb, checkpoint=ck)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - ts
+    grads = (qb, ub, vb, wb)
+    # the stage-adjoint kernel alone, timed with HIP events on the library stream
+    lam = (2.0 * TOP * qT).contiguous()
+    o = [torch.zeros_like(Q0) for _ in range(4)]
+    reps, kt = 20, 0.0
+    for _ in range(reps):
+        adv.adjoint_tendency(U, V, W, qT, lam, *o)
+        kt += adv.last_kernel_ms()
+    adj_kernel_s = kt * 1e-3 / reps
+    ncell = maxx * max(maxy, 1) * nz
+    # algorithmic bytes of one stage adjoint per cell: q, adjoint in, 3 winds,
+    # adjoint out, the q0 accumulator read+write, 3 wind adjoints read+write
+    adj_bytes = (8 + 8 + 24 + 8 + 16 + 48) * ncell
+    achieved = adj_bytes / adj_kernel_s / 1e9
+    # central differences along seeded directions
+    rng = np.random.default_rng(2024)
+    fd_rows = []
+    with torch.no_grad():
+        j0 = J(Q0, U, V, W)
+        for n in range(args.fd_dirs):
+            d = [np.zeros(g.shape)] * 4
+            x = cases.fill_periodic(g, rng.standard_normal(g.shape))
+            if n % 4 == 3:
+                x[0] = 0.0
+            d[n % 4] = x
+            D = [torch.from_numpy(a).to(dev) for a in d]
+            ad = float(sum((gr * y).sum() for gr, y in zip(grads, D)))
+            row = {"wrt": "q0 u v w".split()[n % 4], "adjoint": ad, "fd": None, "eps": None}
+            for eps in (1e-6, 1e-7, 1e-8):
+                jp = J(*(a + eps * b for a, b in zip((Q0, U, V, W), D)))
+                jm = J(*(a - eps * b for a, b in zip((Q0, U, V, W), D)))
+                if abs((jp - j0) - (j0 - jm)) / eps <= 1e-4 * abs(ad):
+                    row.update(fd=(jp - jm) / (2 * eps), eps=eps)
+                    break
+            if row["fd"] is not None:
+                row["rel_err"] = abs(row["fd"] - ad) / abs(ad)
+            fd_rows.append(row)
+            log(f"fd direction {n}: {row}")
+    nseg = -(-nsteps // ck) if 0 < ck < nsteps else 1
+    kk = ck if 0 < ck < nsteps else nsteps
+    out = {
+        "metric": "adjoint/forward cost ratio",
+        "value": t_adj / t_fwd,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": nsteps,
+        "warmup": args.warmup,
+        "ms_per_step": t_adj * 1e3 / nsteps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded turbulent wind, log-normal scalar)",
+        "config": {"workload": f"trajectory adjoint {maxx}x{max(maxy, 1)}x{nz}, weno, ssprk3, {nsteps} steps, "
+                               f"checkpoint every {ck or nsteps} steps, J = sum(top half rho0 q(T)^2), "
+                               f"gradient wrt (q0, u, v, w)",
+                   "cells": ncell, "forward_s": t_fwd, "adjoint_s": t_adj, "dt_s": dt,
+                   "device_arrays": 3 + nseg + 3 * kk},
+        "roofline": {"bound": "hbm", "kernel": "adjoint_kernel<64,8> (one stage adjoint)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": adj_bytes, "avg_launch_us": adj_kernel_s * 1e6},
+        "fd_check": fd_rows,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +293,9 @@ def main():
     ap.add_argument("--size", default="256,256,128")
     ap.add_argument("--cpu-steps", type=int, default=6, help="Heun steps of the CPU-baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (at most 16)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "fused"])
+    ap.add_argument("--workload", default="plain", choices=["plain", "fused", "adjoint"])
+    ap.add_argument("--checkpoint", type=int, default=20, help="adjoint workload: steps between checkpoints")
+    ap.add_argument("--fd-dirs", type=int, default=4, help="adjoint workload: finite-difference directions")
     ap.add_argument("--group32", type=int, default=0)
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
@@ -180,6 +313,8 @@ def main():
 
     if args.workload == "fused":
         return fused(args)
+    if args.workload == "adjoint":
+        return adjoint(args)
     maxx, maxy, nz = (int(s) for s in args.size.split(","))
     t0 = time.time()
     g, u, v, w, fields, dt = build_case(maxx, maxy, nz, seed=2, rms=2.0)

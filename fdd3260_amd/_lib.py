@@ -49,6 +49,8 @@ EXPORTED = (
     "mimadv_batch_tendency",
     "mimadv_batch_advance",
     "mimadv_batch_budget",
+    "mimadv_adjoint_tendency",
+    "mimadv_adjoint_advance",
 )
 
 
@@ -138,6 +140,10 @@ def load() -> C.CDLL:
     lib.mimadv_batch_advance.argtypes = [vp, i, i, d, vp, vp, vp, i, ip, pp, u]
     lib.mimadv_batch_advance.restype = i
     lib.mimadv_batch_budget.argtypes = [vp, i, ip, pp, C.POINTER(d)]
+    lib.mimadv_adjoint_tendency.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.mimadv_adjoint_tendency.restype = i
+    lib.mimadv_adjoint_advance.argtypes = [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, i]
+    lib.mimadv_adjoint_advance.restype = i
     lib.mimadv_batch_budget.restype = i
     _lib = lib
     return lib

@@ -280,3 +280,22 @@ class Advection:
         out = (C.c_double * max(1, len(fields)))()
         self._check(self.lib.mimadv_batch_budget(self._h, len(fields), pr, pq, out), "mimadv_batch_budget")
         return [out[n] for n in range(len(fields))]
+
+    # -- adjoint of the plain FP64 'weno' path --------------------------------
+    def adjoint_tendency(self, u, v, w, q, lam, qbar, ubar, vbar, wbar):
+        """qbar = (dL/dq)^T lam (interior cells); ubar, vbar, wbar += (dL/d(u,v,w))^T lam."""
+        ptrs, hflag = self._ptrs([u, v, w, q, lam, qbar, ubar, vbar, wbar], writable=(qbar, ubar, vbar, wbar))
+        if hflag:
+            raise MimadvError("the adjoint works on device tensors")
+        rc = self.lib.mimadv_adjoint_tendency(self._h, *ptrs)
+        self._check(rc, "mimadv_adjoint_tendency")
+
+    def adjoint_advance(self, integrator, nsteps, dt, u, v, w, q0, qbar, ubar, vbar, wbar, checkpoint=0):
+        """Adjoint of advance(integrator, nsteps, dt, u, v, w, [q]) from q0: qbar holds dJ/dq(T) on entry and
+        dJ/dq0 on return; ubar, vbar, wbar are accumulated."""
+        ptrs, hflag = self._ptrs([u, v, w, q0, qbar, ubar, vbar, wbar], writable=(qbar, ubar, vbar, wbar))
+        if hflag:
+            raise MimadvError("the adjoint works on device tensors")
+        rc = self.lib.mimadv_adjoint_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1],
+                                             ptrs[2], ptrs[3], ptrs[4], ptrs[5], ptrs[6], ptrs[7], int(checkpoint))
+        self._check(rc, "mimadv_adjoint_advance")

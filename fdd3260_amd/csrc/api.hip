@@ -1,6 +1,7 @@
 // api.hip -- the C ABI of libmimadv (declared in include/mimadv.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,6 +37,9 @@ struct mimadv_ctx {
     mimadv_tuning tuning{0, 0, 0};
     hipGraphExec_t graph_exec = nullptr;
     std::vector<const void *> graph_key;
+    // adjoint: adjoint arrays, checkpoints and stage states
+    double *adj = nullptr;
+    size_t adj_elems = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
@@ -193,6 +197,7 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->stage) hipFree(c->stage);
     if (c->work) hipFree(c->work);
     if (c->bwork) hipFree(c->bwork);
+    if (c->adj) hipFree(c->adj);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -778,6 +783,185 @@ int mimadv_last_kernel_ms(mimadv_ctx *c, double *ms)
         c->timing_pending = false;
     }
     *ms = c->last_ms;
+    return MIMADV_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int check_adjoint(mimadv_ctx *c, const double *u, const double *v, const double *w)
+{
+    if (c->sp.scheme != MIMADV_WENO) return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint implements the WENO scheme");
+    if (c->sp.weno_bottom != MIMADV_WENO_BOTTOM_X0_IS_X1)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint implements the x(0) = x(1) floor convention");
+    for (int b = 0; b < 4; ++b)
+        if (c->g.bcl[b] != MIMADV_BC_PER && !(c->g.ny == 1 && b >= 2))
+            return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint needs periodic lateral boundaries");
+    if (c->g.bcl[4] != MIMADV_BC_NNF || c->g.bcl[5] != MIMADV_BC_NNF)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint needs no-flux ('nnf') floor and lid");
+    if (!u || !w || (c->g.ny > 1 && !v)) return fail(c, MIMADV_ERR_ARG, "NULL velocity");
+    return MIMADV_OK;
+}
+
+AdjArgs adj_base(mimadv_ctx *c, const double *u, const double *v, const double *w, double *ub, double *vb,
+                 double *wb)
+{
+    AdjArgs A{};
+    A.nx = c->g.nx;
+    A.ny = c->g.ny;
+    A.nz = c->g.nz;
+    A.dx1 = 1.0 / c->g.dx;
+    A.dy1 = 1.0 / c->g.dy;
+    A.dz1w = c->g.zm.dz1w;
+    A.u = u;
+    A.v = c->g.ny > 1 ? v : u;
+    A.w = w;
+    A.ub = ub;
+    A.vb = c->g.ny > 1 ? vb : ub;
+    A.wb = wb;
+    A.kchunk = c->tuning.kchunk;
+    return A;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mimadv_adjoint_tendency(mimadv_ctx *c, const double *u, const double *v, const double *w, const double *q,
+                            const double *lam, double *qbar, double *ubar, double *vbar, double *wbar)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_adjoint(c, u, v, w);
+    if (rc) return rc;
+    if (!q || !lam || !qbar || !ubar || !wbar || (c->g.ny > 1 && !vbar))
+        return fail(c, MIMADV_ERR_ARG, "NULL array");
+    if (qbar == lam || qbar == q) return fail(c, MIMADV_ERR_ARG, "qbar must not alias lam or q");
+    AdjArgs A = adj_base(c, u, v, w, ubar, vbar, wbar);
+    A.q = q;
+    A.qbn = lam;
+    A.out = qbar;
+    A.ic = 0.0;
+    A.ls = 1.0;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launch_adjoint_stage(A, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                           const double *w, const double *q0, double *qbar, double *ubar, double *vbar,
+                           double *wbar, int checkpoint)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_adjoint(c, u, v, w);
+    if (rc) return rc;
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "adjoint integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
+    if (nsteps < 0) return fail(c, MIMADV_ERR_ARG, "negative step count");
+    if (!q0 || !qbar || !ubar || !wbar || (c->g.ny > 1 && !vbar)) return fail(c, MIMADV_ERR_ARG, "NULL array");
+    const size_t n3 = c->n3();
+    const int S = integrator == MIMADV_INT_HEUN ? 2 : 3;
+    const int K = (checkpoint <= 0 || checkpoint >= nsteps) ? (nsteps > 0 ? nsteps : 1) : checkpoint;
+    const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
+    // layout: P, T, ACC (adjoint arrays), nseg checkpoints, S*K stage states
+    const size_t need = n3 * ((size_t)3 + nseg + (size_t)S * K);
+    if (need > c->adj_elems) {
+        if (c->adj) hipFree(c->adj);
+        c->adj = nullptr;
+        c->adj_elems = 0;
+        if (hipMalloc(&c->adj, need * sizeof(double)) != hipSuccess)
+            return fail(c, MIMADV_ERR_NOMEM, "adjoint trajectory allocation failed (reduce checkpoint)");
+        c->adj_elems = need;
+    }
+    double *P = c->adj, *T = c->adj + n3, *ACC = c->adj + 2 * n3;
+    double *ck = c->adj + 3 * n3, *st = ck + (size_t)nseg * n3;
+    auto X = [&](int n, int s) { return st + ((size_t)n * S + s) * n3; };  // stage s input of segment step n
+    SchemeParams sp = c->sp;
+    sp.dt0 = dt;
+
+    // one step of the forward, exactly as mimadv_advance takes it, from X(n,0)
+    auto fwd_step = [&](int n, double *qout) -> hipError_t {
+        for (int s = 1; s <= S; ++s) {
+            ScalarArgs a{};
+            a.u = u;
+            a.v = v;
+            a.w = w;
+            a.adv = 0;
+            a.dirmask = 7;
+            a.fill = 1;
+            a.q0 = X(n, 0);
+            a.q = X(n, s - 1);
+            a.qn = s < S ? X(n, s) : qout;
+            if (integrator == MIMADV_INT_HEUN) {
+                a.epi = EPI_HEUN;
+                a.tend = T;
+                a.b_lm = s == 1 ? 0.0 : 1.0;
+                a.adt = s == 1 ? 1.0 * dt : 0.5 * dt;
+            } else {
+                a.epi = EPI_SO;
+                a.dt = dt;
+                a.c0 = s == 1 ? 0.0 : (s == 2 ? 0.75 : 1.0 / 3.0);
+                a.c1 = s == 1 ? 1.0 : (s == 2 ? 0.25 : 2.0 / 3.0);
+            }
+            hipError_t e = launch_scalar_fp64(c->g, sp, a, c->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    // replay segment g from its checkpoint; the state after its last step goes to qlast
+    auto replay = [&](int g, double *qlast) -> hipError_t {
+        int len = std::min(K, nsteps - g * K);
+        hipError_t e = hipMemcpyAsync(X(0, 0), ck + (size_t)g * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c->stream);
+        for (int n = 0; n < len && e == hipSuccess; ++n) e = fwd_step(n, n + 1 < len ? X(n + 1, 0) : qlast);
+        return e;
+    };
+    // Shu-Osher coefficients of the stages (Heun as its two-stage form)
+    const double c0s[3] = {0.0, integrator == MIMADV_INT_HEUN ? 0.5 : 0.75, 1.0 / 3.0};
+    const double c1s[3] = {1.0, integrator == MIMADV_INT_HEUN ? 0.5 : 0.25, 2.0 / 3.0};
+
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->adj, 0, 3 * n3 * sizeof(double), c->stream));
+    HIPCHK(c, hipMemcpyAsync(P, qbar, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (nseg > 0) {
+        HIPCHK(c, hipMemcpyAsync(ck, q0, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (int g = 0; g + 1 < nseg; ++g) HIPCHK(c, replay(g, ck + (size_t)(g + 1) * n3));
+    }
+    AdjArgs base = adj_base(c, u, v, w, ubar, vbar, wbar);
+    for (int g = nseg - 1; g >= 0; --g) {
+        HIPCHK(c, replay(g, ACC));
+        int len = std::min(K, nsteps - g * K);
+        for (int n = len - 1; n >= 0; --n) {
+            // P = adjoint of the step output; walk the stages backwards
+            double *in = P, *tmp = T;
+            for (int s = S - 1; s >= 0; --s) {
+                AdjArgs A = base;
+                A.q = X(n, s);
+                A.qbn = in;
+                A.ic = c1s[s];
+                A.ls = c1s[s] * dt;
+                A.c0 = c0s[s];
+                if (s > 0) {
+                    A.out = tmp;
+                    A.accumulate = 0;
+                    A.acc0 = ACC;
+                    A.acc0_mode = s == S - 1 ? 1 : 2;
+                } else {
+                    A.out = ACC;  // the first stage reads the step input itself (c0 = 0)
+                    A.accumulate = 1;
+                    A.acc0_mode = 0;
+                }
+                HIPCHK(c, launch_adjoint_stage(A, c->stream));
+                if (s > 0) std::swap(in, tmp);
+            }
+            std::swap(P, ACC);  // P: adjoint of the step input
+        }
+    }
+    HIPCHK(c, launch_interior_copy(c->g, P, qbar, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
     return MIMADV_OK;
 }
 

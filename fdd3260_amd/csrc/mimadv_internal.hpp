@@ -40,7 +40,30 @@ struct ScalarArgs {
     int fill;  // write periodic ghost images of every interior value (stage epilogues)
 };
 
-// Launchers (scalar_fp64.hip, momentum_fp64.hip, util.hip).
+// Adjoint of one Shu-Osher stage of the 'weno' flux-form tendency (adjoint.hip):
+//   out (=|+=) ic*qbn + (dL/dq)^T (ls*qbn);  acc0 (=|+=) c0*qbn;
+//   ub/vb/wb += (dL/d(u,v,w))^T (ls*qbn).  Interior cells only.
+struct AdjArgs {
+    int nx, ny, nz;
+    double dx1, dy1;
+    const double *dz1w;  // device metric
+    const double *u, *v, *w;
+    const double *q;    // linearisation state (the stage input)
+    const double *qbn;  // adjoint of the stage output; never aliases out or acc0
+    double *out;
+    double *acc0;
+    double *ub, *vb, *wb;
+    double ic, ls, c0;
+    int accumulate;  // out += instead of out =
+    int acc0_mode;   // 0: untouched, 1: acc0 = c0*qbn, 2: acc0 += c0*qbn
+    int kchunk;      // levels per workgroup (0: default)
+    int ntx, nty, nwork;  // set by the launcher
+};
+
+// Launchers (scalar_fp64.hip, momentum_fp64.hip, util.hip, adjoint.hip).
+hipError_t launch_adjoint_stage(const AdjArgs &a, hipStream_t s);
+// dst = src on interior cells, 0 on ghost cells
+hipError_t launch_interior_copy(const GridDev &g, const double *src, double *dst, hipStream_t s);
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s);
 hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
                                 const double *mv, const double *mw, const double *ut, const double *vt,

@@ -196,6 +196,37 @@ int mimadv_batch_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt,
  * FP64 (out[f]). */
 int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const void *const *q, double *out);
 
+/* ---- adjoint of the plain FP64 'weno' path (no reference counterpart: the
+ * reference has no adjoint or tangent-linear model) ----
+ * Scheme MIMADV_WENO in flux form with the default floor convention
+ * (MIMADV_WENO_BOTTOM_X0_IS_X1), periodic lateral boundaries, 'nnf' floor
+ * and lid.  Fields and winds carry periodic ghost images, as the forward
+ * reads them; derivatives are taken with respect to interior values: the
+ * east/north boundary faces u(nx-1), v(ny-1) are credited to their interior
+ * images u(4), v(4).  One-sided derivatives at the limiter kinks: the
+ * selected candidate of min(), zero on the clamped side of min(|r|,1), the
+ * branch taken at the ratio-20 weight switch, the u >= 0 side at u = 0. */
+
+/* Vector-Jacobian product of the tendency L(q; u, v, w) = -div(q u):
+ * qbar = (dL/dq)^T lam on interior cells (ghost entries are not written);
+ * ubar, vbar, wbar += (dL/d(u, v, w))^T lam.  Device arrays. */
+int mimadv_adjoint_tendency(mimadv_ctx *ctx, const double *u, const double *v, const double *w,
+                            const double *q, const double *lam, double *qbar, double *ubar, double *vbar,
+                            double *wbar);
+
+/* Adjoint of mimadv_advance(integrator, nsteps, dt, u, v, w) of one field
+ * in flux form started from q0.  On entry qbar holds dJ/dq(T) on interior
+ * cells; on return it holds dJ/dq0 (ghost entries zero).  ubar, vbar, wbar
+ * are accumulated.  The trajectory is recomputed from q0 on the device: a
+ * first sweep keeps the state of every `checkpoint`-th step, then each
+ * segment of `checkpoint` steps is replayed with its stage states held in
+ * device memory and swept backwards (checkpoint <= 0 or >= nsteps: one
+ * sweep, every stage state held).  Device memory:
+ * (3 + nsteps/checkpoint + stages*checkpoint) arrays of nx*ny*nz doubles. */
+int mimadv_adjoint_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
+                           const double *v, const double *w, const double *q0, double *qbar, double *ubar,
+                           double *vbar, double *wbar, int checkpoint);
+
 /* Kernel time of the last mimadv_advance / mimadv_scalar_tendency call,
  * measured with HIP events on the context stream (ms). */
 int mimadv_last_kernel_ms(mimadv_ctx *ctx, double *ms);

@@ -1,0 +1,266 @@
+"""The hand-written adjoint of the plain FP64 'weno' path.
+
+* dot-product tests against the oracle's tangent-linear model (forward-mode
+  derivative of the reference scheme, same one-sided conventions at the
+  kinks): <lam, TLM d> = <ADJ lam, d> to 1e-13 of the sum of magnitudes,
+  for one tendency and for whole multi-step RK trajectories;
+* the checkpointed trajectory adjoint is independent of the checkpoint
+  interval, bit for bit (the replays recompute identical states);
+* central finite differences of J = sum over the top half of rho0*q(T)^2
+  through a torch.autograd.Function that wraps the forward and the adjoint
+  (test harness only).
+"""
+import numpy as np
+import pytest
+import torch
+
+from fdd3260_amd import Advection, _lib, cases
+from fdd3260_amd import grid as gm
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+INT = {"ssprk3": _lib.INT_SSPRK3, "heun": _lib.INT_HEUN}
+
+
+def _case(three_d=True, kind="lognormal", seed=61, maxx=24, maxy=20, nz=16):
+    g = gm.stretched(maxx, maxy if three_d else 0, nz)
+    u, v, w = cases.turbulent_wind(g, seed=seed, rms=3.0)
+    if kind == "lognormal":
+        q = cases.lognormal(g, seed=seed + 1, mu=-9.0, sigma=1.5, zero_frac=0.3)
+    else:
+        q = cases.fill_periodic(g, np.random.default_rng(seed + 1).standard_normal(g.shape))
+    return g, u, v, w, q
+
+
+def _dirs(g, rng, qscale):
+    dq = cases.fill_periodic(g, rng.standard_normal(g.shape) * qscale)
+    du, dv, dw = (cases.fill_periodic(g, rng.standard_normal(g.shape)) for _ in range(3))
+    if not g.three_d:
+        dv[...] = 0.0
+    dw[0] = 0.0  # w(1) is the floor, never read
+    return dq, du, dv, dw
+
+
+def _interior_only(g, a):
+    out = np.zeros(g.shape)
+    I = g.interior()
+    out[I] = a[I]
+    return out
+
+
+def _dev(cuda, *arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a)).to(cuda) for a in arrs]
+
+
+def _check_dot(lhs_terms, rhs_terms, tol=1e-13):
+    lhs = sum(float(np.sum(t)) for t in lhs_terms)
+    rhs = sum(float(np.sum(t)) for t in rhs_terms)
+    scale = sum(float(np.sum(np.abs(t))) for t in lhs_terms + rhs_terms)
+    assert abs(lhs - rhs) <= tol * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("three_d", [True, False])
+@pytest.mark.parametrize("kind", ["lognormal", "gaussian"])
+def test_adjoint_tendency_dot_product(cuda, three_d, kind):
+    print(f"case adjoint_tendency three_d={three_d} kind={kind}", flush=True)
+    g, u, v, w, q = _case(three_d, kind)
+    rng = np.random.default_rng(71)
+    dq, du, dv, dw = _dirs(g, rng, np.abs(q).mean())
+    lam = _interior_only(g, rng.standard_normal(g.shape))
+    _, tl = Oracle(g, scal_adv="weno").tlm_weno_tendency(q, dq, u, v, w, du, dv, dw)
+    with Advection(g, scal_adv="weno") as a:
+        U, V, W, Q, L = _dev(cuda, u, v, w, q, lam)
+        qb, ub, vb, wb = (torch.zeros_like(Q) for _ in range(4))
+        a.adjoint_tendency(U, V, W, Q, L, qb, ub, vb, wb)
+        a.stream_sync()
+        qb, ub, vb, wb = (x.cpu().numpy() for x in (qb, ub, vb, wb))
+    I = g.interior()
+    assert np.all(qb[~_mask(g)] == 0.0), "ghost entries of qbar were written"
+    _check_dot([lam[I] * tl[I]], [dq * qb, du * ub, dv * vb, dw * wb])
+
+
+def _mask(g):
+    m = np.zeros(g.shape, dtype=bool)
+    m[g.interior()] = True
+    return m
+
+
+def _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq0, du, dv, dw):
+    """Oracle state and tangent after nsteps, stage states ghost-filled."""
+    q, dq = q0.copy(), dq0.copy()
+    for _ in range(nsteps):
+        if integrator == "ssprk3":
+            cur, dcur = q, dq
+            for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
+                xs, dxs = ora.tlm_weno_tendency(cur, dcur, u, v, w, du, dv, dw)
+                nxt = c0 * q + c1 * (cur + dt * xs)
+                dnxt = c0 * dq + c1 * (dcur + dt * dxs)
+                ora.fill_scalar(nxt)
+                ora.fill_scalar(dnxt)
+                cur, dcur = nxt, dnxt
+            q, dq = cur, dcur
+        else:
+            # integ.h form: q1 = q + dt a1; q+ = q + dt/2 (a1 + a2)
+            a1, da1 = ora.tlm_weno_tendency(q, dq, u, v, w, du, dv, dw)
+            q1, dq1 = q + dt * a1, dq + dt * da1
+            ora.fill_scalar(q1)
+            ora.fill_scalar(dq1)
+            a2, da2 = ora.tlm_weno_tendency(q1, dq1, u, v, w, du, dv, dw)
+            qn, dqn = q + 0.5 * dt * (a1 + a2), dq + 0.5 * dt * (da1 + da2)
+            ora.fill_scalar(qn)
+            ora.fill_scalar(dqn)
+            q, dq = qn, dqn
+    return q, dq
+
+
+@pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
+@pytest.mark.parametrize("checkpoint", [0, 2])
+def test_adjoint_advance_dot_product(cuda, integrator, checkpoint):
+    print(f"case adjoint_advance {integrator} checkpoint={checkpoint}", flush=True)
+    g, u, v, w, q0 = _case(True, "lognormal", seed=81)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    nsteps = 5
+    rng = np.random.default_rng(83)
+    dq, du, dv, dw = _dirs(g, rng, np.abs(q0).mean())
+    lam = _interior_only(g, rng.standard_normal(g.shape))
+    ora = Oracle(g, scal_adv="weno")
+    qT, dqT = _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq, du, dv, dw)
+    with Advection(g, scal_adv="weno") as a:
+        U, V, W, Q0, L = _dev(cuda, u, v, w, q0, lam)
+        # the forward itself agrees with the oracle trajectory
+        Q = Q0.clone()
+        a.advance(INT[integrator], nsteps, dt, U, V, W, [Q])
+        a.stream_sync()
+        I = g.interior()
+        assert np.abs(Q.cpu().numpy()[I] - qT[I]).max() <= 1e-12 * np.abs(qT[I]).max()
+        qb = L.clone()
+        ub, vb, wb = (torch.zeros_like(Q0) for _ in range(3))
+        a.adjoint_advance(INT[integrator], nsteps, dt, U, V, W, Q0, qb, ub, vb, wb, checkpoint=checkpoint)
+        a.stream_sync()
+        qb, ub, vb, wb = (x.cpu().numpy() for x in (qb, ub, vb, wb))
+    assert np.all(qb[~_mask(g)] == 0.0)
+    _check_dot([lam[I] * dqT[I]], [dq * qb, du * ub, dv * vb, dw * wb])
+
+
+def test_adjoint_checkpoint_interval_is_bit_invariant(cuda):
+    print("case adjoint checkpoint invariance", flush=True)
+    g, u, v, w, q0 = _case(True, "gaussian", seed=91, nz=20)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    lam = _interior_only(g, np.random.default_rng(92).standard_normal(g.shape))
+    outs = []
+    with Advection(g, scal_adv="weno") as a:
+        U, V, W, Q0, L = _dev(cuda, u, v, w, q0, lam)
+        for ck in (0, 1, 3, 7):
+            qb = L.clone()
+            ub, vb, wb = (torch.zeros_like(Q0) for _ in range(3))
+            a.adjoint_advance(_lib.INT_SSPRK3, 7, dt, U, V, W, Q0, qb, ub, vb, wb, checkpoint=ck)
+            a.stream_sync()
+            outs.append([x.cpu() for x in (qb, ub, vb, wb)])
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert torch.equal(x, y)
+
+
+def test_adjoint_kchunk_is_invariant(cuda):
+    """Level chunking of the adjoint kernel changes only the work split."""
+    print("case adjoint kchunk invariance", flush=True)
+    g, u, v, w, q = _case(True, "gaussian", seed=95, nz=23)
+    lam = _interior_only(g, np.random.default_rng(96).standard_normal(g.shape))
+    outs = []
+    with Advection(g, scal_adv="weno") as a:
+        U, V, W, Q, L = _dev(cuda, u, v, w, q, lam)
+        for kc in (0, 1, 5, 23):
+            a.set_tuning(kchunk=kc)
+            qb, ub, vb, wb = (torch.zeros_like(Q) for _ in range(4))
+            a.adjoint_tendency(U, V, W, Q, L, qb, ub, vb, wb)
+            a.stream_sync()
+            outs.append([x.cpu() for x in (qb, ub, vb, wb)])
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert torch.equal(x, y)
+
+
+def test_adjoint_rejects_unsupported(cuda):
+    g, u, v, w, q = _case(True, "gaussian")
+    with Advection(g, scal_adv="muscl") as a:
+        U, V, W, Q = _dev(cuda, u, v, w, q)
+        z = [torch.zeros_like(Q) for _ in range(4)]
+        with pytest.raises(_lib.MimadvError):
+            a.adjoint_tendency(U, V, W, Q, Q, *z)
+    with Advection(g, scal_adv="weno", weno_bottom=1) as a:
+        with pytest.raises(_lib.MimadvError):
+            a.adjoint_tendency(U, V, W, Q, Q, *z)
+
+
+class _Advance(torch.autograd.Function):
+    """Differentiable wrapper of Advection.advance for one FP64 field (tests only)."""
+
+    @staticmethod
+    def forward(ctx, q0, u, v, w, adv, integrator, nsteps, dt, checkpoint):
+        q = q0.detach().clone()
+        adv.advance(integrator, nsteps, dt, u.detach(), v.detach(), w.detach(), [q])
+        ctx.save_for_backward(q0, u, v, w)
+        ctx.cfg = (adv, integrator, nsteps, dt, checkpoint)
+        return q
+
+    @staticmethod
+    def backward(ctx, gq):
+        q0, u, v, w = ctx.saved_tensors
+        adv, integrator, nsteps, dt, checkpoint = ctx.cfg
+        qb = gq.detach().contiguous().clone()
+        ub, vb, wb = (torch.zeros_like(x) for x in (u, v, w))
+        adv.adjoint_advance(integrator, nsteps, dt, u.detach(), v.detach(), w.detach(), q0.detach(), qb, ub, vb, wb,
+                            checkpoint)
+        return qb, ub, vb, wb, None, None, None, None, None
+
+
+def test_adjoint_autograd_matches_central_differences(cuda):
+    """J = sum over the top half of rho0*q(T)^2 after 6 RK3 steps; gradient
+    through autograd against central differences along seeded directions in
+    q0, u, v and w.
+
+    The reference scheme is piecewise smooth with jumps (the ratio-20 weight
+    switch, and limiter_weno at zero-valued cells), so a central difference
+    is only meaningful when the segment [-eps, eps] straddles no jump.  Each
+    direction starts at eps = 1e-6 of the data scale and halves the step
+    by decades until the two one-sided differences agree to 1e-4 of the
+    derivative, then holds the central difference to 1e-5."""
+    print("case adjoint autograd FD", flush=True)
+    g, u, v, w, q0 = _case(True, "gaussian", seed=101, maxx=28, maxy=24, nz=20)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    nsteps = 6
+    rho0 = 1.2 * np.exp(-g.zc() / 8000.0)
+    top = np.zeros(g.shape)
+    top[g.nz // 2:] = rho0[g.nz // 2:, None, None]
+    top = _interior_only(g, top)
+    with Advection(g, scal_adv="weno") as a:
+        a.set_stream(torch.cuda.current_stream(cuda).cuda_stream)
+        U, V, W, Q0, TOP = _dev(cuda, u, v, w, q0, top)
+        leaves = [x.clone().requires_grad_(True) for x in (Q0, U, V, W)]
+
+        def J(q0_, u_, v_, w_):
+            qT = _Advance.apply(q0_, u_, v_, w_, a, _lib.INT_SSPRK3, nsteps, dt, 4)
+            return (TOP * qT * qT).sum()
+
+        val = J(*leaves)
+        grads = torch.autograd.grad(val, leaves)
+        rng = np.random.default_rng(103)
+        with torch.no_grad():
+            j0 = float(J(Q0, U, V, W))
+            for n in range(8):
+                dq, du, dv, dw = _dirs(g, rng, 1.0)
+                d = [np.zeros(g.shape)] * 4
+                d[n % 4] = [dq, du, dv, dw][n % 4]
+                D = _dev(cuda, *d)
+                ad = float(sum((gr * y).sum() for gr, y in zip(grads, D)))
+                fd = None
+                for eps in (1e-6, 1e-7, 1e-8):
+                    jp = float(J(*(x + eps * y for x, y in zip((Q0, U, V, W), D))))
+                    jm = float(J(*(x - eps * y for x, y in zip((Q0, U, V, W), D))))
+                    if abs((jp - j0) - (j0 - jm)) / eps <= 1e-4 * abs(ad):
+                        fd = (jp - jm) / (2 * eps)
+                        break
+                assert fd is not None, f"direction {n}: every step straddles a jump"
+                print(f"  direction {n}: eps {eps:g} fd {fd:.12e} adjoint {ad:.12e}", flush=True)
+                assert abs(fd - ad) <= 1e-5 * abs(ad), (n, eps, fd, ad)

@@ -72,6 +72,27 @@ def test_weno_linear_weight_closed_form():
     assert not xa[:3].any() and not xa[n - 2:].any()
 
 
+def test_weno_limiter_is_discontinuous_at_zero():
+    """limiter_weno rescales about the left cell b with phi = min(|b/(minp -
+    b + eps)|, 1) (advq_weno.f90 limiter_weno).  For the stencil (0, t, 1, 1)
+    and u < 0: t = 0 gives phi = 0 and face value b = 0 exactly, while
+    t = -1e-12 gives minp = 1.5t, phi = min(|t/(0.5t + eps)|, 1) = 1 and the
+    nonlinear weights pick the flat candidate 1.5c - 0.5d = 1, so the flux
+    is u*1.  A 1e-12 change of the input moves the flux by |u|: the scheme
+    is discontinuous at zero-valued cells on the scale eps = 1e-15, which
+    is what bounds any reduced-precision trajectory through such cells."""
+    ora = _line_oracle()
+    n, f = 20, 8
+    fluxes = []
+    for t in (0.0, -1e-12):
+        x = np.zeros(n)
+        x[f - 1] = t          # cell f (1-based) is b
+        x[f:f + 2] = 1.0      # c, d
+        fluxes.append(_face_flux(ora, x, f, -1.0))
+    assert fluxes[0] == 0.0
+    assert abs(abs(fluxes[1]) - 1.0) <= 1e-9
+
+
 def test_weno_step_is_upwind():
     """At a jump the nonlinear weights pick the upwind candidate: the flux is
     u times the upwind cell value."""
