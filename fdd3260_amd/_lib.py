@@ -51,6 +51,10 @@ EXPORTED = (
     "mimadv_batch_budget",
     "mimadv_adjoint_tendency",
     "mimadv_adjoint_advance",
+    "mimadv_set_reference_density",
+    "mimadv_get_momentum",
+    "mimadv_get_advective",
+    "mimadv_get_conserved",
 )
 
 
@@ -145,6 +149,15 @@ def load() -> C.CDLL:
     lib.mimadv_adjoint_advance.argtypes = [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, i]
     lib.mimadv_adjoint_advance.restype = i
     lib.mimadv_batch_budget.restype = i
+    dp = C.POINTER(C.c_double)
+    lib.mimadv_set_reference_density.argtypes = [vp, dp, dp]
+    lib.mimadv_set_reference_density.restype = i
+    lib.mimadv_get_momentum.argtypes = [vp, vp, vp, vp, i, u]
+    lib.mimadv_get_momentum.restype = i
+    lib.mimadv_get_advective.argtypes = [vp, d, d, vp, vp, vp, vp, vp, vp, u]
+    lib.mimadv_get_advective.restype = i
+    lib.mimadv_get_conserved.argtypes = [vp, vp, i, pp, i, u]
+    lib.mimadv_get_conserved.restype = i
     _lib = lib
     return lib
 

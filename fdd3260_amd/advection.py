@@ -299,3 +299,33 @@ class Advection:
         rc = self.lib.mimadv_adjoint_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1],
                                              ptrs[2], ptrs[3], ptrs[4], ptrs[5], ptrs[6], ptrs[7], int(checkpoint))
         self._check(rc, "mimadv_adjoint_advance")
+
+    # -- density weighting (funcpack.f90, anelastic) ---------------------------
+    def set_reference_density(self, den0, avden0=None):
+        """den0(k) and (optionally) avden0(k); avden0 defaults to the fdz0-weighted face average."""
+        d = np.ascontiguousarray(den0, dtype=np.float64)
+        a = None if avden0 is None else np.ascontiguousarray(avden0, dtype=np.float64)
+        if d.shape != (self.grid.nz,) or (a is not None and a.shape != d.shape):
+            raise MimadvError("density profiles need nz values")
+        dp = C.POINTER(C.c_double)
+        rc = self.lib.mimadv_set_reference_density(self._h, d.ctypes.data_as(dp),
+                                                   None if a is None else a.ctypes.data_as(dp))
+        self._check(rc, "mimadv_set_reference_density")
+
+    def get_momentum(self, u, v, w, inverse=False):
+        """In place: u, v *= den0(k), w *= avden0(k) (inverse: get_velocities)."""
+        ptrs, hflag = self._ptrs([u, v, w], writable=(u, v, w))
+        rc = self.lib.mimadv_get_momentum(self._h, ptrs[0], ptrs[1], ptrs[2], 1 if inverse else 0, hflag)
+        self._check(rc, "mimadv_get_momentum")
+
+    def get_advective(self, u0shift, v0shift, u, v, w, ua, va, wa):
+        ptrs, hflag = self._ptrs([u, v, w, ua, va, wa], writable=(ua, va, wa))
+        rc = self.lib.mimadv_get_advective(self._h, float(u0shift), float(v0shift), *ptrs, hflag)
+        self._check(rc, "mimadv_get_advective")
+
+    def get_conserved(self, dens, fields, inverse=False):
+        """In place: q *= dens (inverse: q /= dens, get_scalars) for every field."""
+        ptrs, hflag = self._ptrs([dens] + list(fields), writable=fields)
+        rc = self.lib.mimadv_get_conserved(self._h, ptrs[0], len(fields), _lib.ptr_array(ptrs[1:]),
+                                           1 if inverse else 0, hflag)
+        self._check(rc, "mimadv_get_conserved")

@@ -40,6 +40,8 @@ struct mimadv_ctx {
     // adjoint: adjoint arrays, checkpoints and stage states
     double *adj = nullptr;
     size_t adj_elems = 0;
+    // density weighting: den0, avden0 on the device (2*nz), or null
+    double *d_den0 = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
@@ -198,6 +200,7 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->work) hipFree(c->work);
     if (c->bwork) hipFree(c->bwork);
     if (c->adj) hipFree(c->adj);
+    if (c->d_den0) hipFree(c->d_den0);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -869,6 +872,7 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
     const size_t need = n3 * ((size_t)3 + nseg + (size_t)S * K);
     if (need > c->adj_elems) {
         if (c->adj) hipFree(c->adj);
+    if (c->d_den0) hipFree(c->d_den0);
         c->adj = nullptr;
         c->adj_elems = 0;
         if (hipMalloc(&c->adj, need * sizeof(double)) != hipSuccess)
@@ -962,6 +966,118 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
     HIPCHK(c, launch_interior_copy(c->g, P, qbar, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int mimadv_set_reference_density(mimadv_ctx *c, const double *den0, const double *avden0)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!den0) return fail(c, MIMADV_ERR_ARG, "den0 is NULL");
+    const int nz = c->g.nz;
+    std::vector<double> h(2 * (size_t)nz), fdz0(nz);
+    for (int k = 0; k < nz; ++k) {
+        if (!(den0[k] != 0.0) || !std::isfinite(den0[k])) return fail(c, MIMADV_ERR_ARG, "den0 must be finite, non-zero");
+        h[k] = den0[k];
+    }
+    if (avden0) {
+        for (int k = 0; k < nz; ++k) h[nz + k] = avden0[k];
+    } else {
+        HIPCHK(c, hipMemcpy(fdz0.data(), c->g.zm.fdz0, nz * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 1; k < nz; ++k)
+            h[nz + k] = (fdz0[k] * den0[k] + fdz0[k - 1] * den0[k - 1]) / (fdz0[k] + fdz0[k - 1]);
+        h[nz] = den0[0];
+    }
+    if (!c->d_den0) {
+        if (hipMalloc(&c->d_den0, 2 * (size_t)nz * sizeof(double)) != hipSuccess)
+            return fail(c, MIMADV_ERR_NOMEM, "density profile allocation failed");
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_den0, h.data(), 2 * (size_t)nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MIMADV_OK;
+}
+
+int mimadv_get_momentum(mimadv_ctx *c, double *u, double *v, double *w, int inverse, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!c->d_den0) return fail(c, MIMADV_ERR_ARG, "no reference density: call mimadv_set_reference_density");
+    const bool three_d = c->g.ny > 1;
+    if (!u || !w || (three_d && !v)) return fail(c, MIMADV_ERR_ARG, "NULL wind array");
+    const double *p[3] = {c->d_den0, c->d_den0, c->d_den0 + c->g.nz};
+    double *x[3] = {u, three_d ? v : nullptr, w};
+    const size_t bytes = c->n3() * sizeof(double);
+    const int div = inverse ? 1 : 0;
+    for (int a = 0; a < 3; ++a) {
+        if (!x[a]) continue;
+        if (flags & MIMADV_HOST_PTRS) {
+            int rc = ensure_stage(c, 1);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->stage, x[a], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_level_scale(c->g, p[a], c->stage, c->stage, div, 0.0, c->stream));
+            HIPCHK(c, hipMemcpyAsync(x[a], c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, launch_level_scale(c->g, p[a], x[a], x[a], div, 0.0, c->stream));
+        }
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_get_advective(mimadv_ctx *c, double u0shift, double v0shift, const double *u, const double *v,
+                         const double *w, double *ua, double *va, double *wa, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!c->d_den0) return fail(c, MIMADV_ERR_ARG, "no reference density: call mimadv_set_reference_density");
+    const bool three_d = c->g.ny > 1;
+    if (!u || !w || !ua || !wa || (three_d && (!v || !va))) return fail(c, MIMADV_ERR_ARG, "NULL wind array");
+    const double *p[3] = {c->d_den0, c->d_den0, c->d_den0 + c->g.nz};
+    const double *in[3] = {u, three_d ? v : nullptr, w};
+    double *out[3] = {ua, three_d ? va : nullptr, wa};
+    const double shift[3] = {u0shift, v0shift, 0.0};
+    const size_t bytes = c->n3() * sizeof(double);
+    for (int a = 0; a < 3; ++a) {
+        if (!in[a]) continue;
+        if (flags & MIMADV_HOST_PTRS) {
+            int rc = ensure_stage(c, 1);
+            if (rc) return rc;
+            HIPCHK(c, hipMemcpyAsync(c->stage, in[a], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_level_scale(c->g, p[a], c->stage, c->stage, 1, shift[a], c->stream));
+            HIPCHK(c, hipMemcpyAsync(out[a], c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, launch_level_scale(c->g, p[a], in[a], out[a], 1, shift[a], c->stream));
+        }
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_get_conserved(mimadv_ctx *c, const double *dens, int nf, double *const *q, int inverse, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!dens) return fail(c, MIMADV_ERR_ARG, "dens is NULL");
+    if (nf < 0 || (nf > 0 && !q)) return fail(c, MIMADV_ERR_ARG, "bad field list");
+    const size_t n3 = c->n3(), bytes = n3 * sizeof(double);
+    const double *dd = dens;
+    if (flags & MIMADV_HOST_PTRS) {
+        int rc = ensure_stage(c, 2);
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpyAsync(c->stage + n3, dens, bytes, hipMemcpyHostToDevice, c->stream));
+        dd = c->stage + n3;
+    }
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        if (flags & MIMADV_HOST_PTRS) {
+            HIPCHK(c, hipMemcpyAsync(c->stage, q[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_field_scale(c->g, c->stage, dd, inverse ? 1 : 0, c->stream));
+            HIPCHK(c, hipMemcpyAsync(q[f], c->stage, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        } else {
+            HIPCHK(c, launch_field_scale(c->g, q[f], dd, inverse ? 1 : 0, c->stream));
+        }
+    }
     return MIMADV_OK;
 }
 

@@ -65,6 +65,8 @@ hipError_t launch_adjoint_stage(const AdjArgs &a, hipStream_t s);
 // dst = src on interior cells, 0 on ghost cells
 hipError_t launch_interior_copy(const GridDev &g, const double *src, double *dst, hipStream_t s);
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s);
+// 'weno' scheme, column-marching form (weno_march.hip); kchunk 0 = default
+hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s);
 hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
                                 const double *mv, const double *mw, const double *ut, const double *vt,
                                 const double *wt, double *ua, double *va, double *wa, hipStream_t s);
@@ -72,6 +74,11 @@ hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
 hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
                              double *tend, double *x, hipStream_t s);
 hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out2, hipStream_t s);
+// out = in*p(k) or in/p(k), minus shift; p a level profile (in may alias out)
+hipError_t launch_level_scale(const GridDev &g, const double *prof, const double *in, double *out, int divide,
+                              double shift, hipStream_t s);
+// q = q*d or q/d elementwise
+hipError_t launch_field_scale(const GridDev &g, double *q, const double *d, int divide, hipStream_t s);
 hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_part, double *d_out, hipStream_t s);
 
 // Host-side PPM per-level coefficient table (12 doubles per level).

@@ -229,7 +229,7 @@ static hipError_t launch_scheme(const GridDev &g, const SchemeParams &sp, const 
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s)
 {
     switch (sp.scheme) {
-    case 0: return launch_scheme<0>(g, sp, a, s);
+    case 0: return launch_weno_march(g, sp, a, 0, s);
     case 1: return launch_scheme<1>(g, sp, a, s);
     case 2: return launch_scheme<2>(g, sp, a, s);
     case 3: return launch_scheme<3>(g, sp, a, s);

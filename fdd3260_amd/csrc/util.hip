@@ -231,4 +231,48 @@ hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out, hipS
     return hipGetLastError();
 }
 
+namespace {
+
+// out = in * p(k) (or in / p(k)) - shift, level profile p (funcpack.f90
+// anelastic branches); in and out may alias
+__global__ void level_scale_kernel(size_t plane, size_t n, const double *in, double *out, const double *prof,
+                                   int divide, double shift)
+{
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x) {
+        const double p = prof[id / plane];
+        out[id] = (divide ? in[id] / p : in[id] * p) - shift;
+    }
+}
+
+// q = q * d (or q / d), d a 3D array (get_conserved / get_scalars)
+__global__ void field_scale_kernel(size_t n, double *q, const double *d, int divide)
+{
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < n; id += (size_t)gridDim.x * blockDim.x)
+        q[id] = divide ? q[id] / d[id] : q[id] * d[id];
+}
+
+unsigned elementwise_blocks(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+
+}  // namespace
+
+hipError_t launch_level_scale(const GridDev &g, const double *prof, const double *in, double *out, int divide,
+                              double shift, hipStream_t s)
+{
+    size_t plane = (size_t)g.nx * g.ny, n = plane * g.nz;
+    hipLaunchKernelGGL(level_scale_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, s, plane, n, in, out, prof,
+                       divide, shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_scale(const GridDev &g, double *q, const double *d, int divide, hipStream_t s)
+{
+    size_t n = (size_t)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(field_scale_kernel, dim3(elementwise_blocks(n)), dim3(256), 0, s, n, q, d, divide);
+    return hipGetLastError();
+}
+
 }  // namespace mimadv

@@ -1,0 +1,282 @@
+// weno_march.hip -- plain per-field FP64 kernel of the reference 'weno'
+// scheme, column-marching form.
+//
+// Same results as the generic per-plane kernel (scalar_fp64.hip) and the
+// CPU restatement, bit for bit: every face value is computed with the
+// reference operations (advq_weno.f90:92-110, weight{x,y,z} :329-350,
+// limiter_weno :462-491) in the reference order, only
+//   * once per face instead of once per adjacent cell: the x and y faces of
+//     a level are evaluated by a flattened loop over the workgroup and
+//     exchanged through LDS, the z face k+1/2 is carried to level k+1;
+//   * for the upwind side only.  The reference forms
+//     F = 0.5*fl*(u+|u|) + 0.5*fr*(u-|u|), where the off-side term is an
+//     exact zero, and the two sides' weights and candidates are the same
+//     expressions with outer and inner swapped (fl = w1*f1 + w2*f2,
+//     fr = w3*f2 + w4*f4 with w1, w4 the outer weights), so the upwind
+//     side alone gives the same value up to the sign of an exact zero.
+//
+// Work mapping: a workgroup owns a TX x TY tile of columns and a chunk of
+// levels.  Per level it stages the tile plus a 2-cell halo of q in LDS
+// (ghost columns read as the caller filled them), evaluates the TY*(TX+1)
+// x faces and (TY+1)*TX y faces of the tile, and each thread finishes its
+// cell with the z faces from a register window q(k-1..k+2).  A chunk
+// starting above the floor first evaluates its bottom face.
+//
+// Compiled with -ffp-contract=off.
+#include "mimadv_internal.hpp"
+
+namespace mimadv {
+
+namespace {
+
+// Reference face flux for the upwind side only.  nonlin: weights may
+// switch (x/y always, z for faces 3..nz-3).
+__device__ __forceinline__ double weno_face_up(double a, double b, double c, double d, double uu, bool nonlin)
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15, thres = 20.0;
+    const bool left = uu > 0.0;
+    double wo = g1, wi = g2;  // outer, inner
+    if (nonlin) {
+        const double eta_i = (c - b) * (c - b);
+        const double eta_o = left ? (b - a) * (b - a) : (d - c) * (d - c);
+        // left pair (eta1, eta2) = (outer, inner), right pair (eta2, eta3) =
+        // (inner, outer); max and min do not depend on the order
+        if (dmaxd(eta_o, eta_i) / (dmind(eta_o, eta_i) + eps) > thres) {
+            const double so = (eps + eta_o) * (eps + eta_o);
+            const double si = (eps + eta_i) * (eps + eta_i);
+            const double w_o = g1 / so, w_i = g2 / si;
+            const double sumw = w_o + w_i;
+            wo = w_o / sumw;
+            wi = w_i / sumw;
+        }
+    }
+    double ffim1 = -(0.5 * a) + 1.5 * b;
+    double ffip1 = 0.5 * b + 0.5 * c;
+    double ffip2 = 1.5 * c - 0.5 * d;
+    const double minp = dmind(dmind(dmind(ffim1, ffip1), ffip1), ffip2);
+    const double phi = fmin(fabs((0.0 - b) / (minp - b + eps)), 1.0);
+    const double fo = phi * ((left ? ffim1 : ffip2) - b) + b;
+    const double fi = phi * (ffip1 - b) + b;
+    const double f = wo * fo + wi * fi;
+    return left ? 0.5 * f * (uu + fabs(uu)) : 0.5 * f * (uu - fabs(uu));
+}
+
+// top closure k = nz-1 (advq_weno.f90:262-271): no limiter, linear weights
+__device__ __forceinline__ double weno_face_top(double a, double b, double c, double ww)
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
+    const double ffip1 = 0.5 * b + 0.5 * c;
+    if (ww > 0.0) {
+        const double fl = g1 * (-(0.5 * a) + 1.5 * b) + g2 * ffip1;
+        return 0.5 * fl * (ww + fabs(ww));
+    }
+    return 0.5 * ffip1 * (ww - fabs(ww));
+}
+
+// bottom closure kept (weno_bottom = 1): fl = (x1+x2)/2, fr = 2/3 fl + 1/3 f4
+__device__ __forceinline__ double weno_face_bottom(double b, double c, double d, double ww)
+{
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
+    const double ffip1 = 0.5 * b + 0.5 * c;
+    if (ww > 0.0) return 0.5 * ffip1 * (ww + fabs(ww));
+    const double fr = g2 * ffip1 + g1 * (1.5 * c - 0.5 * d);
+    return 0.5 * fr * (ww - fabs(ww));
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// z face f + 1/2 of a column; q(f-1..f+2) given (q(0) := q(1) by the caller)
+__device__ __forceinline__ double zface(int f, int nz, int bottom, double a, double b, double c, double d, double ww)
+{
+    if (f == nz - 1) return weno_face_top(a, b, c, ww);
+    if (f == 1 && bottom == 1) return weno_face_bottom(b, c, d, ww);
+    return weno_face_up(a, b, c, d, ww, f >= 3 && f <= nz - 3);
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemeParams sp, ScalarArgs a, int kchunk,
+                                                            int ntx, int nwork)
+{
+    constexpr bool THREE_D = TY > 1;
+    constexpr int NT = TX * TY;
+    constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;
+    constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0;
+    __shared__ double tile[LH][LW];
+    __shared__ double fx[TY][TX + 1];
+    __shared__ double fy[THREE_D ? TY + 1 : 1][TX];
+
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const size_t plane = (size_t)nx * ny;
+    const int mx = nx - 5, my = ny - 5;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int nchunk = (nz + kchunk - 1) / kchunk;
+    const double dx1 = 1.0 / g.dx, dy1 = 1.0 / g.dy;
+    const bool adv = a.adv != 0;
+    const int bc5 = g.bcl[4], bc6 = g.bcl[5];
+    const double *__restrict__ dz1w = g.zm.dz1w;
+
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const int tile_id = wi / nchunk, kc = wi - tile_id * nchunk;
+        const int i0 = 4 + (tile_id % ntx) * TX;
+        const int j0 = THREE_D ? 4 + (tile_id / ntx) * TY : 1;
+        const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
+        const int i = i0 + tx, j = j0 + ty;
+        const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
+        // loads of threads past the edge use the last interior column
+        const int il = min(i, nx - 2), jl = THREE_D ? min(j, ny - 2) : 1;
+        const size_t col = (size_t)(il - 1) + (size_t)nx * (jl - 1);
+        auto Qz = [&](int kk) { return a.q[col + (size_t)(clampi(kk, 1, nz) - 1) * plane]; };
+        auto Wz = [&](int kk) { return a.w[col + (size_t)(kk - 1) * plane]; };
+        // window q(k-1..k+2) and the carried bottom face of level k0
+        double z0 = Qz(k0 - 1), z1 = Qz(k0), z2 = Qz(k0 + 1), z3 = Qz(k0 + 2);
+        double fzb = 0.0;
+        if (k0 >= 2 && (a.dirmask & 4)) {
+            const int f = k0 - 1;
+            fzb = zface(f, nz, sp.weno_bottom, f == 1 ? Qz(1) : Qz(f - 1), Qz(f), Qz(f + 1), Qz(f + 2), Wz(f + 1));
+        }
+        for (int k = k0; k <= k1; ++k) {
+            const size_t kb = (size_t)(k - 1) * plane;
+            const double *__restrict__ qk = a.q + kb;
+            for (int t = tid; t < LH * LW; t += NT) {
+                const int r = t / LW, c = t - r * LW;
+                int ii = i0 - 2 + c;
+                if (ii > nx) ii = (ii - 4) % mx + 4;  // past the array: periodic image, feeds no interior cell
+                int jj = 1;
+                if (THREE_D) {
+                    jj = j0 - 2 + r;
+                    if (jj > ny) jj = (jj - 4) % my + 4;
+                }
+                tile[r][c] = qk[(size_t)(ii - 1) + (size_t)nx * (jj - 1)];
+            }
+            __syncthreads();
+            for (int e = tid; e < NXF + NYF; e += NT) {
+                if (e < NXF) {
+                    // x face f + 1/2 (velocity u(f+1)), f = i0-1+fl, of tile row r
+                    const int r = e / (TX + 1), fl = e - r * (TX + 1);
+                    const int f = i0 - 1 + fl;
+                    const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
+                    const int iu = min(f + 1, nx);
+                    const double uu = a.u[kb + (size_t)(iu - 1) + (size_t)nx * (jr - 1)];
+                    const int lr = THREE_D ? r + 2 : 0;
+                    fx[r][fl] = weno_face_up(tile[lr][fl], tile[lr][fl + 1], tile[lr][fl + 2], tile[lr][fl + 3], uu,
+                                             true);
+                } else if constexpr (THREE_D) {
+                    // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
+                    const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
+                    const int gf = j0 - 1 + fr;
+                    const int ic = min(i0 + c, nx - 2);
+                    const int jv = min(gf + 1, ny);
+                    const double vv = a.v[kb + (size_t)(ic - 1) + (size_t)nx * (jv - 1)];
+                    fy[fr][c] = weno_face_up(tile[fr][c + 2], tile[fr + 1][c + 2], tile[fr + 2][c + 2],
+                                             tile[fr + 3][c + 2], vv, true);
+                }
+            }
+            __syncthreads();
+            double fzt = 0.0;
+            const bool need_top = k <= nz - 1 && (a.dirmask & 4);
+            if (need_top) fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, Wz(k + 1));
+            if (valid) {
+                const size_t id = col + kb;
+                double X = 0.0, Y = 0.0, Z = 0.0;
+                if (a.dirmask & 1) {
+                    X = (fx[ty][tx + 1] - fx[ty][tx]) * dx1;
+                    if (adv) {
+                        const double *urow = a.u + kb + (size_t)nx * (j - 1);
+                        X = X - z1 * (urow[i] - urow[i - 1]) * dx1;
+                    }
+                }
+                if (THREE_D && (a.dirmask & 2)) {
+                    Y = (fy[ty + 1][tx] - fy[ty][tx]) * dy1;
+                    if (adv) {
+                        const double *vcol = a.v + kb + (size_t)(i - 1);
+                        Y = Y - z1 * (vcol[(size_t)nx * j] - vcol[(size_t)nx * (j - 1)]) * dy1;
+                    }
+                }
+                if (a.dirmask & 4) {
+                    // z_assemble (no-flux 'nnf' floor/lid, zero z tendency at other ends)
+                    if (k >= 2 && k <= nz - 1) {
+                        Z = (fzt - fzb) * dz1w[k - 1];
+                        if (adv) Z = Z - z1 * (Wz(k + 1) - Wz(k)) * dz1w[k - 1];
+                    } else {
+                        double t = 0.0;
+                        if (k == 1 && bc5 == 1) {
+                            t = t + (fzt - 0.0) * dz1w[0];
+                            if (adv) t = t - z1 * (Wz(2) - 0.0) * dz1w[0];
+                        }
+                        if (k == nz && bc6 == 1) {
+                            t = t + (0.0 - fzb) * dz1w[nz - 1];
+                            if (adv) t = t - z1 * (0.0 - Wz(nz)) * dz1w[nz - 1];
+                        }
+                        Z = t;
+                    }
+                }
+                if (a.epi == EPI_ACCUM || a.epi == EPI_OVERWRITE) {
+                    double r = (a.epi == EPI_ACCUM) ? a.xs[id] : 0.0;
+                    if (a.dirmask & 1) r = r - X;
+                    if (THREE_D && (a.dirmask & 2)) r = r - Y;
+                    if (a.dirmask & 4) r = r - Z;
+                    a.xs[id] = r;
+                } else {
+                    double A = 0.0;
+                    if (a.dirmask & 1) A = A - X;
+                    if (THREE_D && (a.dirmask & 2)) A = A - Y;
+                    if (a.dirmask & 4) A = A - Z;
+                    double qn;
+                    if (a.epi == EPI_HEUN) {
+                        const double T = (a.b_lm != 0.0) ? A + a.b_lm * a.tend[id] : A + 0.0;
+                        a.tend[id] = T;
+                        qn = a.q0[id] + a.adt * T;
+                    } else {
+                        qn = a.c0 * a.q0[id] + a.c1 * (z1 + a.dt * A);
+                    }
+                    a.qn[id] = qn;
+                    if (a.fill) {
+                        // periodic images (qnbc_p4, bc.f90:409-414, :431-436)
+                        int ix = 0, jy = 0;
+                        if (i >= nx - 4) ix = i - mx;
+                        else if (i <= 5) ix = i + mx;
+                        if (THREE_D) {
+                            if (j >= ny - 4) jy = j - my;
+                            else if (j <= 5) jy = j + my;
+                        }
+                        double *qo = a.qn + kb;
+                        if (ix) qo[(size_t)(ix - 1) + (size_t)nx * (j - 1)] = qn;
+                        if (jy) {
+                            qo[(size_t)(i - 1) + (size_t)nx * (jy - 1)] = qn;
+                            if (ix) qo[(size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qn;
+                        }
+                    }
+                }
+            }
+            fzb = fzt;
+            z0 = z1;
+            z1 = z2;
+            z2 = z3;
+            z3 = Qz(k + 3);
+            __syncthreads();
+        }
+    }
+}
+
+template <int TX, int TY>
+hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
+{
+    const int ntx = (g.nx - 5 + TX - 1) / TX;
+    const int nty = TY > 1 ? (g.ny - 5 + TY - 1) / TY : 1;
+    if (kchunk <= 0) kchunk = 16;
+    if (kchunk > g.nz) kchunk = g.nz;
+    const int nchunk = (g.nz + kchunk - 1) / kchunk;
+    const int nwork = ntx * nty * nchunk;
+    hipLaunchKernelGGL((weno_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx, nwork);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
+{
+    if (g.ny > 1) return launch_tile<64, 4>(g, sp, a, kchunk, s);
+    return launch_tile<64, 1>(g, sp, a, kchunk, s);
+}
+
+}  // namespace mimadv

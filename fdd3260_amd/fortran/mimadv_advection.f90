@@ -76,6 +76,31 @@ module mimadv_advection
       type(c_ptr), value :: mu, mv, mw, ut, vt, wt, ua, va, wa
       integer(c_int), value :: flags
     end function
+    integer(c_int) function c_set_reference_density(ctx, den0, avden0) &
+        bind(c, name='mimadv_set_reference_density')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, den0, avden0
+    end function
+    integer(c_int) function c_get_momentum(ctx, u, v, w, inverse, flags) bind(c, name='mimadv_get_momentum')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, u, v, w
+      integer(c_int), value :: inverse, flags
+    end function
+    integer(c_int) function c_get_advective(ctx, u0shift, v0shift, u, v, w, ua, va, wa, flags) &
+        bind(c, name='mimadv_get_advective')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: u0shift, v0shift
+      type(c_ptr), value :: u, v, w, ua, va, wa
+      integer(c_int), value :: flags
+    end function
+    integer(c_int) function c_get_conserved(ctx, dens, nf, q, inverse, flags) bind(c, name='mimadv_get_conserved')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, dens
+      integer(c_int), value :: nf
+      type(c_ptr), value :: q
+      integer(c_int), value :: inverse, flags
+    end function
   end interface
 
   type(c_ptr), save :: ctx = c_null_ptr
@@ -266,6 +291,44 @@ contains
     call check(c_momentum_tendency(ctx, int(ntime, c_int), c_loc(mu), c_loc(mv), c_loc(mw), c_loc(ut), &
                                    c_loc(vt), c_loc(wt), c_loc(ua), c_loc(va), c_loc(wa), MIMADV_HOST_PTRS), &
                'qnadv_uvw')
+  end subroutine
+
+  ! Reference density profiles den0, avden0 (shared_data globals set in
+  ! initial.f90:175); call once after mimadv_init
+  subroutine mimadv_set_density(den0, avden0)
+    real(8), target, intent(in) :: den0(g_nz), avden0(g_nz)
+    call check(c_set_reference_density(ctx, c_loc(den0), c_loc(avden0)), 'set_reference_density')
+  end subroutine
+
+  ! get_momentum / get_velocities (funcpack.f90:99-316), anelastic branch, in place
+  subroutine get_momentum_arrays(u, v, w)
+    real(8), target, intent(inout) :: u(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    call check(c_get_momentum(ctx, c_loc(u), c_loc(v), c_loc(w), 0_c_int, MIMADV_HOST_PTRS), 'get_momentum')
+  end subroutine
+
+  subroutine get_velocities_arrays(u, v, w)
+    real(8), target, intent(inout) :: u(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    call check(c_get_momentum(ctx, c_loc(u), c_loc(v), c_loc(w), 1_c_int, MIMADV_HOST_PTRS), 'get_velocities')
+  end subroutine
+
+  ! get_advective (funcpack.f90:65-92): wind_adv from the momenta
+  subroutine get_advective_arrays(u0shift, v0shift, u, v, w, ua, va, wa)
+    real(8), intent(in) :: u0shift, v0shift
+    real(8), target, intent(in) :: u(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    real(8), target, intent(out) :: ua(g_nx, g_ny, g_nz), va(g_nx, g_ny, g_nz), wa(g_nx, g_ny, g_nz)
+    call check(c_get_advective(ctx, u0shift, v0shift, c_loc(u), c_loc(v), c_loc(w), c_loc(ua), c_loc(va), &
+                               c_loc(wa), MIMADV_HOST_PTRS), 'get_advective')
+  end subroutine
+
+  ! get_conserved / get_scalars (funcpack.f90:397-514) on one array
+  subroutine get_conserved_array(dens, q, inverse)
+    real(8), target, intent(in) :: dens(g_nx, g_ny, g_nz)
+    real(8), target, intent(inout) :: q(g_nx, g_ny, g_nz)
+    logical, intent(in) :: inverse
+    type(c_ptr), target :: qp(1)
+    qp(1) = c_loc(q)
+    call check(c_get_conserved(ctx, c_loc(dens), 1_c_int, c_loc(qp), merge(1_c_int, 0_c_int, inverse), &
+                               MIMADV_HOST_PTRS), merge('get_scalars  ', 'get_conserved', inverse))
   end subroutine
 
 end module mimadv_advection

@@ -159,6 +159,30 @@ int mimadv_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const
                    const double *v, const double *w, int nf, double *const *q, const int *adv_form,
                    unsigned flags);
 
+/* ---- density weighting of the anelastic build (funcpack.f90:65-542) ----
+ * The reference density profile den0(k) and its face average avden0(k) are
+ * the module globals of shared_data; they are handed over once per context.
+ * avden0 may be NULL: it is then computed as initial.f90:543-546 does,
+ * avden0(1) = den0(1), avden0(k) = (fdz0(k) den0(k) + fdz0(k-1) den0(k-1)) /
+ * (fdz0(k) + fdz0(k-1)).  Host arrays of nz values. */
+int mimadv_set_reference_density(mimadv_ctx *ctx, const double *den0, const double *avden0);
+
+/* get_momentum (inverse = 0, funcpack.f90:99-178): u, v *= den0(k),
+ * w *= avden0(k); get_velocities (inverse = 1, :236-316): the same
+ * divisions.  In place, whole arrays; v unused when ny == 1. */
+int mimadv_get_momentum(mimadv_ctx *ctx, double *u, double *v, double *w, int inverse, unsigned flags);
+
+/* get_advective (funcpack.f90:65-92): ua = u/den0(k) - u0shift,
+ * va = v/den0(k) - v0shift, wa = w/avden0(k) from the momenta u, v, w. */
+int mimadv_get_advective(mimadv_ctx *ctx, double u0shift, double v0shift, const double *u, const double *v,
+                         const double *w, double *ua, double *va, double *wa, unsigned flags);
+
+/* get_conserved (inverse = 0, funcpack.f90:397-454): q_f *= dens;
+ * get_scalars (inverse = 1, :457-514): q_f /= dens; dens is the 3D density
+ * array (pressurel%dens).  In place on nf fields. */
+int mimadv_get_conserved(mimadv_ctx *ctx, const double *dens, int nf, double *const *q, int inverse,
+                         unsigned flags);
+
 /* ---- batched mixed-precision path (no reference counterpart: the
  * reference advects field by field in FP64) ---- */
 #define MIMADV_F64 0

@@ -1329,3 +1329,53 @@ void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq,
             }
 #undef DQ
 }
+
+/* ======================================================================
+ * Density weighting, anelastic build (funcpack.f90:65-542).
+ * ====================================================================== */
+
+void orc_avden0(int nz, const double *fdz0, const double *den0, double *avden0)
+{
+    /* initial.f90:543-546 */
+    for (int k = 2; k <= nz; ++k)
+        avden0[k - 1] = (fdz0[k - 1] * den0[k - 1] + fdz0[k - 2] * den0[k - 2]) / (fdz0[k - 1] + fdz0[k - 2]);
+    avden0[0] = den0[0];
+}
+
+static void scale_levels(const orc_grid *g, const double *prof, const double *in, double *out, int inverse,
+                         double shift)
+{
+    size_t plane = (size_t)g->nx * g->ny;
+    for (int k = 1; k <= g->nz; ++k) {
+        double p = prof[k - 1];
+        const double *a = in + (size_t)(k - 1) * plane;
+        double *b = out + (size_t)(k - 1) * plane;
+        for (size_t n = 0; n < plane; ++n) b[n] = (inverse ? a[n] / p : a[n] * p) - shift;
+    }
+}
+
+void orc_get_momentum(const orc_grid *g, const double *den0, const double *avden0, double *u, double *v,
+                      double *w, int inverse)
+{
+    /* funcpack.f90:160-168 (momentum), :294-302 (velocities) */
+    scale_levels(g, den0, u, u, inverse, 0.0);
+    if (g->ny > 1) scale_levels(g, den0, v, v, inverse, 0.0);
+    scale_levels(g, avden0, w, w, inverse, 0.0);
+}
+
+void orc_get_advective(const orc_grid *g, const double *den0, const double *avden0, double u0shift,
+                       double v0shift, const double *u, const double *v, const double *w, double *ua,
+                       double *va, double *wa)
+{
+    /* funcpack.f90:76-86: copy, get_velocities, subtract the translation */
+    scale_levels(g, den0, u, ua, 1, u0shift);
+    if (g->ny > 1) scale_levels(g, den0, v, va, 1, v0shift);
+    scale_levels(g, avden0, w, wa, 1, 0.0);
+}
+
+void orc_get_conserved(const orc_grid *g, const double *dens, double *q, int inverse)
+{
+    /* funcpack.f90:413-416 (x*dref), :473-476 (x/dref) */
+    size_t n = (size_t)g->nx * g->ny * g->nz;
+    for (size_t i = 0; i < n; ++i) q[i] = inverse ? q[i] / dens[i] : q[i] * dens[i];
+}

@@ -16,6 +16,8 @@
  *   src/TOOLS/bc.f90          qnbc_p4 (:378-503), lbcu_p4 (:81-144)
  *   src/INTEG/integpack.f90 + INCLUDE/integ.h   Heun / Euler stage combine
  *   src/TOOLS/averages.f90    totalav (:507-551)
+ *   src/TOOLS/funcpack.f90    get_momentum/get_velocities/get_advective/
+ *                             get_conserved/get_scalars, anelastic (:65-542)
  * plus two schemes the reference does not have (WENO5-JS reconstruction and
  * SSP-RK3 stage coefficients), which are pinned only by closed forms.
  *
@@ -112,6 +114,24 @@ double orc_totalav(const orc_grid *g, const double *x);
 void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
                            const double *v, const double *w, const double *du, const double *dv,
                            const double *dw, double *xs, double *dxs);
+
+/* ---- density weighting of the anelastic build (funcpack.f90:65-542) ---- */
+/* avden0(1) = den0(1); avden0(k) = (fdz0(k)den0(k) + fdz0(k-1)den0(k-1)) /
+ * (fdz0(k) + fdz0(k-1)) (initial.f90:175, :543-546). */
+void orc_avden0(int nz, const double *fdz0, const double *den0, double *avden0);
+/* get_momentum (inverse = 0; funcpack.f90:99-178 anelastic branch):
+ * u,v *= den0(k), w *= avden0(k); get_velocities (inverse = 1; :236-316):
+ * the same divisions.  Whole arrays including ghosts; v skipped when ny == 1. */
+void orc_get_momentum(const orc_grid *g, const double *den0, const double *avden0, double *u, double *v,
+                      double *w, int inverse);
+/* get_advective (funcpack.f90:65-92): ua = u/den0 - u0shift,
+ * va = v/den0 - v0shift, wa = w/avden0. */
+void orc_get_advective(const orc_grid *g, const double *den0, const double *avden0, double u0shift,
+                       double v0shift, const double *u, const double *v, const double *w, double *ua,
+                       double *va, double *wa);
+/* get_conserved (inverse = 0; funcpack.f90:397-454): q *= dens;
+ * get_scalars (inverse = 1; :457-514): q /= dens.  dens is a 3D array. */
+void orc_get_conserved(const orc_grid *g, const double *dens, double *q, int inverse);
 
 /* OpenMP thread count used by the 3D drivers (1 = serial).  Returns the
  * previous value. */

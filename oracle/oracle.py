@@ -69,6 +69,10 @@ def load():
     lib.orc_totalav.argtypes = [G, dp]
     lib.orc_totalav.restype = d
     lib.orc_tlm_weno_tendency.argtypes = [G, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    lib.orc_avden0.argtypes = [i, dp, dp, dp]
+    lib.orc_get_momentum.argtypes = [G, dp, dp, dp, dp, dp, i]
+    lib.orc_get_advective.argtypes = [G, dp, dp, d, d, dp, dp, dp, dp, dp, dp]
+    lib.orc_get_conserved.argtypes = [G, dp, dp, i]
     lib.orc_set_threads.argtypes = [i]
     lib.orc_set_threads.restype = i
     _lib = lib
@@ -153,6 +157,28 @@ class Oracle:
         self.lib.orc_tlm_weno_tendency(C.byref(self.g), _dp(q), _dp(dq), _dp(u), _dp(v), _dp(w), _dp(du), _dp(dv),
                                        _dp(dw), _dp(xs), _dp(dxs))
         return xs, dxs
+
+    def avden0(self, den0):
+        den0 = np.ascontiguousarray(den0, dtype=np.float64)
+        out = np.zeros_like(den0)
+        self.lib.orc_avden0(den0.size, _dp(self._keep[0]), _dp(den0), _dp(out))
+        return out
+
+    def get_momentum(self, den0, avden0, u, v, w, inverse=False):
+        """In place: u,v *= den0(k), w *= avden0(k) (inverse: divided)."""
+        den0, avden0 = (np.ascontiguousarray(x, dtype=np.float64) for x in (den0, avden0))
+        self.lib.orc_get_momentum(C.byref(self.g), _dp(den0), _dp(avden0), _dp(u), _dp(v), _dp(w),
+                                  1 if inverse else 0)
+
+    def get_advective(self, den0, avden0, u0shift, v0shift, u, v, w):
+        den0, avden0 = (np.ascontiguousarray(x, dtype=np.float64) for x in (den0, avden0))
+        ua, va, wa = (np.zeros_like(u) for _ in range(3))
+        self.lib.orc_get_advective(C.byref(self.g), _dp(den0), _dp(avden0), float(u0shift), float(v0shift), _dp(u),
+                                   _dp(v), _dp(w), _dp(ua), _dp(va), _dp(wa))
+        return ua, va, wa
+
+    def get_conserved(self, dens, q, inverse=False):
+        self.lib.orc_get_conserved(C.byref(self.g), _dp(dens), _dp(q), 1 if inverse else 0)
 
     def fill_scalar(self, x):
         self.lib.orc_fill_scalar(C.byref(self.g), _dp(x))

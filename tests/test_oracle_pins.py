@@ -644,3 +644,70 @@ def test_tlm_matches_central_differences(wrt):
     frac_bad = np.mean(err > 1e-5 * np.abs(tl[I]).max())
     assert frac_bad < 1e-3
     assert np.median(err) <= 1e-8 * np.abs(tl[I]).max()
+
+
+# ------------------------------------------------ density weighting (A9) -----
+
+def test_avden0_is_the_fdz0_weighted_face_average():
+    """initial.f90:543-546 on a grid whose fdz0 alternates 1, 3: the face
+    average of levels k-1, k is (fdz0(k-1) den0(k-1) + fdz0(k) den0(k)) / 4;
+    level 1 keeps den0(1)."""
+    g = gm.uniform(8, 0, 10)
+    g.fdz[:] = np.where(np.arange(g.nz) % 2 == 0, 1.0, 3.0)
+    ora = Oracle(g)
+    den0 = 1.0 + 0.25 * np.arange(g.nz)
+    av = ora.avden0(den0)
+    assert av[0] == den0[0]
+    for k in range(1, g.nz):
+        exp = (g.fdz[k - 1] * den0[k - 1] + g.fdz[k] * den0[k]) / (g.fdz[k - 1] + g.fdz[k])
+        assert av[k] == pytest.approx(exp, rel=1e-15)
+    assert av[1] == pytest.approx((1.0 * 1.0 + 3.0 * 1.25) / 4.0, rel=1e-15)
+
+
+@pytest.mark.parametrize("three_d", [True, False])
+def test_get_momentum_level_profiles(three_d):
+    """Power-of-two profiles make every product exact: u and v are scaled by
+    den0 of their own level, w by avden0 (not den0), and get_velocities
+    undoes it bit for bit; v is left alone in 2D."""
+    g = gm.uniform(6, 5 if three_d else 0, 7)
+    ora = Oracle(g)
+    den0 = 2.0 ** -np.arange(g.nz)
+    avden0 = 2.0 ** -(np.arange(g.nz) + 3)  # distinct from den0 on every level
+    rng = np.random.default_rng(3)
+    u0, v0, w0 = (rng.standard_normal(g.shape) for _ in range(3))
+    u, v, w = u0.copy(), v0.copy(), w0.copy()
+    ora.get_momentum(den0, avden0, u, v, w)
+    assert np.array_equal(u, u0 * den0[:, None, None])
+    assert np.array_equal(w, w0 * avden0[:, None, None])
+    assert np.array_equal(v, v0 * den0[:, None, None] if three_d else v0)
+    ora.get_momentum(den0, avden0, u, v, w, inverse=True)
+    assert np.array_equal(u, u0) and np.array_equal(v, v0) and np.array_equal(w, w0)
+
+
+def test_get_advective_known_answer():
+    """funcpack.f90:76-86: velocities from momenta minus the translation
+    (u0shift on u, v0shift on v, none on w)."""
+    g = gm.uniform(6, 5, 4)
+    ora = Oracle(g)
+    den0 = np.array([1.0, 0.5, 0.25, 0.125])
+    avden0 = np.array([2.0, 1.0, 0.5, 0.25])
+    u = np.full(g.shape, 3.0)
+    v = np.full(g.shape, -1.0)
+    w = np.full(g.shape, 0.75)
+    ua, va, wa = ora.get_advective(den0, avden0, 1.5, -2.0, u, v, w)
+    assert np.array_equal(ua[:, 0, 0], 3.0 / den0 - 1.5)
+    assert np.array_equal(va[:, 0, 0], -1.0 / den0 + 2.0)
+    assert np.array_equal(wa[:, 0, 0], 0.75 / avden0)
+
+
+def test_get_conserved_and_scalars():
+    g = gm.uniform(6, 5, 4)
+    ora = Oracle(g)
+    rng = np.random.default_rng(4)
+    dens = 2.0 ** rng.integers(-3, 3, g.shape).astype(float)
+    q0 = rng.standard_normal(g.shape)
+    q = q0.copy()
+    ora.get_conserved(dens, q)
+    assert np.array_equal(q, q0 * dens)
+    ora.get_conserved(dens, q, inverse=True)
+    assert np.array_equal(q, q0)
