@@ -106,7 +106,7 @@ def fused(args):
         fields.append(torch.from_numpy(q.astype(np.float32)).to(dev))
     log(f"fused case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
     adv = Advection(g, scal_adv=args.scheme if args.scheme in ("weno", "muscl") else "weno", muscllimiter=args.limiter)
-    adv.set_tuning(group32=args.group32, kchunk=args.kchunk)
+    adv.set_tuning(group32=args.group32, kchunk=args.kchunk, tile_y=args.tile_y)
     code = _lib.INT_HEUN if args.integrator == "heun" else _lib.INT_SSPRK3
     nst = 2 if args.integrator == "heun" else 3
     graph = not args.no_graph
@@ -318,6 +318,7 @@ def main():
     ap.add_argument("--fd-dirs", type=int, default=4, help="adjoint workload: finite-difference directions")
     ap.add_argument("--group32", type=int, default=0)
     ap.add_argument("--kchunk", type=int, default=0)
+    ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of the column tile (2, 4, 8)")
     ap.add_argument("--no-graph", action="store_true")
     args = ap.parse_args()
 

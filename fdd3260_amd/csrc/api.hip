@@ -34,7 +34,7 @@ struct mimadv_ctx {
     // batched path: scratch bytes, tuning, captured step graph
     char *bwork = nullptr;
     size_t bwork_bytes = 0;
-    mimadv_tuning tuning{0, 0, 0};
+    mimadv_tuning tuning{0, 0, 0, 0};
     hipGraphExec_t graph_exec = nullptr;
     std::vector<const void *> graph_key;
     // adjoint: adjoint arrays, checkpoints and stage states
@@ -508,8 +508,8 @@ int mimadv_set_tuning(mimadv_ctx *c, const mimadv_tuning *t)
     if (!c || !t) return MIMADV_ERR_ARG;
     if (t->group32 < 0 || t->group32 > FUSED_MAX32 || t->kchunk < 0 || t->blocks_per_cu < 0)
         return fail(c, MIMADV_ERR_ARG, "tuning values out of range");
-    if (t->group32 > 0 && fused_padded_count(t->group32) != t->group32)
-        return fail(c, MIMADV_ERR_ARG, "group32 must be one of 1, 2, 4, 8, 16, 18");
+    if (t->tile_y != 0 && t->tile_y != 2 && t->tile_y != 4 && t->tile_y != 8)
+        return fail(c, MIMADV_ERR_ARG, "tile_y must be 2, 4 or 8");
     c->tuning = *t;
     return MIMADV_OK;
 }
@@ -523,7 +523,7 @@ struct BatchGroup {
 };
 
 // FP64 fields ride with the first group; FP32 fields are cut into groups of
-// at most group32 (register pressure of the kernel grows with the count).
+// at most group32 (default: all of them, up to the launch limit).
 int plan_groups(mimadv_ctx *c, int nf, const int *prec, std::vector<BatchGroup> &groups)
 {
     std::vector<int> i64, i32;
@@ -533,7 +533,7 @@ int plan_groups(mimadv_ctx *c, int nf, const int *prec, std::vector<BatchGroup> 
         else return fail(c, MIMADV_ERR_ARG, "precision must be MIMADV_F32 or MIMADV_F64");
     }
     if ((int)i64.size() > FUSED_MAX64) return fail(c, MIMADV_ERR_UNSUPPORTED, "at most 2 FP64 fields per batch");
-    int g32 = c->tuning.group32 > 0 ? c->tuning.group32 : 8;
+    int g32 = c->tuning.group32 > 0 ? std::min(c->tuning.group32, FUSED_MAX32) : FUSED_MAX32;
     groups.clear();
     size_t p = 0;
     do {
@@ -580,7 +580,8 @@ FusedArgs base_args(mimadv_ctx *c, const float *u, const float *v, const float *
     A.muscl_limiter = c->sp.muscl_limiter;
     A.lim_tol = c->sp.lim_tol;
     A.weno_bottom = c->sp.weno_bottom;
-    A.kchunk = c->tuning.kchunk > 0 ? c->tuning.kchunk : c->g.nz;
+    A.kchunk = c->tuning.kchunk;
+    A.tile_y = c->tuning.tile_y;
     A.dirmask = 7;
     return A;
 }

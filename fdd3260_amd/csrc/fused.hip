@@ -2,18 +2,18 @@
 //
 // One persistent launch advects a batch of fields that share one velocity
 // (u, v, w in FP32): up to MAX64 fields in FP64 (theta, water vapour: small
-// increments on large values) and the rest in FP32.  Each workgroup owns a
-// TX x TY tile of columns and a range of levels and marches up the column:
+// increments on large values) and the rest in FP32.  A work item is a
+// TX x TY tile of columns and a chunk of KC levels:
 //
-//   * per level the three velocity faces of every column are read once and
-//     serve every field;
-//   * per field the level's tile plus a 2-cell halo goes through LDS; every
-//     thread evaluates the east x-face and north y-face of its cell, the
-//     first TY + TX threads the west/south edge faces of the tile, and the
-//     shared faces are exchanged through LDS (each face evaluated once);
-//   * the z stencil is a register window q(k-1..k+2) per field, refilled
-//     with one coalesced load per level; the top-face flux of level k is
-//     carried to level k+1 as its bottom face;
+//   * the chunk's face velocities are read once into registers and serve
+//     every field of the batch (fields are the outer loop, levels the
+//     inner one, so register use does not grow with the field count);
+//   * per field and level the tile plus a 2-cell halo goes through LDS
+//     (the next level's tile is already in flight), the TY*(TX+1) x faces
+//     and (TY+1)*TX y faces of the tile are evaluated once each by a
+//     flattened loop over the workgroup, and the z face k+1/2 of every
+//     column comes from a register window q(k-1..k+2) and is carried to
+//     level k+1 as its bottom face;
 //   * face fluxes of FP32 fields are FP32; the flux differences and the
 //     divergence are accumulated in FP64 (exact differences of FP32 values,
 //     a compensated sum) and the Shu-Osher stage update
@@ -44,7 +44,12 @@ __device__ __forceinline__ float rcp<float>(float x)
 template <>
 __device__ __forceinline__ double rcp<double>(double x)
 {
-    return 1.0 / x;
+    // hardware estimate refined by two Newton steps (error ~1 ulp)
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
 }
 
 template <class T>
@@ -206,340 +211,292 @@ __device__ __forceinline__ int wrap(int i, int n, int m)
     return (r < 0 ? r + m : r) + 4;
 }
 
-template <class T>
-struct FieldPtrs;
-template <>
-struct FieldPtrs<double> {
-    static __device__ __forceinline__ const double *q(const FusedArgs &A, int f) { return A.q64[f]; }
-    static __device__ __forceinline__ const double *q0(const FusedArgs &A, int f) { return A.q0_64[f]; }
-    static __device__ __forceinline__ double *out(const FusedArgs &A, int f) { return A.out64[f]; }
-};
-template <>
-struct FieldPtrs<float> {
-    static __device__ __forceinline__ const float *q(const FusedArgs &A, int f) { return A.q32[f]; }
-    static __device__ __forceinline__ const float *q0(const FusedArgs &A, int f) { return A.q0_32[f]; }
-    static __device__ __forceinline__ float *out(const FusedArgs &A, int f) { return A.out32[f]; }
+// Geometry of one work item (a TX x TY column tile and a chunk of at most
+// KC levels) shared by every field it advects.
+template <int TX, int TY>
+struct Item {
+    static constexpr bool THREE_D = TY > 1;
+    static constexpr int NT = TX * TY;
+    static constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;  // tile + 2-cell halo
+    static constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
+    static constexpr int FS = (NF + NT - 1) / NT;        // face slots per thread
+    static constexpr int TS = (LH * LW + NT - 1) / NT;   // tile slots per thread
 };
 
-template <int SCHEME, int TX, int TY, class T>
-__device__ __forceinline__ void do_field(const FusedArgs &A, int f, int k, int tid, int tx, int ty, int i0, int j0,
-                                         bool valid, size_t id, T (&zw)[4], T &fzb, T ue, T vn, T uw_e, T vs_e,
-                                         T wt, double dz1w_k, T (*tile)[TX + 4], T (*fx)[TX + 1], T (*fy)[TX],
-                                         const int *hal_r, const int *hal_c, const T *hal_v, const bool *hal_ok)
-{
-    constexpr int NT = TX * TY;
-    constexpr int HPT = (4 * TX + 4 * TY + 16 + NT - 1) / NT;
-    using P = FieldPtrs<T>;
-    const int nz = A.nz;
-    // own value and halo into LDS (the previous field's readers finished at
-    // its second barrier)
-    tile[ty + 2][tx + 2] = zw[1];
+// Per-thread slot geometry of a work item, computed once and shared by
+// every field and level: the thread's tile slots (LDS index, column offset
+// in a level plane) and face slots of the flattened list
+// [x faces row-major | y faces] (LDS offset of the stencil q(f-1..f+2),
+// step 1 for x and LW for y, 0 for an empty slot; plane offset of the face
+// velocity u(f+1) or v(g+1)).  Plane offsets fit in 32 bits.
+template <int TX, int TY>
+struct Slots {
+    using I = Item<TX, TY>;
+    int t_off[I::TS];
+    int f_base[I::FS], f_step[I::FS], f_off[I::FS];
+    __device__ __forceinline__ void init(int tid, int i0, int j0, const FusedArgs &A)
+    {
+        const int nx = A.nx, ny = A.ny;
 #pragma unroll
-    for (int h = 0; h < HPT; ++h)
-        if (hal_ok[h]) tile[hal_r[h]][hal_c[h]] = hal_v[h];
-    __syncthreads();
-    T Fx = hface<SCHEME, T>(tile[ty + 2][tx + 1], tile[ty + 2][tx + 2], tile[ty + 2][tx + 3], tile[ty + 2][tx + 4], ue,
-                            A, A.dx1);
-    fx[ty][tx + 1] = Fx;
-    T Fy = T(0);
-    if (A.three_d) {
-        Fy = hface<SCHEME, T>(tile[ty + 1][tx + 2], tile[ty + 2][tx + 2], tile[ty + 3][tx + 2], tile[ty + 4][tx + 2], vn,
-                              A, A.dy1);
-        fy[ty + 1][tx] = Fy;
-    }
-    if (tid < TY) {  // west edge face of row tid
-        int r = tid;
-        fx[r][0] = hface<SCHEME, T>(tile[r + 2][0], tile[r + 2][1], tile[r + 2][2], tile[r + 2][3], uw_e, A, A.dx1);
-    } else if (A.three_d && tid < TY + TX) {  // south edge face of column tid-TY
-        int c = tid - TY;
-        fy[0][c] = hface<SCHEME, T>(tile[0][c + 2], tile[1][c + 2], tile[2][c + 2], tile[3][c + 2], vs_e, A, A.dy1);
-    }
-    T Fz = zface<SCHEME, T>(k, nz, zw, wt, A);
-    __syncthreads();
-    if (!valid) {
-        fzb = Fz;
-        return;
-    }
-    double div = 0.0;
-    if (A.dirmask & 1) div += (double(Fx) - double(fx[ty][tx])) * A.dx1;
-    if (A.three_d && (A.dirmask & 2)) div += (double(Fy) - double(fy[ty][tx])) * A.dy1;
-    if (A.dirmask & 4) div += (double(Fz) - double(fzb)) * dz1w_k;
-    fzb = Fz;
-    double a = -div;
-    T *out = P::out(A, f);
-    if (A.epi == 0) {
-        out[id] = T(a);
-        return;
-    }
-    double qn;
-    if (A.c0 != 0.0) qn = A.c0 * double(P::q0(A, f)[id]) + A.c1 * (double(zw[1]) + A.dt * a);
-    else qn = A.c1 * (double(zw[1]) + A.dt * a);
-    T qv = T(qn);
-    out[id] = qv;
-    if (A.fill) {
-        const int nx = A.nx, ny = A.ny, mx = A.mx, my = A.my;
-        int i = i0 + tx, j = j0 + ty;
-        int ix = 0, jy = 0;
-        if (i >= nx - 4) ix = i - mx;
-        else if (i <= 5) ix = i + mx;
-        if (A.three_d) {
-            if (j >= ny - 4) jy = j - my;
-            else if (j <= 5) jy = j + my;
+        for (int s = 0; s < I::TS; ++s) {
+            const int e = tid + s * I::NT;
+            t_off[s] = -1;
+            if (e < I::LH * I::LW) {
+                const int r = e / I::LW, c = e - r * I::LW;
+                const int ii = wrap(i0 - 2 + c, nx, A.mx);
+                const int jj = I::THREE_D ? wrap(j0 - 2 + r, ny, A.my) : 1;
+                t_off[s] = (ii - 1) + nx * (jj - 1);
+            }
         }
-        size_t kb = id - ((size_t)(i - 1) + (size_t)nx * (j - 1));
-        if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (j - 1)] = qv;
-        if (jy) {
-            out[kb + (size_t)(i - 1) + (size_t)nx * (jy - 1)] = qv;
-            if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qv;
+#pragma unroll
+        for (int s = 0; s < I::FS; ++s) {
+            const int e = tid + s * I::NT;
+            f_base[s] = 0;
+            f_step[s] = 0;
+            f_off[s] = 0;
+            if (e < I::NXF) {
+                const int r = e / (TX + 1), fl = e - r * (TX + 1);  // x face f = i0-1+fl, row j0+r
+                f_base[s] = (I::THREE_D ? r + 2 : 0) * I::LW + fl;
+                f_step[s] = 1;
+                const int jr = I::THREE_D ? wrap(j0 + r, ny, A.my) : 1;
+                f_off[s] = (wrap(i0 + fl, nx, A.mx) - 1) + nx * (jr - 1);
+            } else if (e < I::NF) {
+                const int e2 = e - I::NXF, fr = e2 / TX, c = e2 - fr * TX;  // y face g = j0-1+fr, column i0+c
+                f_base[s] = fr * I::LW + c + 2;
+                f_step[s] = I::LW;
+                f_off[s] = (wrap(i0 + c, nx, A.mx) - 1) + nx * (wrap(j0 + fr, ny, A.my) - 1);
+            }
         }
+    }
+};
+
+// One field marched up the item's levels.  Per level: the prefetched tile
+// goes to LDS, the next level's tile, face velocities, window value and
+// step-start value are requested, every thread evaluates its face slots
+// and the top z face of its column (register window q(k-1..k+2)), then
+// finishes its cell: FP64 flux divergence of the FP32/FP64 face values and
+// the Shu-Osher stage update, rounded once to the field's precision, with
+// the periodic ghost images.
+template <int SCHEME, int TX, int TY, class T>
+__device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restrict__ q, const T *__restrict__ q0,
+                                            T *__restrict__ out, T *lt, T *lf, int k0, int nk, int i0, int j0,
+                                            int tid, int tx, int ty, bool valid, int col,
+                                            const Slots<TX, TY> &sl)
+{
+    using I = Item<TX, TY>;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t plane = (size_t)nx * ny;
+    auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
+    // window q(k0-1..k0+2) with q(0) := q(1), carried bottom face of k0
+    T zw[4] = {q[col + lev(k0 - 1)], q[col + lev(k0)], q[col + lev(k0 + 1)], q[col + lev(k0 + 2)]};
+    if (k0 == 1) zw[0] = zw[1];
+    T fzb = T(0);
+    if (k0 > 1 && (A.dirmask & 4)) {
+        T zz[4] = {q[col + lev(k0 - 2)], zw[0], zw[1], zw[2]};
+        if (k0 == 2) zz[0] = zz[1];
+        fzb = zface<SCHEME, T>(k0 - 1, nz, zz, T(A.w[col + lev(k0)]), A);
+    }
+    T tv[I::TS];
+    float vv[I::FS], wv;
+    T q0v = T(0), zn = T(0);
+    double d1v = 0.0;
+    const bool need_q0 = A.epi != 0 && A.c0 != 0.0;
+    // everything level k needs from memory, requested one level ahead
+    auto fetch = [&](int k) {
+        const size_t kb = lev(k);
+        const T *qk = q + kb;
+#pragma unroll
+        for (int s = 0; s < I::TS; ++s)
+            if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
+#pragma unroll
+        for (int s = 0; s < I::FS; ++s)
+            vv[s] = sl.f_step[s] == 1 ? A.u[kb + sl.f_off[s]] : (sl.f_step[s] != 0 ? A.v[kb + sl.f_off[s]] : 0.0f);
+        wv = k < nz ? A.w[col + kb + plane] : 0.0f;
+        if (need_q0 && valid) q0v = q0[col + kb];
+        zn = q[col + lev(k + 2)];
+        d1v = A.dz1w[k - 1];
+    };
+    fetch(k0);
+#pragma unroll 1
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = k0 + kk;
+#pragma unroll
+        for (int s = 0; s < I::TS; ++s)
+            if (sl.t_off[s] >= 0) lt[tid + s * I::NT] = tv[s];
+        float vc[I::FS];
+#pragma unroll
+        for (int s = 0; s < I::FS; ++s) vc[s] = vv[s];
+        const float wc = wv;
+        const T q0c = q0v;
+        const double dz1w_k = d1v;
+        // window q(k-1..k+2): the level's top value arrived one level ahead
+        if (kk > 0) {
+            zw[0] = zw[1];
+            zw[1] = zw[2];
+            zw[2] = zw[3];
+            zw[3] = zn;
+        }
+        __syncthreads();
+        if (kk + 1 < nk) fetch(k + 1);
+#pragma unroll
+        for (int s = 0; s < I::FS; ++s) {
+            const int st = sl.f_step[s];
+            if (st == 0) continue;
+            const int b = sl.f_base[s];
+            lf[tid + s * I::NT] = hface<SCHEME, T>(lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], T(vc[s]), A,
+                                                   st == 1 ? A.dx1 : A.dy1);
+        }
+        const T fzt = (A.dirmask & 4) ? zface<SCHEME, T>(k, nz, zw, T(wc), A) : T(0);
+        __syncthreads();
+        if (valid) {
+            const size_t id = col + (size_t)(k - 1) * plane;
+            double div = 0.0;
+            if (A.dirmask & 1) {
+                const int e = ty * (TX + 1) + tx;
+                div += (double(lf[e + 1]) - double(lf[e])) * A.dx1;
+            }
+            if (I::THREE_D && (A.dirmask & 2)) {
+                const int e = I::NXF + ty * TX + tx;
+                div += (double(lf[e + TX]) - double(lf[e])) * A.dy1;
+            }
+            if (A.dirmask & 4) div += (double(fzt) - double(fzb)) * dz1w_k;
+            const double a = -div;
+            if (A.epi == 0) {
+                out[id] = T(a);
+            } else {
+                double qn;
+                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[1]) + A.dt * a);
+                else qn = A.c1 * (double(zw[1]) + A.dt * a);
+                const T qv = T(qn);
+                out[id] = qv;
+                if (A.fill) {
+                    const int i = i0 + tx, j = j0 + ty;
+                    int ix = 0, jy = 0;
+                    if (i >= nx - 4) ix = i - A.mx;
+                    else if (i <= 5) ix = i + A.mx;
+                    if (I::THREE_D) {
+                        if (j >= ny - 4) jy = j - A.my;
+                        else if (j <= 5) jy = j + A.my;
+                    }
+                    const size_t kb = (size_t)(k - 1) * plane;
+                    if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (j - 1)] = qv;
+                    if (jy) {
+                        out[kb + (size_t)(i - 1) + (size_t)nx * (jy - 1)] = qv;
+                        if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qv;
+                    }
+                }
+            }
+        }
+        fzb = fzt;
     }
 }
 
-template <int SCHEME, int N64, int N32, int TX, int TY>
+// T: the precision of every field of the launch (FP64 and FP32 members of
+// a batch go to two launches, so that the FP32 kernel keeps the register
+// budget of FP32 arithmetic).
+template <int SCHEME, int TX, int TY, class T>
 __global__ __launch_bounds__(TX *TY) void fused_kernel(FusedArgs A)
 {
-    constexpr int NT = TX * TY;
-    constexpr int LW = TX + 4, LH = TY + 4;
-    constexpr int NHALO = LW * LH - TX * TY;
-    constexpr int HPT = (NHALO + NT - 1) / NT;  // halo cells per thread
-    __shared__ double tile64[N64 > 0 ? LH : 1][LW];
-    __shared__ double fx64[N64 > 0 ? TY : 1][TX + 1];
-    __shared__ double fy64[N64 > 0 ? TY + 1 : 1][TX];
-    __shared__ float tile32[N32 > 0 ? LH : 1][LW];
-    __shared__ float fx32[N32 > 0 ? TY : 1][TX + 1];
-    __shared__ float fy32[N32 > 0 ? TY + 1 : 1][TX];
+    using I = Item<TX, TY>;
+    __shared__ T tile_lds[I::LH * I::LW];
+    __shared__ T face_lds[I::FS * I::NT];
 
     const int nx = A.nx, ny = A.ny, nz = A.nz;
-    const size_t plane = (size_t)nx * ny;
-    const int tid = threadIdx.x;
-    const int tx = tid % TX, ty = tid / TX;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int tiles = A.ntx * A.nty;
+    const int kc = A.kchunk;
 
     for (int wi = blockIdx.x; wi < A.nwork; wi += gridDim.x) {
         // XCD-aware order: consecutive work ids go round-robin over the 8
-        // XCDs; give every XCD a contiguous range of tiles so that tile halos
-        // and the k-chunks of one column stay in one L2.
-        int tiles = A.ntx * A.nty;
-        int chunk = wi / tiles;
+        // XCDs; every XCD gets a contiguous range of tiles so that tile
+        // halos and velocity rows stay in one L2.
+        const int chunk = wi / tiles;
         int t = wi - chunk * tiles;
         if (A.xcd_remap) {
-            int per = tiles / 8;
+            const int per = tiles / 8;
             t = (t % 8) * per + t / 8;
         }
-        int bx = t % A.ntx, by = t / A.ntx;
-        int i0 = 4 + bx * TX;
-        int j0 = A.three_d ? 4 + by * TY : 1;
-        int ka = 1 + chunk * A.kchunk;
-        int kb = min(nz, ka + A.kchunk - 1);
-        int i = i0 + tx, j = j0 + ty;
-        bool valid = i <= nx - 2 && (!A.three_d || j <= ny - 2) && (A.three_d || ty == 0);
-        int il = wrap(i, nx, A.mx);
-        int jl = A.three_d ? wrap(j, ny, A.my) : 1;
-        size_t col = (size_t)(il - 1) + (size_t)nx * (jl - 1);
-
-        // halo cells owned by this thread (fixed for the work item); slot h
-        // is static so the bookkeeping stays in registers
-        int hal_r[HPT], hal_c[HPT];
-        size_t hal_col[HPT];
-        bool hal_ok[HPT];
-#pragma unroll
-        for (int h = 0; h < HPT; ++h) {
-            int e = tid + h * NT;
-            // ring enumeration: rows 0,1 and LH-2,LH-1 full; middle rows
-            // columns 0,1,LW-2,LW-1
-            int r = 0, c = 0;
-            const int top = 2 * LW;
-            if (e < top) { r = e / LW; c = e % LW; }
-            else if (e < 2 * top) { int e2 = e - top; r = LH - 2 + e2 / LW; c = e2 % LW; }
-            else {
-                int e3 = e - 2 * top;
-                r = 2 + e3 / 4;
-                int cc = e3 % 4;
-                c = cc < 2 ? cc : LW - 4 + cc;
-            }
-            hal_ok[h] = e < NHALO && (A.three_d || r == 2);
-            int ii = wrap(i0 - 2 + c, nx, A.mx);
-            int jj = A.three_d ? wrap(j0 - 2 + r, ny, A.my) : 1;
-            hal_r[h] = r;
-            hal_c[h] = c;
-            hal_col[h] = hal_ok[h] ? (size_t)(ii - 1) + (size_t)nx * (jj - 1) : 0;
+        const int i0 = 4 + (t % A.ntx) * TX;
+        const int j0 = I::THREE_D ? 4 + (t / A.ntx) * TY : 1;
+        const int k0 = 1 + chunk * kc;
+        const int nk = min(kc, nz - k0 + 1);
+        const int i = i0 + tx, j = j0 + ty;
+        const bool valid = i <= nx - 2 && (!I::THREE_D || j <= ny - 2);
+        const int il = wrap(i, nx, A.mx), jl = I::THREE_D ? wrap(j, ny, A.my) : 1;
+        const int col = (il - 1) + nx * (jl - 1);
+        Slots<TX, TY> sl;
+        sl.init(tid, i0, j0, A);
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll 1
+            for (int f = 0; f < A.n64_active; ++f)
+                march_field<SCHEME, TX, TY, T>(A, A.q64[f], A.q0_64[f], A.out64[f], tile_lds, face_lds, k0, nk, i0, j0,
+                                               tid, tx, ty, valid, col, sl);
+        } else {
+#pragma unroll 1
+            for (int f = 0; f < A.n32_active; ++f)
+                march_field<SCHEME, TX, TY, T>(A, A.q32[f], A.q0_32[f], A.out32[f], tile_lds, face_lds, k0, nk, i0, j0,
+                                               tid, tx, ty, valid, col, sl);
         }
-
-        // register windows q(k-1..k+2) with q(0) := q(1); carried bottom face
-        double z64[N64 > 0 ? N64 : 1][4];
-        double fzb64[N64 > 0 ? N64 : 1];
-        float z32[N32 > 0 ? N32 : 1][4];
-        float fzb32[N32 > 0 ? N32 : 1];
-        auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
-#pragma unroll
-        for (int f = 0; f < N64; ++f) {
-            fzb64[f] = 0.0;
-            if (f >= A.n64_active) continue;
-            const double *q = A.q64[f];
-#pragma unroll
-            for (int s = 0; s < 3; ++s) z64[f][s + 1] = q[col + lev(ka - 1 + s)];
-            z64[f][0] = z64[f][1];
-        }
-#pragma unroll
-        for (int f = 0; f < N32; ++f) {
-            fzb32[f] = 0.0f;
-            if (f >= A.n32_active) continue;
-            const float *q = A.q32[f];
-#pragma unroll
-            for (int s = 0; s < 3; ++s) z32[f][s + 1] = q[col + lev(ka - 1 + s)];
-            z32[f][0] = z32[f][1];
-        }
-        // carried bottom face of the first level (zero at the floor)
-        if (ka > 1) {
-            float wb = A.w[col + lev(ka)];
-#pragma unroll
-            for (int f = 0; f < N64; ++f) {
-                if (f >= A.n64_active) continue;
-                const double *q = A.q64[f];
-                double zz[4] = {q[col + lev(ka - 2)], z64[f][1], z64[f][2], z64[f][3]};
-                fzb64[f] = zface<SCHEME, double>(ka - 1, nz, zz, double(wb), A);
-            }
-#pragma unroll
-            for (int f = 0; f < N32; ++f) {
-                if (f >= A.n32_active) continue;
-                const float *q = A.q32[f];
-                float zz[4] = {q[col + lev(ka - 2)], z32[f][1], z32[f][2], z32[f][3]};
-                fzb32[f] = zface<SCHEME, float>(ka - 1, nz, zz, wb, A);
-            }
-        }
-
-        for (int k = ka; k <= kb; ++k) {
-            size_t kbase = (size_t)(k - 1) * plane;
-            size_t id = col + kbase;
-            // velocities of the level, shared by every field
-            int ie = wrap(il + 1, nx, A.mx);
-            float ue = A.u[kbase + (size_t)(ie - 1) + (size_t)nx * (jl - 1)];
-            float vn = 0.0f;
-            if (A.three_d) {
-                int je = wrap(jl + 1, ny, A.my);
-                vn = A.v[kbase + (size_t)(il - 1) + (size_t)nx * (je - 1)];
-            }
-            float wt = (k < nz) ? A.w[col + kbase + plane] : 0.0f;
-            float uw_e = 0.0f, vs_e = 0.0f;
-            if (tid < TY) {
-                int jr = A.three_d ? wrap(j0 + tid, ny, A.my) : 1;
-                uw_e = A.u[kbase + (size_t)(i0 - 1) + (size_t)nx * (jr - 1)];
-            } else if (A.three_d && tid < TY + TX) {
-                int ic = wrap(i0 + tid - TY, nx, A.mx);
-                vs_e = A.v[kbase + (size_t)(ic - 1) + (size_t)nx * (j0 - 1)];
-            }
-            double dz1w_k = A.dz1w[k - 1];
-            // issue every field's window refill and halo loads first
-            size_t kn = lev(k + 2);
-            double h64[N64 > 0 ? N64 : 1][HPT];
-            float h32[N32 > 0 ? N32 : 1][HPT];
-#pragma unroll
-            for (int f = 0; f < N64; ++f) {
-                if (f >= A.n64_active) continue;
-                z64[f][0] = z64[f][1]; z64[f][1] = z64[f][2]; z64[f][2] = z64[f][3];
-                z64[f][3] = A.q64[f][col + kn];
-#pragma unroll
-                for (int h = 0; h < HPT; ++h)
-                    if (hal_ok[h]) h64[f][h] = A.q64[f][hal_col[h] + kbase];
-            }
-#pragma unroll
-            for (int f = 0; f < N32; ++f) {
-                if (f >= A.n32_active) continue;
-                z32[f][0] = z32[f][1]; z32[f][1] = z32[f][2]; z32[f][2] = z32[f][3];
-                z32[f][3] = A.q32[f][col + kn];
-#pragma unroll
-                for (int h = 0; h < HPT; ++h)
-                    if (hal_ok[h]) h32[f][h] = A.q32[f][hal_col[h] + kbase];
-            }
-            if (k == 1) {
-                // q(0) := q(1) for the first window
-#pragma unroll
-                for (int f = 0; f < N64; ++f) z64[f][0] = z64[f][1];
-#pragma unroll
-                for (int f = 0; f < N32; ++f) z32[f][0] = z32[f][1];
-            }
-#pragma unroll
-            for (int f = 0; f < N64; ++f)
-                if (f < A.n64_active)
-                do_field<SCHEME, TX, TY, double>(A, f, k, tid, tx, ty, i0, j0, valid, id, z64[f], fzb64[f], double(ue),
-                                                 double(vn), double(uw_e), double(vs_e), double(wt), dz1w_k, tile64,
-                                                 fx64, fy64, hal_r, hal_c, h64[f], hal_ok);
-#pragma unroll
-            for (int f = 0; f < N32; ++f)
-                if (f < A.n32_active)
-                do_field<SCHEME, TX, TY, float>(A, f, k, tid, tx, ty, i0, j0, valid, id, z32[f], fzb32[f], ue, vn,
-                                                uw_e, vs_e, wt, dz1w_k, tile32, fx32, fy32, hal_r, hal_c, h32[f], hal_ok);
-        }
-        __syncthreads();
     }
 }
 
-template <int SCHEME, int N64, int N32>
-hipError_t launch_combo(const FusedArgs &a0, hipStream_t s, int blocks_per_cu)
+template <int SCHEME, int TX, int TY, class T>
+hipError_t launch_tiles(const FusedArgs &a0, hipStream_t s, int blocks_per_cu)
 {
     FusedArgs A = a0;
-    constexpr int TX = 64, TY = 4;
     A.ntx = (A.nx - 5 + TX - 1) / TX;
-    A.nty = A.three_d ? (A.ny - 5 + TY - 1) / TY : 1;
-    int tiles = A.ntx * A.nty;
+    A.nty = TY > 1 ? (A.ny - 5 + TY - 1) / TY : 1;
+    const int tiles = A.ntx * A.nty;
     A.xcd_remap = (tiles % 8 == 0) ? 1 : 0;
-    if (A.kchunk <= 0) A.kchunk = A.nz;
-    int nch = (A.nz + A.kchunk - 1) / A.kchunk;
-    A.nwork = tiles * nch;
+    if (A.kchunk <= 0 || A.kchunk > A.nz) A.kchunk = A.kchunk <= 0 ? min(32, A.nz) : A.nz;
+    A.nwork = tiles * ((A.nz + A.kchunk - 1) / A.kchunk);
     int dev = 0, ncu = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
     int occ = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fused_kernel<SCHEME, N64, N32, TX, TY>, TX * TY, 0);
-    if (occ < 1) occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fused_kernel<SCHEME, TX, TY, T>, TX * TY, 0) !=
+            hipSuccess ||
+        occ < 1)
+        occ = 1;
     if (blocks_per_cu > 0 && blocks_per_cu < occ) occ = blocks_per_cu;
-    int grid = min(A.nwork, ncu * occ);
-    hipLaunchKernelGGL((fused_kernel<SCHEME, N64, N32, TX, TY>), dim3(grid), dim3(TX * TY), 0, s, A);
+    const int grid = min(A.nwork, ncu * occ);
+    hipLaunchKernelGGL((fused_kernel<SCHEME, TX, TY, T>), dim3(grid), dim3(TX * TY), 0, s, A);
     return hipGetLastError();
 }
 
-template <int SCHEME, int N64>
-hipError_t launch_n32(int n32, const FusedArgs &A, hipStream_t s, int bpc)
+template <int SCHEME, class T>
+hipError_t launch_prec(const FusedArgs &A, hipStream_t s, int bpc)
 {
-    switch (n32) {
-    case 0: return launch_combo<SCHEME, N64, 0>(A, s, bpc);
-    case 1: return launch_combo<SCHEME, N64, 1>(A, s, bpc);
-    case 2: return launch_combo<SCHEME, N64, 2>(A, s, bpc);
-    case 4: return launch_combo<SCHEME, N64, 4>(A, s, bpc);
-    case 8: return launch_combo<SCHEME, N64, 8>(A, s, bpc);
-    case 16: return launch_combo<SCHEME, N64, 16>(A, s, bpc);
-    case 18: return launch_combo<SCHEME, N64, 18>(A, s, bpc);
-    default: return hipErrorInvalidValue;
+    if (!A.three_d) return launch_tiles<SCHEME, 64, 1, T>(A, s, bpc);
+    switch (A.tile_y) {
+    case 2: return launch_tiles<SCHEME, 64, 2, T>(A, s, bpc);
+    case 4: return launch_tiles<SCHEME, 64, 4, T>(A, s, bpc);
+    default: return launch_tiles<SCHEME, 64, 8, T>(A, s, bpc);
     }
+}
+
+template <int SCHEME>
+hipError_t launch_scheme(const FusedArgs &A, hipStream_t s, int bpc)
+{
+    hipError_t e = hipSuccess;
+    if (A.n64_active > 0) e = launch_prec<SCHEME, double>(A, s, bpc);
+    if (e == hipSuccess && A.n32_active > 0) e = launch_prec<SCHEME, float>(A, s, bpc);
+    return e;
 }
 
 }  // namespace
 
 int fused_padded_count(int n32)
 {
-    static const int sizes[] = {0, 1, 2, 4, 8, 16, 18};
-    for (int s : sizes)
-        if (s >= n32) return s;
-    return -1;
+    return (n32 >= 0 && n32 <= FUSED_MAX32) ? n32 : -1;
 }
 
 hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStream_t s, int blocks_per_cu)
 {
-    if (scheme == 0) {
-        switch (n64) {
-        case 0: return launch_n32<0, 0>(n32, A, s, blocks_per_cu);
-        case 1: return launch_n32<0, 1>(n32, A, s, blocks_per_cu);
-        case 2: return launch_n32<0, 2>(n32, A, s, blocks_per_cu);
-        default: return hipErrorInvalidValue;
-        }
-    }
-    if (scheme == 2) {
-        switch (n64) {
-        case 0: return launch_n32<2, 0>(n32, A, s, blocks_per_cu);
-        case 1: return launch_n32<2, 1>(n32, A, s, blocks_per_cu);
-        case 2: return launch_n32<2, 2>(n32, A, s, blocks_per_cu);
-        default: return hipErrorInvalidValue;
-        }
-    }
+    if (n64 < 0 || n64 > FUSED_MAX64 || n32 < 0 || n32 > FUSED_MAX32) return hipErrorInvalidValue;
+    if (scheme == 0) return launch_scheme<0>(A, s, blocks_per_cu);
+    if (scheme == 2) return launch_scheme<2>(A, s, blocks_per_cu);
     return hipErrorInvalidValue;
 }
 

@@ -26,10 +26,11 @@ struct FusedArgs {
     double lim_tol;
     int weno_bottom;
     int ntx, nty, kchunk, nwork, xcd_remap;
+    int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 8)
     int dirmask;  // directions included in the divergence (bit0 x, bit1 y, bit2 z)
 };
 
-// Padded FP32 field count with a compiled kernel (-1 if too many).
+// FP32 field count of one launch (-1 if above FUSED_MAX32).
 int fused_padded_count(int n32);
 hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStream_t s, int blocks_per_cu);
 

@@ -191,9 +191,10 @@ int mimadv_get_conserved(mimadv_ctx *ctx, const double *dens, int nf, double *co
 
 /* Tuning of the batched launches; 0 selects the default of each field. */
 typedef struct mimadv_tuning {
-    int group32;        /* FP32 fields per launch (default 8) */
-    int kchunk;         /* levels per work item (default: whole column) */
+    int group32;        /* FP32 fields per launch (default and maximum 18) */
+    int kchunk;         /* levels per work item (default 32) */
     int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
+    int tile_y;         /* rows of the 64-wide column tile in 3D: 2, 4 or 8 (default 8) */
 } mimadv_tuning;
 int mimadv_set_tuning(mimadv_ctx *ctx, const mimadv_tuning *t);
 
