@@ -9,5 +9,6 @@ synthetic-case builders.
 from ._lib import MimadvError, load  # noqa: F401
 from .advection import Advection  # noqa: F401
 from .grid import Grid, stretched, uniform  # noqa: F401
+from .multidomain import MultiDomain  # noqa: F401
 
-__all__ = ["Advection", "Grid", "MimadvError", "load", "stretched", "uniform"]
+__all__ = ["Advection", "Grid", "MimadvError", "MultiDomain", "load", "stretched", "uniform"]

@@ -55,6 +55,14 @@ EXPORTED = (
     "mimadv_get_momentum",
     "mimadv_get_advective",
     "mimadv_get_conserved",
+    "mimadv_multi_create",
+    "mimadv_multi_destroy",
+    "mimadv_multi_last_error",
+    "mimadv_multi_set_tiles",
+    "mimadv_multi_get_tiles",
+    "mimadv_multi_autotune",
+    "mimadv_multi_advance",
+    "mimadv_multi_last_ms",
 )
 
 
@@ -158,6 +166,22 @@ def load() -> C.CDLL:
     lib.mimadv_get_advective.restype = i
     lib.mimadv_get_conserved.argtypes = [vp, vp, i, pp, i, u]
     lib.mimadv_get_conserved.restype = i
+    lib.mimadv_multi_create.argtypes = [i, pp, pp, pp, pp, ip, C.POINTER(pp), C.POINTER(vp)]
+    lib.mimadv_multi_create.restype = i
+    lib.mimadv_multi_destroy.argtypes = [vp]
+    lib.mimadv_multi_destroy.restype = None
+    lib.mimadv_multi_last_error.argtypes = [vp]
+    lib.mimadv_multi_last_error.restype = C.c_char_p
+    lib.mimadv_multi_set_tiles.argtypes = [vp, ip, ip]
+    lib.mimadv_multi_set_tiles.restype = i
+    lib.mimadv_multi_get_tiles.argtypes = [vp, ip, ip, ip]
+    lib.mimadv_multi_get_tiles.restype = i
+    lib.mimadv_multi_autotune.argtypes = [vp, i]
+    lib.mimadv_multi_autotune.restype = i
+    lib.mimadv_multi_advance.argtypes = [vp, i, i, C.POINTER(d)]
+    lib.mimadv_multi_advance.restype = i
+    lib.mimadv_multi_last_ms.argtypes = [vp, C.POINTER(d)]
+    lib.mimadv_multi_last_ms.restype = i
     _lib = lib
     return lib
 

@@ -1083,3 +1083,306 @@ int mimadv_get_conserved(mimadv_ctx *c, const double *dens, int nf, double *cons
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// many independent subdomains in one launch
+// ---------------------------------------------------------------------------
+
+struct mimadv_multi {
+    int ndom = 0;
+    std::vector<mimadv_ctx *> ctx;
+    std::vector<const float *> u, v, w;
+    std::vector<std::vector<float *>> q;
+    std::vector<int> tile_x, kchunk;
+    std::vector<size_t> n3;
+    float *work = nullptr;               // two stage arrays per field
+    std::vector<std::vector<float *>> s1, s2;
+    FusedArgs *d_table = nullptr;        // one table per stage kind (3 x ndom)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int mfail(mimadv_multi *m, int code, const std::string &msg)
+{
+    if (m) m->err = msg;
+    return code;
+}
+
+#define MHIPCHK(m, expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return mfail(m, MIMADV_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// cost model: face evaluations of the padded tile cover per interior cell
+int model_tile_x(int mx, int my)
+{
+    static const int cand[4] = {64, 32, 128, 16};
+    int best = 64;
+    double bestc = 1e300;
+    for (int tx : cand) {
+        const int ty = MULTI_NT / tx;
+        const double cells = double((mx + tx - 1) / tx * tx) * double((my + ty - 1) / ty * ty);
+        const double c = cells * (2.0 + 1.0 / tx + 1.0 / ty) / (double(mx) * my);
+        if (c < bestc - 1e-12) {
+            bestc = c;
+            best = tx;
+        }
+    }
+    return best;
+}
+
+// argument block of subdomain d for one stage: in/out buffers 0 q, 1 s1, 2 s2
+FusedArgs multi_args(mimadv_multi *m, int d, double c0, double c1, double dt, int in, int out)
+{
+    mimadv_ctx *c = m->ctx[d];
+    FusedArgs A = base_args(c, m->u[d], m->v[d], m->w[d]);
+    A.epi = 1;
+    A.fill = 1;
+    A.c0 = c0;
+    A.c1 = c1;
+    A.dt = dt;
+    A.scheme = c->sp.scheme;
+    A.tile_x = m->tile_x[d];
+    A.tile_y = MULTI_NT / m->tile_x[d];
+    A.kchunk = m->kchunk[d];
+    A.n64_active = 0;
+    A.n32_active = (int)m->q[d].size();
+    auto pick = [&](int which, int f) -> float * {
+        return which == 0 ? m->q[d][f] : (which == 1 ? m->s1[d][f] : m->s2[d][f]);
+    };
+    for (int f = 0; f < A.n32_active; ++f) {
+        A.q32[f] = pick(in, f);
+        A.q0_32[f] = m->q[d][f];
+        A.out32[f] = pick(out, f);
+    }
+    return A;
+}
+
+// number the work items of a table, upload it into table slot `slot`;
+// returns the work count in *nwork
+int multi_upload(mimadv_multi *m, std::vector<FusedArgs> &tab, int slot, int *nwork)
+{
+    int begin = 0;
+    for (FusedArgs &A : tab) {
+        multi_prepare(A, begin);
+        begin += A.nwork;
+    }
+    MHIPCHK(m, hipMemcpyAsync(m->d_table + (size_t)slot * m->ndom, tab.data(), tab.size() * sizeof(FusedArgs),
+                              hipMemcpyHostToDevice, m->stream));
+    *nwork = begin;
+    return MIMADV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mimadv_multi_create(int ndom, mimadv_ctx *const *ctx, const float *const *u, const float *const *v,
+                        const float *const *w, const int *nf, float *const *const *q, mimadv_multi **out)
+{
+    if (!out) return MIMADV_ERR_ARG;
+    *out = nullptr;
+    mimadv_multi *m = new (std::nothrow) mimadv_multi();
+    if (!m) return MIMADV_ERR_NOMEM;
+    *out = m;
+    if (ndom <= 0 || !ctx || !u || !v || !w || !nf || !q) return mfail(m, MIMADV_ERR_ARG, "bad subdomain list");
+    m->ndom = ndom;
+    size_t total = 0;
+    for (int d = 0; d < ndom; ++d) {
+        mimadv_ctx *c = ctx[d];
+        if (!c) return mfail(m, MIMADV_ERR_ARG, "NULL context");
+        if (c->g.ny == 1) return mfail(m, MIMADV_ERR_UNSUPPORTED, "subdomains must be 3D");
+        int rc = check_batch(c, u[d], v[d], w[d]);
+        if (rc) return mfail(m, rc, "subdomain " + std::to_string(d) + ": " + c->err);
+        if (nf[d] < 0 || nf[d] > FUSED_MAX32) return mfail(m, MIMADV_ERR_ARG, "1..18 fields per subdomain");
+        if (nf[d] > 0 && !q[d]) return mfail(m, MIMADV_ERR_ARG, "NULL field list");
+        m->ctx.push_back(c);
+        m->u.push_back(u[d]);
+        m->v.push_back(v[d]);
+        m->w.push_back(w[d]);
+        std::vector<float *> fl;
+        for (int f = 0; f < nf[d]; ++f) {
+            if (!q[d][f]) return mfail(m, MIMADV_ERR_ARG, "NULL field pointer");
+            fl.push_back(q[d][f]);
+        }
+        m->q.push_back(fl);
+        m->n3.push_back(c->n3());
+        m->tile_x.push_back(model_tile_x(c->g.nx - 5, c->g.ny - 5));
+        m->kchunk.push_back(0);
+        total += 2 * c->n3() * (size_t)nf[d];
+    }
+    m->stream = ctx[0]->stream;
+    if (total > 0 && hipMalloc(&m->work, total * sizeof(float)) != hipSuccess)
+        return mfail(m, MIMADV_ERR_NOMEM, "stage array allocation failed");
+    size_t off = 0;
+    for (int d = 0; d < ndom; ++d) {
+        std::vector<float *> a, b;
+        for (size_t f = 0; f < m->q[d].size(); ++f) {
+            a.push_back(m->work + off);
+            off += m->n3[d];
+            b.push_back(m->work + off);
+            off += m->n3[d];
+            // stage arrays start as copies so that their ghost columns are defined
+            MHIPCHK(m, hipMemcpyAsync(a.back(), m->q[d][f], m->n3[d] * sizeof(float), hipMemcpyDeviceToDevice,
+                                      m->stream));
+            MHIPCHK(m, hipMemcpyAsync(b.back(), m->q[d][f], m->n3[d] * sizeof(float), hipMemcpyDeviceToDevice,
+                                      m->stream));
+        }
+        m->s1.push_back(a);
+        m->s2.push_back(b);
+    }
+    if (hipMalloc(&m->d_table, 3 * (size_t)ndom * sizeof(FusedArgs)) != hipSuccess)
+        return mfail(m, MIMADV_ERR_NOMEM, "table allocation failed");
+    MHIPCHK(m, hipEventCreate(&m->ev0));
+    MHIPCHK(m, hipEventCreate(&m->ev1));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return MIMADV_OK;
+}
+
+void mimadv_multi_destroy(mimadv_multi *m)
+{
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    if (m->work) (void)hipFree(m->work);
+    if (m->d_table) (void)hipFree(m->d_table);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    delete m;
+}
+
+const char *mimadv_multi_last_error(const mimadv_multi *m)
+{
+    return m ? m->err.c_str() : "NULL plan";
+}
+
+int mimadv_multi_set_tiles(mimadv_multi *m, const int *tile_x, const int *kchunk)
+{
+    if (!m) return MIMADV_ERR_ARG;
+    for (int d = 0; d < m->ndom; ++d) {
+        if (tile_x && !multi_tile_ok(tile_x[d], MULTI_NT / (tile_x[d] > 0 ? tile_x[d] : 1)))
+            return mfail(m, MIMADV_ERR_ARG, "tile_x must be 16, 32, 64 or 128");
+        if (kchunk && kchunk[d] < 0) return mfail(m, MIMADV_ERR_ARG, "kchunk < 0");
+    }
+    for (int d = 0; d < m->ndom; ++d) {
+        if (tile_x) m->tile_x[d] = tile_x[d];
+        if (kchunk) m->kchunk[d] = kchunk[d];
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int *kchunk)
+{
+    if (!m) return MIMADV_ERR_ARG;
+    for (int d = 0; d < m->ndom; ++d) {
+        if (tile_x) tile_x[d] = m->tile_x[d];
+        if (tile_y) tile_y[d] = MULTI_NT / m->tile_x[d];
+        if (kchunk) kchunk[d] = m->kchunk[d] > 0 ? std::min(m->kchunk[d], m->ctx[d]->g.nz)
+                                                 : std::min(32, m->ctx[d]->g.nz);
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_multi_autotune(mimadv_multi *m, int reps)
+{
+    if (!m) return MIMADV_ERR_ARG;
+    if (reps < 1) reps = 1;
+    static const int cand_x[4] = {64, 32, 128, 16};
+    static const int cand_k[2] = {16, 32};
+    struct Key {
+        int mx, my, nz, nf, scheme;
+        bool operator==(const Key &o) const
+        {
+            return mx == o.mx && my == o.my && nz == o.nz && nf == o.nf && scheme == o.scheme;
+        }
+    };
+    std::vector<std::pair<Key, std::pair<int, int>>> memo;
+    for (int d = 0; d < m->ndom; ++d) {
+        mimadv_ctx *c = m->ctx[d];
+        Key key{c->g.nx - 5, c->g.ny - 5, c->g.nz, (int)m->q[d].size(), c->sp.scheme};
+        bool found = false;
+        for (auto &e : memo)
+            if (e.first == key) {
+                m->tile_x[d] = e.second.first;
+                m->kchunk[d] = e.second.second;
+                found = true;
+                break;
+            }
+        if (found || m->q[d].empty()) continue;
+        float best = 1e30f;
+        int bx = m->tile_x[d], bk = m->kchunk[d];
+        for (int tx : cand_x) {
+            for (int kc : cand_k) {
+                m->tile_x[d] = tx;
+                m->kchunk[d] = kc;
+                // a middle stage into s2 from q: the fields are only read
+                std::vector<FusedArgs> tab{multi_args(m, d, 0.75, 0.25, 1e-3, 0, 2)};
+                int nw = 0;
+                int rc = multi_upload(m, tab, 0, &nw);
+                if (rc) return rc;
+                MHIPCHK(m, launch_multi(m->d_table, 1, nw, m->stream));  // warm-up
+                MHIPCHK(m, hipEventRecord(m->ev0, m->stream));
+                for (int r = 0; r < reps; ++r) MHIPCHK(m, launch_multi(m->d_table, 1, nw, m->stream));
+                MHIPCHK(m, hipEventRecord(m->ev1, m->stream));
+                MHIPCHK(m, hipEventSynchronize(m->ev1));
+                float ms = 0.0f;
+                MHIPCHK(m, hipEventElapsedTime(&ms, m->ev0, m->ev1));
+                if (ms < best) {
+                    best = ms;
+                    bx = tx;
+                    bk = kc;
+                }
+            }
+        }
+        m->tile_x[d] = bx;
+        m->kchunk[d] = bk;
+        memo.push_back({key, {bx, bk}});
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_multi_advance(mimadv_multi *m, int integrator, int nsteps, const double *dt)
+{
+    if (!m) return MIMADV_ERR_ARG;
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return mfail(m, MIMADV_ERR_UNSUPPORTED, "integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
+    if (nsteps < 0 || !dt) return mfail(m, MIMADV_ERR_ARG, "bad step count or dt");
+    struct Stage {
+        double c0, c1;
+        int in, out;
+    };
+    std::vector<Stage> st;
+    if (integrator == MIMADV_INT_HEUN) st = {{0.0, 1.0, 0, 1}, {0.5, 0.5, 1, 0}};
+    else st = {{0.0, 1.0, 0, 1}, {0.75, 0.25, 1, 2}, {1.0 / 3.0, 2.0 / 3.0, 2, 0}};
+    std::vector<std::vector<FusedArgs>> tabs(st.size());
+    std::vector<int> nwork(st.size(), 0);
+    for (size_t s = 0; s < st.size(); ++s) {
+        for (int d = 0; d < m->ndom; ++d)
+            if (!m->q[d].empty()) tabs[s].push_back(multi_args(m, d, st[s].c0, st[s].c1, dt[d], st[s].in, st[s].out));
+        int rc = multi_upload(m, tabs[s], (int)s, &nwork[s]);
+        if (rc) return rc;
+    }
+    MHIPCHK(m, hipEventRecord(m->ev0, m->stream));
+    for (int n = 0; n < nsteps; ++n)
+        for (size_t s = 0; s < st.size(); ++s)
+            MHIPCHK(m, launch_multi(m->d_table + s * m->ndom, (int)tabs[s].size(), nwork[s], m->stream));
+    MHIPCHK(m, hipEventRecord(m->ev1, m->stream));
+    // the host tables must outlive their (pageable, staged) upload
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return MIMADV_OK;
+}
+
+int mimadv_multi_last_ms(mimadv_multi *m, double *ms)
+{
+    if (!m || !ms) return MIMADV_ERR_ARG;
+    MHIPCHK(m, hipEventSynchronize(m->ev1));
+    float t = 0.0f;
+    MHIPCHK(m, hipEventElapsedTime(&t, m->ev0, m->ev1));
+    *ms = t;
+    return MIMADV_OK;
+}
+
+}  // extern "C"

@@ -27,11 +27,23 @@ struct FusedArgs {
     int weno_bottom;
     int ntx, nty, kchunk, nwork, xcd_remap;
     int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 8)
+    // multi-domain launches only
+    int scheme;        // 0 WENO, 2 MUSCL
+    int tile_x;        // tile width; tile_x * tile_y = 512
+    int work_begin;    // first work id of this subdomain in the launch
     int dirmask;  // directions included in the divergence (bit0 x, bit1 y, bit2 z)
 };
 
 // FP32 field count of one launch (-1 if above FUSED_MAX32).
 int fused_padded_count(int n32);
 hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStream_t s, int blocks_per_cu);
+
+// Multi-domain launch: table[0..ndom-1] on the device, one entry per
+// subdomain (FP32 fields only, 3D, tile_x * tile_y = 512, ntx/nty/kchunk/
+// work_begin filled in by multi_prepare), nwork the total work count.
+constexpr int MULTI_NT = 512;
+bool multi_tile_ok(int tile_x, int tile_y);
+void multi_prepare(FusedArgs &A, int work_begin);  // sets ntx, nty, nwork; kchunk clamped
+hipError_t launch_multi(const FusedArgs *d_table, int ndom, int nwork, hipStream_t s);
 
 }  // namespace mimadv

@@ -1,0 +1,189 @@
+// fused_faces.hpp -- per-face formulas of the batched kernels (FP32/FP64):
+// the reference WENO face (advq_weno.f90) and MUSCL face (advq_muscl.f90)
+// in operation-reduced form, their vertical closures and the periodic
+// index wrap.  Shared by the single-grid and multi-domain batched kernels.
+#pragma once
+#include "fused.hpp"
+
+namespace mimadv {
+namespace faces {
+
+template <class T>
+__device__ __forceinline__ T rcp(T x);
+template <>
+__device__ __forceinline__ float rcp<float>(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+template <>
+__device__ __forceinline__ double rcp<double>(double x)
+{
+    // hardware estimate refined by two Newton steps (error ~1 ulp)
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+template <class T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+
+// Reference WENO face flux, operation-reduced.  a..d = q(f-1..f+2), uu the
+// face velocity, nonlin: whether the smoothness switch is active at this face
+// (always in x/y, k = 3..nz-3 in z).
+template <class T>
+__device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
+{
+    const T eps = T(1e-15), thr = T(20);
+    T ffim1 = T(1.5) * b - T(0.5) * a;
+    T ffip1 = T(0.5) * (b + c);
+    T ffip2 = T(1.5) * c - T(0.5) * d;
+    // limiter_weno: rescale about b (the left/lower cell)
+    T minp = tmin(tmin(ffim1, ffip1), ffip2);
+    T phi = tmin(fabs(b * rcp((minp - b) + eps)), T(1));
+    bool pos = uu > T(0);
+    T e2 = (c - b) * (c - b);
+    T oth = pos ? (b - a) : (d - c);
+    T eo = oth * oth;
+    T fo = pos ? ffim1 : ffip2;
+    T wo = T(1.0 / 3.0);
+    if (nonlin) {
+        T mx = tmax(eo, e2), mn = tmin(eo, e2);
+        if (mx > thr * (mn + eps)) {
+            T so = eps + eo;
+            so = so * so;
+            T si = eps + e2;
+            si = si * si;
+            // (g1/so) / (g1/so + g2/si) with g2 = 2 g1
+            wo = si * rcp(si + T(2) * so);
+        }
+    }
+    T F = ffip1 + wo * (fo - ffip1);
+    F = b + phi * (F - b);
+    return uu * F;
+}
+
+// top closure k = nz-1 of advsz_weno (no limiter, linear weights)
+template <class T>
+__device__ __forceinline__ T weno_top_face(T a, T b, T c, T uu)
+{
+    T ffim1 = T(1.5) * b - T(0.5) * a;
+    T ffip1 = T(0.5) * (b + c);
+    T F = uu > T(0) ? T(1.0 / 3.0) * ffim1 + T(2.0 / 3.0) * ffip1 : ffip1;
+    return uu * F;
+}
+
+// MUSCL face (advq_muscl.f90), x/y form with slopes in index units.
+template <class T>
+__device__ __forceinline__ T muscl_phi(int lim, T r)
+{
+    switch (lim) {
+    case 1: return tmax(T(0), tmin(tmin(T(2) * r, T(0.5) * (r + T(1))), T(2)));
+    case 2: return (r + fabs(r)) * rcp(T(1) + fabs(r));
+    case 0: return tmax(T(0), tmin(T(1), r));
+    default: return T(0.5) * (r + T(1));
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T muscl_face(T a, T b, T c, T d, T uu, int lim, T lt, T d1)
+{
+    const T eps = T(1e-18);
+    T sp = c - b, sp2 = d - c, sm = b - a;  // slopes times dx
+    T fl, fr;
+    T tol = lt * fabs(b) + eps;
+    if (fabs(sm) > tol || fabs(sp2) > tol) {
+        // limiter_muscl compares per-metre slopes with lim_tol*|x|
+        // (advq_muscl.f90:361-362)
+        T rp1 = T(1), rp2 = T(1);
+        if (fabs(sm) * d1 > lt * fabs(b) && fabs(sp) < fabs(sm) * rcp(lt)) rp1 = sp * rcp(sm);
+        if (fabs(sp2) * d1 > lt * fabs(b) && fabs(sp) < fabs(sp2) * rcp(lt)) rp2 = sp * rcp(sp2);
+        fr = c - T(0.5) * muscl_phi(lim, rp2) * sp2;
+        fl = b + T(0.5) * muscl_phi(lim, rp1) * sm;
+    } else {
+        fr = c - T(0.5) * sp;
+        fl = b + T(0.5) * sp;
+    }
+    return uu * (uu > T(0) ? fl : fr);
+}
+
+// MUSCL z face f (1..nz-1) with the reference's stretched metrics
+template <class T>
+__device__ __forceinline__ T muscl_zface(int f, int nz, T a, T b, T c, T d, T ww, const FusedArgs &A)
+{
+    const T eps = T(1e-18);
+    int lim = A.muscl_limiter;
+    T lt = T(A.lim_tol), dz = T(A.dz);
+    const double *dz1 = A.dz1, *fdz0 = A.fdz0;
+    T m = T(A.dzw[f - 1]);  // 1/dz1w(f)
+    T sp = (c - b) * T(dz1[f - 1]);
+    T sm, sp2;
+    bool limited;
+    if (f == 1) {
+        sm = sp;  // no-flux bottom: bc51 = 0
+        sp2 = (d - c) * T(dz1[1]);
+        limited = T(fdz0[1]) * dz * fabs(sp2) > lt * fabs(b) + eps;
+    } else if (f == nz - 1) {
+        sm = (b - a) * T(dz1[f - 2]);
+        sp2 = sp;  // no-flux top: bc61 = 0
+        limited = T(fdz0[f - 2]) * dz * fabs(sm) > lt * fabs(b) + eps;
+    } else {
+        sm = (b - a) * T(dz1[f - 2]);
+        sp2 = (d - c) * T(dz1[f]);
+        limited = T(fdz0[f - 2]) * dz * fabs(sm) > lt * fabs(b) + eps ||
+                  T(fdz0[f]) * dz * fabs(sp2) > lt * fabs(b) + eps;
+    }
+    T fl, fr;
+    if (limited) {
+        T rp1 = T(1), rp2 = T(1);
+        if (fabs(sm) > lt * fabs(b) && fabs(sp) < fabs(sm) * rcp(lt)) rp1 = sp * rcp(sm);
+        if (fabs(sp2) > lt * fabs(b) && fabs(sp) < fabs(sp2) * rcp(lt)) rp2 = sp * rcp(sp2);
+        fr = c - T(0.5) * muscl_phi(lim, rp2) * sp2 * m;
+        fl = b + T(0.5) * muscl_phi(lim, rp1) * sm * m;
+    } else {
+        fr = c - T(0.5) * sp * m;
+        fl = b + T(0.5) * sp * m;
+    }
+    return ww * (ww > T(0) ? fl : fr);
+}
+
+template <int SCHEME, class T>
+__device__ __forceinline__ T hface(T a, T b, T c, T d, T uu, const FusedArgs &A, double d1)
+{
+    if constexpr (SCHEME == 0) return weno_face(a, b, c, d, uu, true);
+    else return muscl_face(a, b, c, d, uu, A.muscl_limiter, T(A.lim_tol), T(d1));
+}
+
+template <int SCHEME, class T>
+__device__ __forceinline__ T zface(int f, int nz, const T *zw, T ww, const FusedArgs &A)
+{
+    // zw = q(f-1), q(f), q(f+1), q(f+2) (q(0) := q(1))
+    if (f >= nz) return T(0);
+    if constexpr (SCHEME == 0) {
+        if (f == nz - 1) return weno_top_face(zw[0], zw[1], zw[2], ww);
+        if (f == 1 && A.weno_bottom == 1) {
+            T ffip1 = T(0.5) * (zw[1] + zw[2]);
+            T ffip2 = T(1.5) * zw[2] - T(0.5) * zw[3];
+            T F = ww > T(0) ? ffip1 : T(2.0 / 3.0) * ffip1 + T(1.0 / 3.0) * ffip2;
+            return ww * F;
+        }
+        return weno_face(zw[0], zw[1], zw[2], zw[3], ww, f >= 3 && f <= nz - 3);
+    } else {
+        return muscl_zface(f, nz, zw[0], zw[1], zw[2], zw[3], ww, A);
+    }
+}
+
+__device__ __forceinline__ int wrap(int i, int n, int m)
+{
+    // 1-based index i mapped into [1, n]; outside positions take their
+    // periodic interior image (period m, interior starts at 4)
+    if (i >= 1 && i <= n) return i;
+    int r = (i - 4) % m;
+    return (r < 0 ? r + m : r) + 4;
+}
+
+}  // namespace faces
+}  // namespace mimadv

@@ -1,0 +1,257 @@
+// multi.hip -- one launch advecting many independent periodic subdomains.
+//
+// A batch of unequal subdomains (each with its own grid, metrics, FP32
+// velocity, dt, scheme and set of FP32 fields) is advanced by one
+// persistent launch per Shu-Osher stage.  The subdomains' work items (a
+// tile of columns and a chunk of levels) are numbered consecutively; a
+// workgroup finds its subdomain by a binary search of the first work id of
+// every subdomain, reads that subdomain's argument block from a device
+// table, and runs the same per-field column march as the single-grid
+// batched kernel (fused.hip): tile plus 2-cell halo in LDS, flattened x/y
+// face list, z face carried up the column, FP64 divergence and stage
+// update rounded once to FP32.
+//
+// The tile shape is chosen per subdomain (TX x TY with TX*TY = 512:
+// 16x32, 32x16, 64x8, 128x4) so that small or narrow subdomains waste
+// fewer lanes; faces are evaluated with the same formulas as the
+// single-grid kernel, so the results do not depend on the shape.
+#include "fused.hpp"
+#include "fused_faces.hpp"
+
+namespace mimadv {
+
+namespace {
+
+using namespace faces;
+
+constexpr int NT = MULTI_NT;
+constexpr int FS = 3;                  // 2*NT + TX + TY faces per tile
+constexpr int TS = 3;                  // (TX+4)*(TY+4) tile values
+constexpr int TILE_MAX = 132 * 8;      // largest (TX+4)*(TY+4) of the shapes
+
+struct Geom {
+    int TX, TY, LW, LH, NXF, NF;
+    __device__ __forceinline__ void set(int tx_, int ty_)
+    {
+        TX = tx_;
+        TY = ty_;
+        LW = TX + 4;
+        LH = TY + 4;
+        NXF = TY * (TX + 1);
+        NF = NXF + (TY + 1) * TX;
+    }
+};
+
+struct SlotsR {
+    int t_off[TS];
+    int f_base[FS], f_step[FS], f_off[FS];
+    __device__ __forceinline__ void init(const Geom &G, int tid, int i0, int j0, const FusedArgs &A)
+    {
+        const int nx = A.nx, ny = A.ny;
+#pragma unroll
+        for (int s = 0; s < TS; ++s) {
+            const int e = tid + s * NT;
+            t_off[s] = -1;
+            if (e < G.LH * G.LW) {
+                const int r = e / G.LW, c = e - r * G.LW;
+                t_off[s] = (wrap(i0 - 2 + c, nx, A.mx) - 1) + nx * (wrap(j0 - 2 + r, ny, A.my) - 1);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < FS; ++s) {
+            const int e = tid + s * NT;
+            f_base[s] = 0;
+            f_step[s] = 0;
+            f_off[s] = 0;
+            if (e < G.NXF) {
+                const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);  // x face f = i0-1+fl, row j0+r
+                f_base[s] = (r + 2) * G.LW + fl;
+                f_step[s] = 1;
+                f_off[s] = (wrap(i0 + fl, nx, A.mx) - 1) + nx * (wrap(j0 + r, ny, A.my) - 1);
+            } else if (e < G.NF) {
+                const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;  // y face g = j0-1+fr
+                f_base[s] = fr * G.LW + c + 2;
+                f_step[s] = G.LW;
+                f_off[s] = (wrap(i0 + c, nx, A.mx) - 1) + nx * (wrap(j0 + fr, ny, A.my) - 1);
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ float face_any(const FusedArgs &A, float a, float b, float c, float d, float uu, double d1)
+{
+    return A.scheme == 0 ? hface<0, float>(a, b, c, d, uu, A, d1) : hface<2, float>(a, b, c, d, uu, A, d1);
+}
+
+__device__ __forceinline__ float zface_any(const FusedArgs &A, int f, int nz, const float *zw, float ww)
+{
+    return A.scheme == 0 ? zface<0, float>(f, nz, zw, ww, A) : zface<2, float>(f, nz, zw, ww, A);
+}
+
+__device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, const float *__restrict__ q,
+                                            const float *__restrict__ q0, float *__restrict__ out, float *lt,
+                                            float *lf, int k0, int nk, int i0, int j0, int tid, int tx, int ty,
+                                            bool valid, int col, const SlotsR &sl)
+{
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const size_t plane = (size_t)nx * ny;
+    auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
+    // a chunk above the floor first evaluates its bottom face in a z-only
+    // pass of the level loop: every z face comes from the same code
+    const int kstart = k0 > 1 ? -1 : 0;
+    const int ks = k0 + kstart;
+    float zw[4] = {q[col + lev(ks - 1)], q[col + lev(ks)], q[col + lev(ks + 1)], q[col + lev(ks + 2)]};
+    if (ks == 1) zw[0] = zw[1];
+    const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
+    float fzb = 0.0f;
+    float tv[TS], vv[FS], wv, q0v = 0.0f, zn = 0.0f;
+    double d1v = 0.0;
+    const bool need_q0 = A.epi != 0 && A.c0 != 0.0;
+    auto fetch = [&](int k) {
+        const size_t kb = lev(k);
+        const float *qk = q + kb;
+#pragma unroll
+        for (int s = 0; s < TS; ++s)
+            if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
+#pragma unroll
+        for (int s = 0; s < FS; ++s)
+            vv[s] = sl.f_step[s] == 1 ? A.u[kb + sl.f_off[s]] : (sl.f_step[s] != 0 ? A.v[kb + sl.f_off[s]] : 0.0f);
+        wv = k < nz ? A.w[col + kb + plane] : 0.0f;
+        if (need_q0 && valid) q0v = q0[col + kb];
+        zn = q[col + lev(k + 2)];
+        d1v = A.dz1w[k - 1];
+    };
+    const int ex = ty * (G.TX + 1) + tx, ey = G.NXF + ty * G.TX + tx;
+    fetch(k0);
+#pragma unroll 1
+    for (int kk = kstart; kk < nk; ++kk) {
+        const int k = k0 + kk;
+        if (kk > kstart) {
+            zw[0] = zw[1];
+            zw[1] = zw[2];
+            zw[2] = zw[3];
+            zw[3] = zn;
+        }
+        const float fzt = zface_any(A, k, nz, zw, kk < 0 ? wz0 : wv);
+        if (kk < 0) {
+            fzb = fzt;
+            continue;
+        }
+#pragma unroll
+        for (int s = 0; s < TS; ++s)
+            if (sl.t_off[s] >= 0) lt[tid + s * NT] = tv[s];
+        float vc[FS];
+#pragma unroll
+        for (int s = 0; s < FS; ++s) vc[s] = vv[s];
+        const float q0c = q0v;
+        const double dz1w_k = d1v;
+        __syncthreads();
+        if (kk + 1 < nk) fetch(k + 1);
+#pragma unroll
+        for (int s = 0; s < FS; ++s) {
+            const int st = sl.f_step[s];
+            if (st == 0) continue;
+            const int b = sl.f_base[s];
+            lf[tid + s * NT] = face_any(A, lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], vc[s],
+                                        st == 1 ? A.dx1 : A.dy1);
+        }
+        __syncthreads();
+        if (valid) {
+            const size_t id = col + (size_t)(k - 1) * plane;
+            double div = (double(lf[ex + 1]) - double(lf[ex])) * A.dx1;
+            div += (double(lf[ey + G.TX]) - double(lf[ey])) * A.dy1;
+            div += (double(fzt) - double(fzb)) * dz1w_k;
+            const double a = -div;
+            if (A.epi == 0) {
+                out[id] = float(a);
+            } else {
+                double qn;
+                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[1]) + A.dt * a);
+                else qn = A.c1 * (double(zw[1]) + A.dt * a);
+                const float qv = float(qn);
+                out[id] = qv;
+                if (A.fill) {
+                    const int i = i0 + tx, j = j0 + ty;
+                    int ix = 0, jy = 0;
+                    if (i >= nx - 4) ix = i - A.mx;
+                    else if (i <= 5) ix = i + A.mx;
+                    if (j >= ny - 4) jy = j - A.my;
+                    else if (j <= 5) jy = j + A.my;
+                    const size_t kb = (size_t)(k - 1) * plane;
+                    if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (j - 1)] = qv;
+                    if (jy) {
+                        out[kb + (size_t)(i - 1) + (size_t)nx * (jy - 1)] = qv;
+                        if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qv;
+                    }
+                }
+            }
+        }
+        fzb = fzt;
+    }
+}
+
+__global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__ table, int ndom, int nwork)
+{
+    __shared__ float tile_lds[TILE_MAX];
+    __shared__ float face_lds[FS * NT];
+    const int tid = threadIdx.x;
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        // subdomain of this work id: last d with work_begin <= wi
+        int lo = 0, hi = ndom - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (table[mid].work_begin <= wi) lo = mid;
+            else hi = mid - 1;
+        }
+        const FusedArgs &A = table[lo];
+        Geom G;
+        G.set(A.tile_x, A.tile_y);
+        const int local = wi - A.work_begin;
+        const int tiles = A.ntx * A.nty;
+        const int chunk = local / tiles, t = local - chunk * tiles;
+        const int i0 = 4 + (t % A.ntx) * G.TX, j0 = 4 + (t / A.ntx) * G.TY;
+        const int k0 = 1 + chunk * A.kchunk, nk = min(A.kchunk, A.nz - k0 + 1);
+        const int tx = tid % G.TX, ty = tid / G.TX;
+        const int i = i0 + tx, j = j0 + ty;
+        const bool valid = i <= A.nx - 2 && j <= A.ny - 2;
+        const int col = (wrap(i, A.nx, A.mx) - 1) + A.nx * (wrap(j, A.ny, A.my) - 1);
+        SlotsR sl;
+        sl.init(G, tid, i0, j0, A);
+#pragma unroll 1
+        for (int f = 0; f < A.n32_active; ++f)
+            march_multi(A, G, A.q32[f], A.q0_32[f], A.out32[f], tile_lds, face_lds, k0, nk, i0, j0, tid, tx, ty, valid,
+                        col, sl);
+    }
+}
+
+}  // namespace
+
+bool multi_tile_ok(int tile_x, int tile_y)
+{
+    return tile_x * tile_y == MULTI_NT && (tile_x == 16 || tile_x == 32 || tile_x == 64 || tile_x == 128);
+}
+
+void multi_prepare(FusedArgs &A, int work_begin)
+{
+    A.ntx = (A.nx - 5 + A.tile_x - 1) / A.tile_x;
+    A.nty = (A.ny - 5 + A.tile_y - 1) / A.tile_y;
+    if (A.kchunk <= 0 || A.kchunk > A.nz) A.kchunk = A.kchunk <= 0 ? min(32, A.nz) : A.nz;
+    A.nwork = A.ntx * A.nty * ((A.nz + A.kchunk - 1) / A.kchunk);
+    A.work_begin = work_begin;
+}
+
+hipError_t launch_multi(const FusedArgs *d_table, int ndom, int nwork, hipStream_t s)
+{
+    if (ndom <= 0 || nwork <= 0) return hipSuccess;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, multi_kernel, NT, 0) != hipSuccess || occ < 1) occ = 1;
+    const int grid = min(nwork, ncu * occ);
+    hipLaunchKernelGGL(multi_kernel, dim3(grid), dim3(NT), 0, s, d_table, ndom, nwork);
+    return hipGetLastError();
+}
+
+}  // namespace mimadv

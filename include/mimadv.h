@@ -221,6 +221,37 @@ int mimadv_batch_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt,
  * FP64 (out[f]). */
 int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const void *const *q, double *out);
 
+/* ---- many independent subdomains in one launch (no reference counterpart:
+ * the reference runs one domain per MPI job) ----
+ * Subdomain d is described by a context ctx[d] (its grid, periodic lateral
+ * and 'nnf' vertical boundaries, scheme MIMADV_WENO or MIMADV_MUSCL, 3D),
+ * its FP32 velocity (u[d], v[d], w[d]) and nf[d] FP32 fields q[d][0..nf-1]
+ * (device arrays, at most 18 per subdomain).  Every stage of every
+ * subdomain is one work list of one persistent launch; the tile shape
+ * (tile_x * tile_y = 512 with tile_x in 16, 32, 64, 128) and the level
+ * chunk are chosen per subdomain.  The plan keeps two stage arrays per
+ * field and runs on the stream of ctx[0]. */
+typedef struct mimadv_multi mimadv_multi;
+int mimadv_multi_create(int ndom, mimadv_ctx *const *ctx, const float *const *u, const float *const *v,
+                        const float *const *w, const int *nf, float *const *const *q, mimadv_multi **out);
+void mimadv_multi_destroy(mimadv_multi *m);
+const char *mimadv_multi_last_error(const mimadv_multi *m);
+/* Tile shape and level chunk per subdomain (arrays of ndom; kchunk 0 =
+ * default).  Without a call, a cost model picks the shape that wastes the
+ * fewest face evaluations. */
+int mimadv_multi_set_tiles(mimadv_multi *m, const int *tile_x, const int *kchunk);
+int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int *kchunk);
+/* Autotuner: times every candidate (tile shape x level chunk in 16, 32) of
+ * every subdomain with single-subdomain launches of one stage, reps times
+ * each, keeps the fastest, and reuses a result for subdomains of equal size,
+ * field count and scheme.  The fields are not modified. */
+int mimadv_multi_autotune(mimadv_multi *m, int reps);
+/* nsteps Shu-Osher steps (MIMADV_INT_SSPRK3 or MIMADV_INT_HEUN) of every
+ * subdomain with its own dt[d]; one launch per stage for the whole batch. */
+int mimadv_multi_advance(mimadv_multi *m, int integrator, int nsteps, const double *dt);
+/* Time of the last mimadv_multi_advance between HIP events on its stream. */
+int mimadv_multi_last_ms(mimadv_multi *m, double *ms);
+
 /* ---- adjoint of the plain FP64 'weno' path (no reference counterpart: the
  * reference has no adjoint or tangent-linear model) ----
  * Scheme MIMADV_WENO in flux form with the default floor convention

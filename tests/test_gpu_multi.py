@@ -1,0 +1,128 @@
+"""Many uneven periodic subdomains in one launch per stage.
+
+* the batch advances every subdomain exactly as its own single-grid batched
+  launch does (same face formulas, FP64 divergence, one rounding), to FP32
+  round-off;
+* the tile shape and level chunk chosen per subdomain (by hand or by the
+  autotuner) change only the work split: results are bit for bit equal;
+* against the FP64 oracle trajectory the FP32 fields (without exact zeros)
+  stay within 1e-5 relative RMS.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fdd3260_amd import Advection, MultiDomain, _lib, cases
+from fdd3260_amd import grid as gm
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(20, 16, 12, "weno"), (33, 17, 14, "muscl"), (48, 40, 16, "weno"), (16, 50, 10, "muscl"),
+         (70, 24, 12, "weno")]
+
+
+def _batch(cuda, seed=201):
+    doms = []
+    for d, (mx, my, nz, scheme) in enumerate(SIZES):
+        g = gm.stretched(mx, my, nz)
+        u, v, w = cases.turbulent_wind(g, seed=seed + 10 * d, rms=1.0 + d * 0.7)
+        u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
+        dt = cases.cfl_dt(g, u, v, w, 0.5)
+        nspec = 2 + d % 3
+        qs = [cases.lognormal(g, seed=seed + 10 * d + s + 1, mu=-12 + 3 * s, sigma=1.0).astype(np.float32)
+              .astype(np.float64) for s in range(nspec)]
+        doms.append(dict(g=g, u=u, v=v, w=w, dt=dt, qs=qs, scheme=scheme))
+    return doms
+
+
+def _tensors(cuda, dom):
+    vel = tuple(torch.from_numpy(a.astype(np.float32)).to(cuda) for a in (dom["u"], dom["v"], dom["w"]))
+    fl = [torch.from_numpy(q.astype(np.float32)).to(cuda) for q in dom["qs"]]
+    return vel, fl
+
+
+def _run_multi(cuda, doms, nsteps, tiles=None, kchunk=None, autotune=False):
+    advs = [Advection(d["g"], scal_adv=d["scheme"], muscllimiter="MC") for d in doms]
+    vels, fields = zip(*[_tensors(cuda, d) for d in doms])
+    torch.cuda.synchronize()
+    with MultiDomain(advs, list(vels), list(fields)) as m:
+        if tiles is not None or kchunk is not None:
+            m.set_tiles(tiles, kchunk)
+        if autotune:
+            m.autotune(reps=1)
+        chosen = m.tiles()
+        m.advance(_lib.INT_SSPRK3, nsteps, [d["dt"] for d in doms])
+        out = [[f.double().cpu().numpy() for f in fl] for fl in fields]
+    for a in advs:
+        a.close()
+    return out, chosen
+
+
+def test_multi_matches_single_grid_batches(cuda):
+    print("case multi vs single-grid batches", flush=True)
+    doms = _batch(cuda)
+    got, _ = _run_multi(cuda, doms, 3)
+    for d, dom in enumerate(doms):
+        with Advection(dom["g"], scal_adv=dom["scheme"], muscllimiter="MC") as a:
+            (u, v, w), fl = _tensors(cuda, dom)
+            a.batch_advance(_lib.INT_SSPRK3, 3, dom["dt"], u, v, w, fl)
+            a.stream_sync()
+            for n, f in enumerate(fl):
+                ref = f.double().cpu().numpy()
+                I = dom["g"].interior()
+                err = np.abs(got[d][n] - ref)[I].max() / np.abs(ref[I]).max()
+                assert err <= 1e-6, (d, n, err)
+
+
+@pytest.mark.parametrize("tile_x", [16, 32, 64, 128])
+def test_multi_tile_shape_is_bit_invariant(cuda, tile_x):
+    print(f"case multi tile_x={tile_x}", flush=True)
+    doms = _batch(cuda, seed=301)
+    ref, _ = _run_multi(cuda, doms, 2)
+    got, chosen = _run_multi(cuda, doms, 2, tiles=[tile_x] * len(doms), kchunk=[5] * len(doms))
+    assert all(c[0] == tile_x and c[0] * c[1] == 512 for c in chosen)
+    for a, b in zip(ref, got):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_multi_autotune_keeps_results(cuda):
+    print("case multi autotune", flush=True)
+    doms = _batch(cuda, seed=401)
+    ref, _ = _run_multi(cuda, doms, 2)
+    got, chosen = _run_multi(cuda, doms, 2, autotune=True)
+    assert all(c[0] in (16, 32, 64, 128) and c[0] * c[1] == 512 and (c[2] in (16, 32) or c[2] == dom["g"].nz)
+               for c, dom in zip(chosen, doms))
+    for a, b in zip(ref, got):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
+def test_multi_against_oracle(cuda):
+    print("case multi vs oracle", flush=True)
+    doms = _batch(cuda, seed=501)
+    got, _ = _run_multi(cuda, doms, 3)
+    for d, dom in enumerate(doms):
+        ora = Oracle(dom["g"], scal_adv=dom["scheme"], muscllimiter="MC")
+        I = dom["g"].interior()
+        for n, q in enumerate(dom["qs"]):
+            r = q.copy()
+            for _ in range(3):
+                ora.ssprk3_step(dom["dt"], dom["u"], dom["v"], dom["w"], r)
+            rms = np.sqrt(np.mean((got[d][n] - r)[I] ** 2)) / np.sqrt(np.mean(r[I] ** 2))
+            assert rms <= 1e-5, (d, n, rms)
+
+
+def test_multi_rejects_bad_input(cuda):
+    g2 = gm.stretched(20, 0, 12)
+    with Advection(g2) as a:
+        t = torch.zeros(g2.shape, dtype=torch.float32, device=cuda)
+        with pytest.raises(_lib.MimadvError, match="3D"):
+            MultiDomain([a], [(t, t, t)], [[t.clone()]])
+    g = gm.stretched(20, 16, 12)
+    with Advection(g) as a:
+        t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
+        with MultiDomain([a], [(t, t, t)], [[t.clone()]]) as m:
+            with pytest.raises(_lib.MimadvError, match="tile_x"):
+                m.set_tiles([48])
