@@ -579,6 +579,7 @@ FusedArgs base_args(mimadv_ctx *c, const float *u, const float *v, const float *
     A.w = w;
     A.muscl_limiter = c->sp.muscl_limiter;
     A.lim_tol = c->sp.lim_tol;
+    A.lim_tol_inv = 1.0 / c->sp.lim_tol;
     A.weno_bottom = c->sp.weno_bottom;
     A.kchunk = c->tuning.kchunk;
     A.tile_y = c->tuning.tile_y;
@@ -1291,7 +1292,20 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
     if (!m) return MIMADV_ERR_ARG;
     if (reps < 1) reps = 1;
     static const int cand_x[4] = {64, 32, 128, 16};
-    static const int cand_k[2] = {16, 32};
+    static const int cand_k[3] = {16, 32, 64};
+    // A candidate is timed at saturation: the subdomain's work list is
+    // repeated until the launch holds at least `target` work items (the
+    // copies write the same values to the same scratch stage array), so a
+    // small subdomain is not rewarded for merely creating more work items.
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
+    const int target = 8 * ncu;
+    const int max_copies = 256;
+    FusedArgs *d_tune = nullptr;
+    if (hipMalloc(&d_tune, (size_t)max_copies * sizeof(FusedArgs)) != hipSuccess)
+        return mfail(m, MIMADV_ERR_NOMEM, "autotune table allocation failed");
     struct Key {
         int mx, my, nz, nf, scheme;
         bool operator==(const Key &o) const
@@ -1300,7 +1314,8 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
         }
     };
     std::vector<std::pair<Key, std::pair<int, int>>> memo;
-    for (int d = 0; d < m->ndom; ++d) {
+    int status = MIMADV_OK;
+    for (int d = 0; d < m->ndom && status == MIMADV_OK; ++d) {
         mimadv_ctx *c = m->ctx[d];
         Key key{c->g.nx - 5, c->g.ny - 5, c->g.nz, (int)m->q[d].size(), c->sp.scheme};
         bool found = false;
@@ -1312,36 +1327,47 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
                 break;
             }
         if (found || m->q[d].empty()) continue;
-        float best = 1e30f;
+        double best = 1e300;
         int bx = m->tile_x[d], bk = m->kchunk[d];
         for (int tx : cand_x) {
             for (int kc : cand_k) {
+                if (kc > 2 * c->g.nz) continue;
                 m->tile_x[d] = tx;
                 m->kchunk[d] = kc;
                 // a middle stage into s2 from q: the fields are only read
-                std::vector<FusedArgs> tab{multi_args(m, d, 0.75, 0.25, 1e-3, 0, 2)};
-                int nw = 0;
-                int rc = multi_upload(m, tab, 0, &nw);
-                if (rc) return rc;
-                MHIPCHK(m, launch_multi(m->d_table, 1, nw, m->stream));  // warm-up
-                MHIPCHK(m, hipEventRecord(m->ev0, m->stream));
-                for (int r = 0; r < reps; ++r) MHIPCHK(m, launch_multi(m->d_table, 1, nw, m->stream));
-                MHIPCHK(m, hipEventRecord(m->ev1, m->stream));
-                MHIPCHK(m, hipEventSynchronize(m->ev1));
+                FusedArgs one = multi_args(m, d, 0.75, 0.25, 1e-3, 0, 2);
+                multi_prepare(one, 0);
+                const int copies = std::min(max_copies, std::max(1, (target + one.nwork - 1) / one.nwork));
+                std::vector<FusedArgs> tab(copies, one);
+                for (int r = 0; r < copies; ++r) tab[r].work_begin = r * one.nwork;
+                const int nw = copies * one.nwork;
+                hipError_t e = hipMemcpy(d_tune, tab.data(), tab.size() * sizeof(FusedArgs), hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = launch_multi(d_tune, copies, nw, m->stream);  // warm-up
+                if (e == hipSuccess) e = hipEventRecord(m->ev0, m->stream);
+                for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_multi(d_tune, copies, nw, m->stream);
+                if (e == hipSuccess) e = hipEventRecord(m->ev1, m->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(m->ev1);
                 float ms = 0.0f;
-                MHIPCHK(m, hipEventElapsedTime(&ms, m->ev0, m->ev1));
-                if (ms < best) {
-                    best = ms;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, m->ev0, m->ev1);
+                if (e != hipSuccess) {
+                    status = mfail(m, MIMADV_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+                    break;
+                }
+                const double per = (double)ms / copies;
+                if (per < best) {
+                    best = per;
                     bx = tx;
                     bk = kc;
                 }
             }
+            if (status != MIMADV_OK) break;
         }
         m->tile_x[d] = bx;
         m->kchunk[d] = bk;
         memo.push_back({key, {bx, bk}});
     }
-    return MIMADV_OK;
+    (void)hipFree(d_tune);
+    return status;
 }
 
 int mimadv_multi_advance(mimadv_multi *m, int integrator, int nsteps, const double *dt)

@@ -23,7 +23,7 @@ struct FusedArgs {
     double c0, c1, dt;           // out = c0*q0 + c1*(q + dt*(-div))
     int fill;                    // write periodic ghost images
     int muscl_limiter;
-    double lim_tol;
+    double lim_tol, lim_tol_inv;
     int weno_bottom;
     int ntx, nty, kchunk, nwork, xcd_remap;
     int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 8)

@@ -88,38 +88,43 @@ __device__ __forceinline__ T muscl_phi(int lim, T r)
     }
 }
 
+// Only the upwind side's face value is formed: the reference forms both
+// (fl from the slope ratio sp/sm, fr from sp/sp2) and keeps one by the sign
+// of u; the limited/centred switch still looks at both outer slopes.
+// rlt = 1/lim_tol.
 template <class T>
-__device__ __forceinline__ T muscl_face(T a, T b, T c, T d, T uu, int lim, T lt, T d1)
+__device__ __forceinline__ T muscl_face(T a, T b, T c, T d, T uu, int lim, T lt, T rlt, T d1)
 {
     const T eps = T(1e-18);
-    T sp = c - b, sp2 = d - c, sm = b - a;  // slopes times dx
-    T fl, fr;
-    T tol = lt * fabs(b) + eps;
+    const bool pos = uu > T(0);
+    const T sp = c - b, sm = b - a, sp2 = d - c;  // slopes times dx
+    const T so = pos ? sm : sp2;                  // upwind outer slope
+    const T base = pos ? b : c, h = pos ? T(0.5) : T(-0.5);
+    const T tol = lt * fabs(b) + eps;
+    T F;
     if (fabs(sm) > tol || fabs(sp2) > tol) {
         // limiter_muscl compares per-metre slopes with lim_tol*|x|
         // (advq_muscl.f90:361-362)
-        T rp1 = T(1), rp2 = T(1);
-        if (fabs(sm) * d1 > lt * fabs(b) && fabs(sp) < fabs(sm) * rcp(lt)) rp1 = sp * rcp(sm);
-        if (fabs(sp2) * d1 > lt * fabs(b) && fabs(sp) < fabs(sp2) * rcp(lt)) rp2 = sp * rcp(sp2);
-        fr = c - T(0.5) * muscl_phi(lim, rp2) * sp2;
-        fl = b + T(0.5) * muscl_phi(lim, rp1) * sm;
+        T r = T(1);
+        if (fabs(so) * d1 > lt * fabs(b) && fabs(sp) < fabs(so) * rlt) r = sp * rcp(so);
+        F = base + h * muscl_phi(lim, r) * so;
     } else {
-        fr = c - T(0.5) * sp;
-        fl = b + T(0.5) * sp;
+        F = base + h * sp;
     }
-    return uu * (uu > T(0) ? fl : fr);
+    return uu * F;
 }
 
-// MUSCL z face f (1..nz-1) with the reference's stretched metrics
+// MUSCL z face f (1..nz-1) with the reference's stretched metrics, upwind
+// side only
 template <class T>
 __device__ __forceinline__ T muscl_zface(int f, int nz, T a, T b, T c, T d, T ww, const FusedArgs &A)
 {
     const T eps = T(1e-18);
-    int lim = A.muscl_limiter;
-    T lt = T(A.lim_tol), dz = T(A.dz);
+    const int lim = A.muscl_limiter;
+    const T lt = T(A.lim_tol), rlt = T(A.lim_tol_inv), dz = T(A.dz);
     const double *dz1 = A.dz1, *fdz0 = A.fdz0;
-    T m = T(A.dzw[f - 1]);  // 1/dz1w(f)
-    T sp = (c - b) * T(dz1[f - 1]);
+    const T m = T(A.dzw[f - 1]);  // 1/dz1w(f)
+    const T sp = (c - b) * T(dz1[f - 1]);
     T sm, sp2;
     bool limited;
     if (f == 1) {
@@ -136,25 +141,25 @@ __device__ __forceinline__ T muscl_zface(int f, int nz, T a, T b, T c, T d, T ww
         limited = T(fdz0[f - 2]) * dz * fabs(sm) > lt * fabs(b) + eps ||
                   T(fdz0[f]) * dz * fabs(sp2) > lt * fabs(b) + eps;
     }
-    T fl, fr;
+    const bool pos = ww > T(0);
+    const T so = pos ? sm : sp2;
+    const T base = pos ? b : c, h = pos ? T(0.5) : T(-0.5);
+    T F;
     if (limited) {
-        T rp1 = T(1), rp2 = T(1);
-        if (fabs(sm) > lt * fabs(b) && fabs(sp) < fabs(sm) * rcp(lt)) rp1 = sp * rcp(sm);
-        if (fabs(sp2) > lt * fabs(b) && fabs(sp) < fabs(sp2) * rcp(lt)) rp2 = sp * rcp(sp2);
-        fr = c - T(0.5) * muscl_phi(lim, rp2) * sp2 * m;
-        fl = b + T(0.5) * muscl_phi(lim, rp1) * sm * m;
+        T r = T(1);
+        if (fabs(so) > lt * fabs(b) && fabs(sp) < fabs(so) * rlt) r = sp * rcp(so);
+        F = base + h * muscl_phi(lim, r) * so * m;
     } else {
-        fr = c - T(0.5) * sp * m;
-        fl = b + T(0.5) * sp * m;
+        F = base + h * sp * m;
     }
-    return ww * (ww > T(0) ? fl : fr);
+    return ww * F;
 }
 
 template <int SCHEME, class T>
 __device__ __forceinline__ T hface(T a, T b, T c, T d, T uu, const FusedArgs &A, double d1)
 {
     if constexpr (SCHEME == 0) return weno_face(a, b, c, d, uu, true);
-    else return muscl_face(a, b, c, d, uu, A.muscl_limiter, T(A.lim_tol), T(d1));
+    else return muscl_face(a, b, c, d, uu, A.muscl_limiter, T(A.lim_tol), T(A.lim_tol_inv), T(d1));
 }
 
 template <int SCHEME, class T>

@@ -203,7 +203,10 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
             if (table[mid].work_begin <= wi) lo = mid;
             else hi = mid - 1;
         }
-        const FusedArgs &A = table[lo];
+        // a private copy of the argument block: its scalars stay in registers
+        // instead of being reloaded after every store (the table is global
+        // memory); the field pointers are read from the table once per field
+        const FusedArgs A = table[lo];
         Geom G;
         G.set(A.tile_x, A.tile_y);
         const int local = wi - A.work_begin;
@@ -219,8 +222,8 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
         sl.init(G, tid, i0, j0, A);
 #pragma unroll 1
         for (int f = 0; f < A.n32_active; ++f)
-            march_multi(A, G, A.q32[f], A.q0_32[f], A.out32[f], tile_lds, face_lds, k0, nk, i0, j0, tid, tx, ty, valid,
-                        col, sl);
+            march_multi(A, G, table[lo].q32[f], table[lo].q0_32[f], table[lo].out32[f], tile_lds, face_lds, k0, nk, i0,
+                        j0, tid, tx, ty, valid, col, sl);
     }
 }
 

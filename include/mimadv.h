@@ -241,10 +241,12 @@ const char *mimadv_multi_last_error(const mimadv_multi *m);
  * fewest face evaluations. */
 int mimadv_multi_set_tiles(mimadv_multi *m, const int *tile_x, const int *kchunk);
 int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int *kchunk);
-/* Autotuner: times every candidate (tile shape x level chunk in 16, 32) of
- * every subdomain with single-subdomain launches of one stage, reps times
- * each, keeps the fastest, and reuses a result for subdomains of equal size,
- * field count and scheme.  The fields are not modified. */
+/* Autotuner: times every candidate (tile shape x level chunk in 16, 32, 64)
+ * of every subdomain with launches of one stage of that subdomain alone,
+ * its work list repeated until the launch fills the GPU (so a candidate is
+ * judged by its cost per work item at saturation), reps times each; keeps
+ * the fastest and reuses a result for subdomains of equal size, field count
+ * and scheme.  The fields are not modified. */
 int mimadv_multi_autotune(mimadv_multi *m, int reps);
 /* nsteps Shu-Osher steps (MIMADV_INT_SSPRK3 or MIMADV_INT_HEUN) of every
  * subdomain with its own dt[d]; one launch per stage for the whole batch. */
