@@ -30,6 +30,7 @@ OVERWRITE = 8
 HOST_PTRS = 16
 GRAPH = 32
 F64, F32 = 0, 1
+OWN_STREAM = C.c_void_p(-1 & ((1 << 64) - 1)).value
 
 EXPORTED = (
     "mimadv_create",
@@ -51,6 +52,8 @@ EXPORTED = (
     "mimadv_batch_budget",
     "mimadv_adjoint_tendency",
     "mimadv_adjoint_advance",
+    "mimadv_advance_record",
+    "mimadv_adjoint_advance_recorded",
     "mimadv_set_reference_density",
     "mimadv_get_momentum",
     "mimadv_get_advective",
@@ -156,6 +159,10 @@ def load() -> C.CDLL:
     lib.mimadv_adjoint_tendency.restype = i
     lib.mimadv_adjoint_advance.argtypes = [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, i]
     lib.mimadv_adjoint_advance.restype = i
+    lib.mimadv_advance_record.argtypes = [vp, i, i, d, vp, vp, vp, vp, i, pp]
+    lib.mimadv_advance_record.restype = i
+    lib.mimadv_adjoint_advance_recorded.argtypes = [vp, i, i, d, vp, vp, vp, i, pp, vp, vp, vp, vp]
+    lib.mimadv_adjoint_advance_recorded.restype = i
     lib.mimadv_batch_budget.restype = i
     dp = C.POINTER(C.c_double)
     lib.mimadv_set_reference_density.argtypes = [vp, dp, dp]

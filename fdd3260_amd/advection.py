@@ -143,6 +143,8 @@ class Advection:
         self._check(self.lib.mimadv_synchronize(self._h), "mimadv_synchronize")
 
     def set_stream(self, stream_ptr):
+        """Run on a caller stream (a hipStream_t handle; 0 is the null stream, which is
+        torch's default stream); _lib.OWN_STREAM returns to the context's own stream."""
         self._check(self.lib.mimadv_set_stream(self._h, stream_ptr), "mimadv_set_stream")
 
     def last_kernel_ms(self) -> float:
@@ -299,6 +301,25 @@ class Advection:
         rc = self.lib.mimadv_adjoint_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1],
                                              ptrs[2], ptrs[3], ptrs[4], ptrs[5], ptrs[6], ptrs[7], int(checkpoint))
         self._check(rc, "mimadv_adjoint_advance")
+
+    def advance_record(self, integrator, nsteps, dt, u, v, w, q, checkpoint, ck):
+        """advance() of one flux-form field that copies the state at steps 0, K, 2K, ... into ck."""
+        ptrs, hflag = self._ptrs([u, v, w, q] + list(ck), writable=[q] + list(ck))
+        if hflag:
+            raise MimadvError("advance_record works on device tensors")
+        rc = self.lib.mimadv_advance_record(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1],
+                                            ptrs[2], ptrs[3], int(checkpoint), _lib.ptr_array(ptrs[4:]))
+        self._check(rc, "mimadv_advance_record")
+
+    def adjoint_advance_recorded(self, integrator, nsteps, dt, u, v, w, checkpoint, ck, qbar, ubar, vbar, wbar):
+        """Adjoint from checkpoints recorded by advance_record (no forward sweep of its own)."""
+        ptrs, hflag = self._ptrs([u, v, w, qbar, ubar, vbar, wbar] + list(ck), writable=(qbar, ubar, vbar, wbar))
+        if hflag:
+            raise MimadvError("the adjoint works on device tensors")
+        rc = self.lib.mimadv_adjoint_advance_recorded(self._h, int(integrator), int(nsteps), float(dt), ptrs[0],
+                                                      ptrs[1], ptrs[2], int(checkpoint), _lib.ptr_array(ptrs[7:]),
+                                                      ptrs[3], ptrs[4], ptrs[5], ptrs[6])
+        self._check(rc, "mimadv_adjoint_advance_recorded")
 
     # -- density weighting (funcpack.f90, anelastic) ---------------------------
     def set_reference_density(self, den0, avden0=None):

@@ -122,6 +122,69 @@ int check_ptr_dev(mimadv_ctx *c, const void *p, const char *name)
     return MIMADV_OK;
 }
 
+// nsteps steps of the advection-only loop for nf fields (mimadv_advance
+// without its checks and timing): per stage, tendency of every field,
+// stage combine and periodic ghost refresh in one launch per field.
+int advance_steps(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                  const double *w, int nf, double *const *q, const int *adv_form)
+{
+    int rc = ensure_work(c, (size_t)2 * nf);
+    if (rc) return rc;
+    size_t n3 = c->n3();
+    SchemeParams sp = c->sp;
+    sp.dt0 = dt;
+    // the scratch arrays start from the state so their ghosts are defined
+    for (int f = 0; f < nf; ++f) {
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f + 1) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        int nst = integrator == MIMADV_INT_HEUN ? 2 : 3;
+        for (int st = 1; st <= nst; ++st) {
+            for (int f = 0; f < nf; ++f) {
+                double *w1 = c->work + (2 * f) * n3, *w2 = c->work + (2 * f + 1) * n3;
+                ScalarArgs a{};
+                a.u = u;
+                a.v = v;
+                a.w = w;
+                a.adv = adv_form ? adv_form[f] : 0;
+                a.dirmask = 7;
+                a.fill = 1;
+                if (integrator == MIMADV_INT_HEUN) {
+                    // w1: stage state x*, w2: accumulated tendency (integ.h)
+                    a.epi = EPI_HEUN;
+                    a.tend = w2;
+                    a.q0 = q[f];
+                    if (st == 1) {
+                        a.q = q[f];
+                        a.qn = w1;
+                        a.b_lm = 0.0;
+                        a.adt = 1.0 * dt;
+                    } else {
+                        a.q = w1;
+                        a.qn = q[f];
+                        a.b_lm = 1.0;
+                        a.adt = 0.5 * dt;
+                    }
+                } else {
+                    a.epi = EPI_SO;
+                    a.dt = dt;
+                    a.q0 = q[f];
+                    if (st == 1) {
+                        a.q = q[f]; a.qn = w1; a.c0 = 0.0; a.c1 = 1.0;
+                    } else if (st == 2) {
+                        a.q = w1; a.qn = w2; a.c0 = 0.75; a.c1 = 0.25;
+                    } else {
+                        a.q = w2; a.qn = q[f]; a.c0 = 1.0 / 3.0; a.c1 = 2.0 / 3.0;
+                    }
+                }
+                HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
+            }
+        }
+    }
+    return MIMADV_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -217,7 +280,7 @@ const char *mimadv_last_error(const mimadv_ctx *c)
 int mimadv_set_stream(mimadv_ctx *c, void *stream)
 {
     if (!c) return MIMADV_ERR_ARG;
-    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    c->stream = stream == MIMADV_OWN_STREAM ? c->own_stream : (hipStream_t)stream;
     return MIMADV_OK;
 }
 
@@ -439,61 +502,9 @@ int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const d
     if (!u || !w || (c->g.ny > 1 && !v) || !q) return fail(c, MIMADV_ERR_ARG, "NULL array");
     for (int f = 0; f < nf; ++f)
         if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
-    int rc = ensure_work(c, (size_t)2 * nf);
-    if (rc) return rc;
-    size_t n3 = c->n3();
-    SchemeParams sp = c->sp;
-    sp.dt0 = dt;
-    // the scratch arrays start from the state so their ghosts are defined
-    for (int f = 0; f < nf; ++f) {
-        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f + 1) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int s = 0; s < nsteps; ++s) {
-        int nst = integrator == MIMADV_INT_HEUN ? 2 : 3;
-        for (int st = 1; st <= nst; ++st) {
-            for (int f = 0; f < nf; ++f) {
-                double *w1 = c->work + (2 * f) * n3, *w2 = c->work + (2 * f + 1) * n3;
-                ScalarArgs a{};
-                a.u = u;
-                a.v = v;
-                a.w = w;
-                a.adv = adv_form ? adv_form[f] : 0;
-                a.dirmask = 7;
-                a.fill = 1;
-                if (integrator == MIMADV_INT_HEUN) {
-                    // w1: stage state x*, w2: accumulated tendency (integ.h)
-                    a.epi = EPI_HEUN;
-                    a.tend = w2;
-                    a.q0 = q[f];
-                    if (st == 1) {
-                        a.q = q[f];
-                        a.qn = w1;
-                        a.b_lm = 0.0;
-                        a.adt = 1.0 * dt;
-                    } else {
-                        a.q = w1;
-                        a.qn = q[f];
-                        a.b_lm = 1.0;
-                        a.adt = 0.5 * dt;
-                    }
-                } else {
-                    a.epi = EPI_SO;
-                    a.dt = dt;
-                    a.q0 = q[f];
-                    if (st == 1) {
-                        a.q = q[f]; a.qn = w1; a.c0 = 0.0; a.c1 = 1.0;
-                    } else if (st == 2) {
-                        a.q = w1; a.qn = w2; a.c0 = 0.75; a.c1 = 0.25;
-                    } else {
-                        a.q = w2; a.qn = q[f]; a.c0 = 1.0 / 3.0; a.c1 = 2.0 / 3.0;
-                    }
-                }
-                HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
-            }
-        }
-    }
+    int rc = advance_steps(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form);
+    if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.0f;
@@ -855,26 +866,25 @@ int mimadv_adjoint_tendency(mimadv_ctx *c, const double *u, const double *v, con
     return MIMADV_OK;
 }
 
-int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
-                           const double *w, const double *q0, double *qbar, double *ubar, double *vbar,
-                           double *wbar, int checkpoint)
+}  // extern "C"
+
+namespace {
+
+// Backward sweep of nsteps steps with checkpoints every K steps: the state
+// at step g*K comes from ext[g] when the caller recorded the forward, else
+// from a first forward sweep from q0 into internal storage.
+int adjoint_core(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                 const double *w, const double *q0, const double *const *ext, int K, double *qbar, double *ubar,
+                 double *vbar, double *wbar)
 {
-    if (!c) return MIMADV_ERR_ARG;
-    int rc = check_adjoint(c, u, v, w);
-    if (rc) return rc;
-    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
-        return fail(c, MIMADV_ERR_UNSUPPORTED, "adjoint integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
-    if (nsteps < 0) return fail(c, MIMADV_ERR_ARG, "negative step count");
-    if (!q0 || !qbar || !ubar || !wbar || (c->g.ny > 1 && !vbar)) return fail(c, MIMADV_ERR_ARG, "NULL array");
     const size_t n3 = c->n3();
     const int S = integrator == MIMADV_INT_HEUN ? 2 : 3;
-    const int K = (checkpoint <= 0 || checkpoint >= nsteps) ? (nsteps > 0 ? nsteps : 1) : checkpoint;
     const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
-    // layout: P, T, ACC (adjoint arrays), nseg checkpoints, S*K stage states
-    const size_t need = n3 * ((size_t)3 + nseg + (size_t)S * K);
+    const int own_ck = ext ? 0 : nseg;
+    // layout: P, T, ACC (adjoint arrays), own checkpoints, S*K stage states
+    const size_t need = n3 * ((size_t)3 + own_ck + (size_t)S * K);
     if (need > c->adj_elems) {
-        if (c->adj) hipFree(c->adj);
-    if (c->d_den0) hipFree(c->d_den0);
+        if (c->adj) (void)hipFree(c->adj);
         c->adj = nullptr;
         c->adj_elems = 0;
         if (hipMalloc(&c->adj, need * sizeof(double)) != hipSuccess)
@@ -882,7 +892,8 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
         c->adj_elems = need;
     }
     double *P = c->adj, *T = c->adj + n3, *ACC = c->adj + 2 * n3;
-    double *ck = c->adj + 3 * n3, *st = ck + (size_t)nseg * n3;
+    double *ck = c->adj + 3 * n3, *st = ck + (size_t)own_ck * n3;
+    auto ckpt = [&](int g) -> const double * { return ext ? ext[g] : ck + (size_t)g * n3; };
     auto X = [&](int n, int s) { return st + ((size_t)n * S + s) * n3; };  // stage s input of segment step n
     SchemeParams sp = c->sp;
     sp.dt0 = dt;
@@ -919,8 +930,7 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
     // replay segment g from its checkpoint; the state after its last step goes to qlast
     auto replay = [&](int g, double *qlast) -> hipError_t {
         int len = std::min(K, nsteps - g * K);
-        hipError_t e = hipMemcpyAsync(X(0, 0), ck + (size_t)g * n3, n3 * sizeof(double), hipMemcpyDeviceToDevice,
-                                      c->stream);
+        hipError_t e = hipMemcpyAsync(X(0, 0), ckpt(g), n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
         for (int n = 0; n < len && e == hipSuccess; ++n) e = fwd_step(n, n + 1 < len ? X(n + 1, 0) : qlast);
         return e;
     };
@@ -931,7 +941,7 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, hipMemsetAsync(c->adj, 0, 3 * n3 * sizeof(double), c->stream));
     HIPCHK(c, hipMemcpyAsync(P, qbar, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if (nseg > 0) {
+    if (!ext && nseg > 0) {
         HIPCHK(c, hipMemcpyAsync(ck, q0, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         for (int g = 0; g + 1 < nseg; ++g) HIPCHK(c, replay(g, ck + (size_t)(g + 1) * n3));
     }
@@ -969,6 +979,80 @@ int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt,
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
     return MIMADV_OK;
+}
+
+int check_adjoint_call(mimadv_ctx *c, int integrator, int nsteps, const double *u, const double *v, const double *w,
+                       double *qbar, double *ubar, double *vbar, double *wbar)
+{
+    int rc = check_adjoint(c, u, v, w);
+    if (rc) return rc;
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "adjoint integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
+    if (nsteps < 0) return fail(c, MIMADV_ERR_ARG, "negative step count");
+    if (!qbar || !ubar || !wbar || (c->g.ny > 1 && !vbar)) return fail(c, MIMADV_ERR_ARG, "NULL array");
+    return MIMADV_OK;
+}
+
+int checkpoint_interval(int checkpoint, int nsteps)
+{
+    return (checkpoint <= 0 || checkpoint >= nsteps) ? (nsteps > 0 ? nsteps : 1) : checkpoint;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mimadv_adjoint_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                           const double *w, const double *q0, double *qbar, double *ubar, double *vbar,
+                           double *wbar, int checkpoint)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_adjoint_call(c, integrator, nsteps, u, v, w, qbar, ubar, vbar, wbar);
+    if (rc) return rc;
+    if (!q0) return fail(c, MIMADV_ERR_ARG, "q0 is NULL");
+    return adjoint_core(c, integrator, nsteps, dt, u, v, w, q0, nullptr, checkpoint_interval(checkpoint, nsteps), qbar,
+                        ubar, vbar, wbar);
+}
+
+int mimadv_advance_record(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                          const double *w, double *q, int checkpoint, double *const *ck)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_adjoint(c, u, v, w);
+    if (rc) return rc;
+    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
+    if (nsteps < 0 || !q) return fail(c, MIMADV_ERR_ARG, "bad step count or NULL q");
+    const int K = checkpoint_interval(checkpoint, nsteps);
+    const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
+    if (nseg > 0 && !ck) return fail(c, MIMADV_ERR_ARG, "NULL checkpoint list");
+    for (int g = 0; g < nseg; ++g)
+        if (!ck[g]) return fail(c, MIMADV_ERR_ARG, "NULL checkpoint array");
+    const size_t bytes = c->n3() * sizeof(double);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int g = 0; g < nseg; ++g) {
+        HIPCHK(c, hipMemcpyAsync(ck[g], q, bytes, hipMemcpyDeviceToDevice, c->stream));
+        rc = advance_steps(c, integrator, std::min(K, nsteps - g * K), dt, u, v, w, 1, &q, nullptr);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    return MIMADV_OK;
+}
+
+int mimadv_adjoint_advance_recorded(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u,
+                                    const double *v, const double *w, int checkpoint, const double *const *ck,
+                                    double *qbar, double *ubar, double *vbar, double *wbar)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    int rc = check_adjoint_call(c, integrator, nsteps, u, v, w, qbar, ubar, vbar, wbar);
+    if (rc) return rc;
+    const int K = checkpoint_interval(checkpoint, nsteps);
+    const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
+    if (nseg > 0 && !ck) return fail(c, MIMADV_ERR_ARG, "NULL checkpoint list");
+    for (int g = 0; g < nseg; ++g)
+        if (!ck[g]) return fail(c, MIMADV_ERR_ARG, "NULL checkpoint array");
+    return adjoint_core(c, integrator, nsteps, dt, u, v, w, nullptr, ck, K, qbar, ubar, vbar, wbar);
 }
 
 }  // extern "C"

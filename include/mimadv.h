@@ -105,7 +105,11 @@ typedef struct mimadv_ctx mimadv_ctx;
 int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx **out);
 void mimadv_destroy(mimadv_ctx *ctx);
 const char *mimadv_last_error(const mimadv_ctx *ctx);
-/* Run subsequent work on a caller stream (hipStream_t); NULL = own stream. */
+/* Run subsequent work on a caller stream (hipStream_t).  NULL is HIP's null
+ * (default) stream, the one a framework's default stream maps to;
+ * MIMADV_OWN_STREAM returns to the context's own non-blocking stream (the
+ * initial setting), which is not ordered with any other stream. */
+#define MIMADV_OWN_STREAM ((void *)-1)
 int mimadv_set_stream(mimadv_ctx *ctx, void *stream);
 int mimadv_synchronize(mimadv_ctx *ctx);
 /* Change the scheme-related options of an existing context. */
@@ -284,6 +288,19 @@ int mimadv_adjoint_tendency(mimadv_ctx *ctx, const double *u, const double *v, c
 int mimadv_adjoint_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
                            const double *v, const double *w, const double *q0, double *qbar, double *ubar,
                            double *vbar, double *wbar, int checkpoint);
+
+/* The same adjoint split in two, the way a caller that runs the forward
+ * anyway would use it: mimadv_advance_record advances q in place exactly as
+ * mimadv_advance does for one flux-form field and copies the state at steps
+ * 0, K, 2K, ... into ck[0..ceil(nsteps/K)-1] (K = checkpoint, <= 0: nsteps);
+ * mimadv_adjoint_advance_recorded then needs no forward sweep of its own:
+ * per segment it replays K steps from ck[g] into stage storage
+ * (3 + stages*K device arrays) and sweeps them backwards. */
+int mimadv_advance_record(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
+                          const double *v, const double *w, double *q, int checkpoint, double *const *ck);
+int mimadv_adjoint_advance_recorded(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
+                                    const double *v, const double *w, int checkpoint, const double *const *ck,
+                                    double *qbar, double *ubar, double *vbar, double *wbar);
 
 /* Kernel time of the last mimadv_advance / mimadv_scalar_tendency call,
  * measured with HIP events on the context stream (ms). */
