@@ -391,6 +391,19 @@ def multi(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_probe(mib):
+    """A device-to-device copy of `mib` MiB: reads and writes exactly that
+    many bytes, the reference for the HBM byte counters."""
+    import torch
+
+    x = torch.empty(mib * 2 ** 20 // 4, dtype=torch.float32, device="cuda:0")
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    y = x.clone()
+    torch.cuda.synchronize()
+    del x, y
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -410,8 +423,15 @@ def main():
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of the column tile (2, 4, 8)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="after the run, copy this many MiB on the device (a known-size kernel that scales "
+                         "hardware byte counters)")
     args = ap.parse_args()
 
+    if args.copy_probe > 0:
+        import atexit
+
+        atexit.register(copy_probe, args.copy_probe)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the advection operator is built and measured on a single "
               "MI355X; runs over several GPUs are not part of this build.", file=sys.stderr)

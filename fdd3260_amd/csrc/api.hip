@@ -1365,8 +1365,7 @@ int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int 
     for (int d = 0; d < m->ndom; ++d) {
         if (tile_x) tile_x[d] = m->tile_x[d];
         if (tile_y) tile_y[d] = MULTI_NT / m->tile_x[d];
-        if (kchunk) kchunk[d] = m->kchunk[d] > 0 ? std::min(m->kchunk[d], m->ctx[d]->g.nz)
-                                                 : std::min(32, m->ctx[d]->g.nz);
+        if (kchunk) kchunk[d] = m->kchunk[d] > 0 ? std::min(m->kchunk[d], 8) : 8;
     }
     return MIMADV_OK;
 }
@@ -1376,7 +1375,7 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
     if (!m) return MIMADV_ERR_ARG;
     if (reps < 1) reps = 1;
     static const int cand_x[4] = {64, 32, 128, 16};
-    static const int cand_k[3] = {16, 32, 64};
+    static const int cand_k[2] = {4, 8};
     // A candidate is timed at saturation: the subdomain's work list is
     // repeated until the launch holds at least `target` work items (the
     // copies write the same values to the same scratch stage array), so a
@@ -1415,7 +1414,6 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
         int bx = m->tile_x[d], bk = m->kchunk[d];
         for (int tx : cand_x) {
             for (int kc : cand_k) {
-                if (kc > 2 * c->g.nz) continue;
                 m->tile_x[d] = tx;
                 m->kchunk[d] = kc;
                 // a middle stage into s2 from q: the fields are only read

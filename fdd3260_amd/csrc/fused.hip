@@ -47,19 +47,48 @@ struct Item {
     static constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
     static constexpr int FS = (NF + NT - 1) / NT;        // face slots per thread
     static constexpr int TS = (LH * LW + NT - 1) / NT;   // tile slots per thread
+    static constexpr int NV = NF + NT;                   // velocities per level: faces, then w of every column
 };
 
+// Levels of a work item whose velocities are staged in LDS once and read by
+// every field (the chunk length is capped at this).
+constexpr int FUSED_KC = 8;
+
+// Face e of the flattened list [x faces row-major | y faces] of a tile:
+// kind (1 x, 2 y), LDS offset/step of its stencil and the column of its
+// velocity (u(f+1) of x face f + 1/2, v(g+1) of y face g + 1/2).
+template <int TX, int TY>
+__device__ __forceinline__ int face_slot_geom(int e, int i0, int j0, const FusedArgs &A, int &base, int &step,
+                                              size_t &vcol)
+{
+    using I = Item<TX, TY>;
+    const int nx = A.nx, ny = A.ny;
+    if (e < I::NXF) {
+        const int r = e / (TX + 1), fl = e - r * (TX + 1);
+        base = (I::THREE_D ? r + 2 : 0) * I::LW + fl;
+        step = 1;
+        const int jr = I::THREE_D ? wrap(j0 + r, ny, A.my) : 1;
+        vcol = (size_t)(wrap(i0 + fl, nx, A.mx) - 1) + (size_t)nx * (jr - 1);
+        return 1;
+    }
+    const int e2 = e - I::NXF, fr = e2 / TX, c = e2 - fr * TX;
+    base = fr * I::LW + c + 2;
+    step = I::LW;
+    vcol = (size_t)(wrap(i0 + c, nx, A.mx) - 1) + (size_t)nx * (wrap(j0 + fr, ny, A.my) - 1);
+    return 2;
+}
+
 // Per-thread slot geometry of a work item, computed once and shared by
-// every field and level: the thread's tile slots (LDS index, column offset
-// in a level plane) and face slots of the flattened list
+// every field and level: the thread's tile slots (column offset in a level
+// plane, -1 if none) and face slots of the flattened list
 // [x faces row-major | y faces] (LDS offset of the stencil q(f-1..f+2),
-// step 1 for x and LW for y, 0 for an empty slot; plane offset of the face
-// velocity u(f+1) or v(g+1)).  Plane offsets fit in 32 bits.
+// step 1 for x and LW for y, 0 for an empty slot).  Plane offsets fit in
+// 32 bits.
 template <int TX, int TY>
 struct Slots {
     using I = Item<TX, TY>;
     int t_off[I::TS];
-    int f_base[I::FS], f_step[I::FS], f_off[I::FS];
+    int f_base[I::FS], f_step[I::FS];
     __device__ __forceinline__ void init(int tid, int i0, int j0, const FusedArgs &A)
     {
         const int nx = A.nx, ny = A.ny;
@@ -79,18 +108,14 @@ struct Slots {
             const int e = tid + s * I::NT;
             f_base[s] = 0;
             f_step[s] = 0;
-            f_off[s] = 0;
             if (e < I::NXF) {
                 const int r = e / (TX + 1), fl = e - r * (TX + 1);  // x face f = i0-1+fl, row j0+r
                 f_base[s] = (I::THREE_D ? r + 2 : 0) * I::LW + fl;
                 f_step[s] = 1;
-                const int jr = I::THREE_D ? wrap(j0 + r, ny, A.my) : 1;
-                f_off[s] = (wrap(i0 + fl, nx, A.mx) - 1) + nx * (jr - 1);
             } else if (e < I::NF) {
                 const int e2 = e - I::NXF, fr = e2 / TX, c = e2 - fr * TX;  // y face g = j0-1+fr, column i0+c
                 f_base[s] = fr * I::LW + c + 2;
                 f_step[s] = I::LW;
-                f_off[s] = (wrap(i0 + c, nx, A.mx) - 1) + nx * (wrap(j0 + fr, ny, A.my) - 1);
             }
         }
     }
@@ -105,8 +130,8 @@ struct Slots {
 // the periodic ghost images.
 template <int SCHEME, int TX, int TY, class T>
 __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restrict__ q, const T *__restrict__ q0,
-                                            T *__restrict__ out, T *lt, T *lf, int k0, int nk, int i0, int j0,
-                                            int tid, int tx, int ty, bool valid, int col,
+                                            T *__restrict__ out, T *lt, T *lf, const float *lv, int k0, int nk,
+                                            int i0, int j0, int tid, int tx, int ty, bool valid, int col,
                                             const Slots<TX, TY> &sl)
 {
     using I = Item<TX, TY>;
@@ -125,7 +150,6 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
     const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
     T fzb = T(0);
     T tv[I::TS];
-    float vv[I::FS], wv;
     T q0v = T(0), zn = T(0);
     double d1v = 0.0;
     const bool need_q0 = A.epi != 0 && A.c0 != 0.0;
@@ -136,10 +160,6 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 #pragma unroll
         for (int s = 0; s < I::TS; ++s)
             if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
-#pragma unroll
-        for (int s = 0; s < I::FS; ++s)
-            vv[s] = sl.f_step[s] == 1 ? A.u[kb + sl.f_off[s]] : (sl.f_step[s] != 0 ? A.v[kb + sl.f_off[s]] : 0.0f);
-        wv = k < nz ? A.w[col + kb + plane] : 0.0f;
         if (need_q0 && valid) q0v = q0[col + kb];
         zn = q[col + lev(k + 2)];
         d1v = A.dz1w[k - 1];
@@ -155,7 +175,8 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
             zw[2] = zw[3];
             zw[3] = zn;
         }
-        const T fzt = zdir ? zface<SCHEME, T>(k, nz, zw, T(kk < 0 ? wz0 : wv), A) : T(0);
+        const float *lvk = lv + (kk < 0 ? 0 : kk) * I::NV;  // this level's staged velocities
+        const T fzt = zdir ? zface<SCHEME, T>(k, nz, zw, T(kk < 0 ? wz0 : lvk[I::NF + tid]), A) : T(0);
         if (kk < 0) {
             fzb = fzt;
             continue;
@@ -163,9 +184,6 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 #pragma unroll
         for (int s = 0; s < I::TS; ++s)
             if (sl.t_off[s] >= 0) lt[tid + s * I::NT] = tv[s];
-        float vc[I::FS];
-#pragma unroll
-        for (int s = 0; s < I::FS; ++s) vc[s] = vv[s];
         const T q0c = q0v;
         const double dz1w_k = d1v;
         __syncthreads();
@@ -175,8 +193,8 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
             const int st = sl.f_step[s];
             if (st == 0) continue;
             const int b = sl.f_base[s];
-            lf[tid + s * I::NT] = hface<SCHEME, T>(lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], T(vc[s]), A,
-                                                   st == 1 ? A.dx1 : A.dy1);
+            lf[tid + s * I::NT] = hface<SCHEME, T>(lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st],
+                                                   T(lvk[tid + s * I::NT]), A, st == 1 ? A.dx1 : A.dy1);
         }
         __syncthreads();
         if (valid) {
@@ -231,6 +249,7 @@ __global__ __launch_bounds__(TX *TY) void fused_kernel(FusedArgs A)
     using I = Item<TX, TY>;
     __shared__ T tile_lds[I::LH * I::LW];
     __shared__ T face_lds[I::FS * I::NT];
+    __shared__ float vel_lds[FUSED_KC * I::NV];
 
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -257,16 +276,40 @@ __global__ __launch_bounds__(TX *TY) void fused_kernel(FusedArgs A)
         const int col = (il - 1) + nx * (jl - 1);
         Slots<TX, TY> sl;
         sl.init(tid, i0, j0, A);
+        // the chunk's face velocities and column w(k+1), read once for every field
+        {
+            const size_t plane = (size_t)nx * ny;
+            __syncthreads();  // the previous item's fields are done with vel_lds
+            for (int kk = 0; kk < nk; ++kk) {
+                const int k = k0 + kk;
+                const size_t kb = (size_t)(k - 1) * plane;
+                for (int e = tid; e < I::NV; e += I::NT) {
+                    float vel = 0.0f;
+                    if (e < I::NF) {
+                        int base, step;
+                        size_t vcol;
+                        const int kind = face_slot_geom<TX, TY>(e, i0, j0, A, base, step, vcol);
+                        vel = kind == 1 ? A.u[kb + vcol] : A.v[kb + vcol];
+                    } else if (k < nz) {
+                        const int c = e - I::NF, cx = c % TX, cy = c / TX;
+                        const int ci = wrap(i0 + cx, nx, A.mx), cj = I::THREE_D ? wrap(j0 + cy, ny, A.my) : 1;
+                        vel = A.w[kb + plane + (size_t)(ci - 1) + (size_t)nx * (cj - 1)];
+                    }
+                    vel_lds[kk * I::NV + e] = vel;
+                }
+            }
+            __syncthreads();
+        }
         if constexpr (sizeof(T) == 8) {
 #pragma unroll 1
             for (int f = 0; f < A.n64_active; ++f)
-                march_field<SCHEME, TX, TY, T>(A, A.q64[f], A.q0_64[f], A.out64[f], tile_lds, face_lds, k0, nk, i0, j0,
-                                               tid, tx, ty, valid, col, sl);
+                march_field<SCHEME, TX, TY, T>(A, A.q64[f], A.q0_64[f], A.out64[f], tile_lds, face_lds, vel_lds, k0, nk,
+                                               i0, j0, tid, tx, ty, valid, col, sl);
         } else {
 #pragma unroll 1
             for (int f = 0; f < A.n32_active; ++f)
-                march_field<SCHEME, TX, TY, T>(A, A.q32[f], A.q0_32[f], A.out32[f], tile_lds, face_lds, k0, nk, i0, j0,
-                                               tid, tx, ty, valid, col, sl);
+                march_field<SCHEME, TX, TY, T>(A, A.q32[f], A.q0_32[f], A.out32[f], tile_lds, face_lds, vel_lds, k0, nk,
+                                               i0, j0, tid, tx, ty, valid, col, sl);
         }
     }
 }
@@ -279,7 +322,7 @@ hipError_t launch_tiles(const FusedArgs &a0, hipStream_t s, int blocks_per_cu)
     A.nty = TY > 1 ? (A.ny - 5 + TY - 1) / TY : 1;
     const int tiles = A.ntx * A.nty;
     A.xcd_remap = (tiles % 8 == 0) ? 1 : 0;
-    if (A.kchunk <= 0 || A.kchunk > A.nz) A.kchunk = A.kchunk <= 0 ? min(32, A.nz) : A.nz;
+    A.kchunk = A.kchunk <= 0 ? FUSED_KC : min(A.kchunk, FUSED_KC);
     A.nwork = tiles * ((A.nz + A.kchunk - 1) / A.kchunk);
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||

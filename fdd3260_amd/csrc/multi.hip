@@ -28,9 +28,11 @@ constexpr int NT = MULTI_NT;
 constexpr int FS = 3;                  // 2*NT + TX + TY faces per tile
 constexpr int TS = 3;                  // (TX+4)*(TY+4) tile values
 constexpr int TILE_MAX = 132 * 8;      // largest (TX+4)*(TY+4) of the shapes
+constexpr int KC_MAX = 8;              // levels of a work item (velocities staged in LDS)
+constexpr int NV_MAX = 2 * NT + 132 + NT;  // face velocities and column w of one level
 
 struct Geom {
-    int TX, TY, LW, LH, NXF, NF;
+    int TX, TY, LW, LH, NXF, NF, NV;
     __device__ __forceinline__ void set(int tx_, int ty_)
     {
         TX = tx_;
@@ -39,12 +41,13 @@ struct Geom {
         LH = TY + 4;
         NXF = TY * (TX + 1);
         NF = NXF + (TY + 1) * TX;
+        NV = NF + NT;
     }
 };
 
 struct SlotsR {
     int t_off[TS];
-    int f_base[FS], f_step[FS], f_off[FS];
+    int f_base[FS], f_step[FS];
     __device__ __forceinline__ void init(const Geom &G, int tid, int i0, int j0, const FusedArgs &A)
     {
         const int nx = A.nx, ny = A.ny;
@@ -62,17 +65,14 @@ struct SlotsR {
             const int e = tid + s * NT;
             f_base[s] = 0;
             f_step[s] = 0;
-            f_off[s] = 0;
             if (e < G.NXF) {
                 const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);  // x face f = i0-1+fl, row j0+r
                 f_base[s] = (r + 2) * G.LW + fl;
                 f_step[s] = 1;
-                f_off[s] = (wrap(i0 + fl, nx, A.mx) - 1) + nx * (wrap(j0 + r, ny, A.my) - 1);
             } else if (e < G.NF) {
                 const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;  // y face g = j0-1+fr
                 f_base[s] = fr * G.LW + c + 2;
                 f_step[s] = G.LW;
-                f_off[s] = (wrap(i0 + c, nx, A.mx) - 1) + nx * (wrap(j0 + fr, ny, A.my) - 1);
             }
         }
     }
@@ -90,8 +90,8 @@ __device__ __forceinline__ float zface_any(const FusedArgs &A, int f, int nz, co
 
 __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, const float *__restrict__ q,
                                             const float *__restrict__ q0, float *__restrict__ out, float *lt,
-                                            float *lf, int k0, int nk, int i0, int j0, int tid, int tx, int ty,
-                                            bool valid, int col, const SlotsR &sl)
+                                            float *lf, const float *lv, int k0, int nk, int i0, int j0, int tid,
+                                            int tx, int ty, bool valid, int col, const SlotsR &sl)
 {
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const size_t plane = (size_t)nx * ny;
@@ -104,7 +104,7 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
     if (ks == 1) zw[0] = zw[1];
     const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
     float fzb = 0.0f;
-    float tv[TS], vv[FS], wv, q0v = 0.0f, zn = 0.0f;
+    float tv[TS], q0v = 0.0f, zn = 0.0f;
     double d1v = 0.0;
     const bool need_q0 = A.epi != 0 && A.c0 != 0.0;
     auto fetch = [&](int k) {
@@ -113,10 +113,6 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
 #pragma unroll
         for (int s = 0; s < TS; ++s)
             if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
-#pragma unroll
-        for (int s = 0; s < FS; ++s)
-            vv[s] = sl.f_step[s] == 1 ? A.u[kb + sl.f_off[s]] : (sl.f_step[s] != 0 ? A.v[kb + sl.f_off[s]] : 0.0f);
-        wv = k < nz ? A.w[col + kb + plane] : 0.0f;
         if (need_q0 && valid) q0v = q0[col + kb];
         zn = q[col + lev(k + 2)];
         d1v = A.dz1w[k - 1];
@@ -132,7 +128,8 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
             zw[2] = zw[3];
             zw[3] = zn;
         }
-        const float fzt = zface_any(A, k, nz, zw, kk < 0 ? wz0 : wv);
+        const float *lvk = lv + (kk < 0 ? 0 : kk) * G.NV;  // this level's staged velocities
+        const float fzt = zface_any(A, k, nz, zw, kk < 0 ? wz0 : lvk[G.NF + tid]);
         if (kk < 0) {
             fzb = fzt;
             continue;
@@ -140,9 +137,6 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
 #pragma unroll
         for (int s = 0; s < TS; ++s)
             if (sl.t_off[s] >= 0) lt[tid + s * NT] = tv[s];
-        float vc[FS];
-#pragma unroll
-        for (int s = 0; s < FS; ++s) vc[s] = vv[s];
         const float q0c = q0v;
         const double dz1w_k = d1v;
         __syncthreads();
@@ -152,7 +146,7 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
             const int st = sl.f_step[s];
             if (st == 0) continue;
             const int b = sl.f_base[s];
-            lf[tid + s * NT] = face_any(A, lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], vc[s],
+            lf[tid + s * NT] = face_any(A, lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], lvk[tid + s * NT],
                                         st == 1 ? A.dx1 : A.dy1);
         }
         __syncthreads();
@@ -194,6 +188,7 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
 {
     __shared__ float tile_lds[TILE_MAX];
     __shared__ float face_lds[FS * NT];
+    __shared__ float vel_lds[KC_MAX * NV_MAX];
     const int tid = threadIdx.x;
     for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
         // subdomain of this work id: last d with work_begin <= wi
@@ -220,10 +215,38 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
         const int col = (wrap(i, A.nx, A.mx) - 1) + A.nx * (wrap(j, A.ny, A.my) - 1);
         SlotsR sl;
         sl.init(G, tid, i0, j0, A);
+        // the chunk's face velocities and column w(k+1), read once for every field
+        {
+            const int nx = A.nx, ny = A.ny;
+            const size_t plane = (size_t)nx * ny;
+            __syncthreads();  // the previous item's fields are done with vel_lds
+            for (int kk = 0; kk < nk; ++kk) {
+                const int k = k0 + kk;
+                const size_t kb = (size_t)(k - 1) * plane;
+                for (int e = tid; e < G.NV; e += NT) {
+                    float vel = 0.0f;
+                    if (e < G.NXF) {
+                        const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);
+                        vel = A.u[kb + (size_t)(wrap(i0 + fl, nx, A.mx) - 1) +
+                                  (size_t)nx * (wrap(j0 + r, ny, A.my) - 1)];
+                    } else if (e < G.NF) {
+                        const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;
+                        vel = A.v[kb + (size_t)(wrap(i0 + c, nx, A.mx) - 1) +
+                                  (size_t)nx * (wrap(j0 + fr, ny, A.my) - 1)];
+                    } else if (k < A.nz) {
+                        const int c = e - G.NF, cx = c % G.TX, cy = c / G.TX;
+                        vel = A.w[kb + plane + (size_t)(wrap(i0 + cx, nx, A.mx) - 1) +
+                                  (size_t)nx * (wrap(j0 + cy, ny, A.my) - 1)];
+                    }
+                    vel_lds[kk * G.NV + e] = vel;
+                }
+            }
+            __syncthreads();
+        }
 #pragma unroll 1
         for (int f = 0; f < A.n32_active; ++f)
-            march_multi(A, G, table[lo].q32[f], table[lo].q0_32[f], table[lo].out32[f], tile_lds, face_lds, k0, nk, i0,
-                        j0, tid, tx, ty, valid, col, sl);
+            march_multi(A, G, table[lo].q32[f], table[lo].q0_32[f], table[lo].out32[f], tile_lds, face_lds, vel_lds, k0,
+                        nk, i0, j0, tid, tx, ty, valid, col, sl);
     }
 }
 
@@ -238,7 +261,7 @@ void multi_prepare(FusedArgs &A, int work_begin)
 {
     A.ntx = (A.nx - 5 + A.tile_x - 1) / A.tile_x;
     A.nty = (A.ny - 5 + A.tile_y - 1) / A.tile_y;
-    if (A.kchunk <= 0 || A.kchunk > A.nz) A.kchunk = A.kchunk <= 0 ? min(32, A.nz) : A.nz;
+    A.kchunk = A.kchunk <= 0 ? KC_MAX : min(A.kchunk, KC_MAX);
     A.nwork = A.ntx * A.nty * ((A.nz + A.kchunk - 1) / A.kchunk);
     A.work_begin = work_begin;
 }
