@@ -30,6 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch (per stage for the batched workloads) of the dominant
+# kernels, measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of
+# this script, each counter scaled by what it reads for a 256 MiB device copy
+# (--copy-probe 256: FETCH_SIZE counts half the bytes read, WRITE_SIZE all
+# bytes written).  Reported only for the exact configuration measured.
+HBM_TRAFFIC = {
+    ("plain", "256,256,128", "weno", "heun", 2): 554.3e6,
+    ("fused", "512,512,160", "weno", "ssprk3", 17): 13.79e9,
+}
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
 
 
@@ -144,7 +153,8 @@ def fused(args):
                                f"{adv.scal_adv}, {args.integrator}, hipGraph={graph}",
                    "cells": ncell, "fields": nf, "stages_per_step": nst, "dt_s": dt},
         "roofline": {"bound": "hbm", "kernel": "fused_kernel (all groups of one stage)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": HBM_TRAFFIC.get(("fused", size, adv.scal_adv, args.integrator, nf)),
                      "alg_bytes_per_stage": sum(per_stage) / nst, "avg_stage_us": stage_s * 1e6},
         "cpu_baseline": None,
     }
@@ -527,7 +537,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": HBM_TRAFFIC.get(("plain", args.size, args.scheme, args.integrator, nfield)),
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_us": avg_launch_s * 1e6,
         },
