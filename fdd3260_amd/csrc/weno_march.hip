@@ -29,11 +29,25 @@ namespace mimadv {
 
 namespace {
 
+// fl(x/d) > 20 without the division, d > 0: the quotient rounds above 20
+// iff x/d > 20 + 2^-49 (that tie rounds to the even 20), i.e. iff
+// x - 20d > h = 2^-49 d.  y = fl(x - 20d) from one fma has relative error at
+// most 2^-53 and h is exact, so outside a 2^-40 band around h the answer is
+// certain; inside it (never met in practice) the division decides.
+__device__ __forceinline__ bool ratio_above_20(double x, double d)
+{
+    const double y = fma(-20.0, d, x);
+    const double h = ldexp(d, -49);
+    if (y > h * (1.0 + 0x1p-40)) return true;
+    if (y < h * (1.0 - 0x1p-40)) return false;
+    return x / d > 20.0;
+}
+
 // Reference face flux for the upwind side only.  nonlin: weights may
 // switch (x/y always, z for faces 3..nz-3).
 __device__ __forceinline__ double weno_face_up(double a, double b, double c, double d, double uu, bool nonlin)
 {
-    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15, thres = 20.0;
+    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15;
     const bool left = uu > 0.0;
     double wo = g1, wi = g2;  // outer, inner
     if (nonlin) {
@@ -41,7 +55,7 @@ __device__ __forceinline__ double weno_face_up(double a, double b, double c, dou
         const double eta_o = left ? (b - a) * (b - a) : (d - c) * (d - c);
         // left pair (eta1, eta2) = (outer, inner), right pair (eta2, eta3) =
         // (inner, outer); max and min do not depend on the order
-        if (dmaxd(eta_o, eta_i) / (dmind(eta_o, eta_i) + eps) > thres) {
+        if (ratio_above_20(dmaxd(eta_o, eta_i), dmind(eta_o, eta_i) + eps)) {
             const double so = (eps + eta_o) * (eps + eta_o);
             const double si = (eps + eta_i) * (eps + eta_i);
             const double w_o = g1 / so, w_i = g2 / si;
@@ -100,10 +114,10 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
     constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;
-    constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0;
-    __shared__ double tile[LH][LW];
-    __shared__ double fx[TY][TX + 1];
-    __shared__ double fy[THREE_D ? TY + 1 : 1][TX];
+    constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
+    constexpr int FS = (NF + NT - 1) / NT, TS = (LH * LW + NT - 1) / NT;
+    __shared__ double tile[LH * LW];
+    __shared__ double face[FS * NT];
 
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t plane = (size_t)nx * ny;
@@ -114,6 +128,9 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
     const bool adv = a.adv != 0;
     const int bc5 = g.bcl[4], bc6 = g.bcl[5];
     const double *__restrict__ dz1w = g.zm.dz1w;
+    const bool zdir = (a.dirmask & 4) != 0;
+    const bool accum = a.epi == EPI_ACCUM, heun = a.epi == EPI_HEUN, so = a.epi == EPI_SO;
+    const bool need_tend = heun && a.b_lm != 0.0, need_q0 = heun || so;
 
     for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
         const int tile_id = wi / nchunk, kc = wi - tile_id * nchunk;
@@ -126,108 +143,164 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
         const int il = min(i, nx - 2), jl = THREE_D ? min(j, ny - 2) : 1;
         const size_t col = (size_t)(il - 1) + (size_t)nx * (jl - 1);
         auto Qz = [&](int kk) { return a.q[col + (size_t)(clampi(kk, 1, nz) - 1) * plane]; };
-        auto Wz = [&](int kk) { return a.w[col + (size_t)(kk - 1) * plane]; };
-        // window q(k-1..k+2) and the carried bottom face of level k0
-        double z0 = Qz(k0 - 1), z1 = Qz(k0), z2 = Qz(k0 + 1), z3 = Qz(k0 + 2);
-        double fzb = 0.0;
-        if (k0 >= 2 && (a.dirmask & 4)) {
-            const int f = k0 - 1;
-            fzb = zface(f, nz, sp.weno_bottom, f == 1 ? Qz(1) : Qz(f - 1), Qz(f), Qz(f + 1), Qz(f + 2), Wz(f + 1));
-        }
-        for (int k = k0; k <= k1; ++k) {
-            const size_t kb = (size_t)(k - 1) * plane;
-            const double *__restrict__ qk = a.q + kb;
-            for (int t = tid; t < LH * LW; t += NT) {
-                const int r = t / LW, c = t - r * LW;
+        // fixed slots of this thread: tile values (ghost columns read as the
+        // caller filled them; positions past the array take a periodic image
+        // that feeds no interior cell) and faces of the flattened list
+        // [x faces row-major | y faces] with their velocity offsets
+        int t_off[TS], f_base[FS], f_step[FS];
+        size_t f_off[FS];
+#pragma unroll
+        for (int s = 0; s < TS; ++s) {
+            const int e = tid + s * NT;
+            t_off[s] = -1;
+            if (e < LH * LW) {
+                const int r = e / LW, c = e - r * LW;
                 int ii = i0 - 2 + c;
-                if (ii > nx) ii = (ii - 4) % mx + 4;  // past the array: periodic image, feeds no interior cell
+                if (ii > nx) ii = (ii - 4) % mx + 4;
                 int jj = 1;
                 if (THREE_D) {
                     jj = j0 - 2 + r;
                     if (jj > ny) jj = (jj - 4) % my + 4;
                 }
-                tile[r][c] = qk[(size_t)(ii - 1) + (size_t)nx * (jj - 1)];
+                t_off[s] = (ii - 1) + nx * (jj - 1);
             }
-            __syncthreads();
-            for (int e = tid; e < NXF + NYF; e += NT) {
-                if (e < NXF) {
-                    // x face f + 1/2 (velocity u(f+1)), f = i0-1+fl, of tile row r
-                    const int r = e / (TX + 1), fl = e - r * (TX + 1);
-                    const int f = i0 - 1 + fl;
-                    const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
-                    const int iu = min(f + 1, nx);
-                    const double uu = a.u[kb + (size_t)(iu - 1) + (size_t)nx * (jr - 1)];
-                    const int lr = THREE_D ? r + 2 : 0;
-                    fx[r][fl] = weno_face_up(tile[lr][fl], tile[lr][fl + 1], tile[lr][fl + 2], tile[lr][fl + 3], uu,
-                                             true);
-                } else if constexpr (THREE_D) {
-                    // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
-                    const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
-                    const int gf = j0 - 1 + fr;
-                    const int ic = min(i0 + c, nx - 2);
-                    const int jv = min(gf + 1, ny);
-                    const double vv = a.v[kb + (size_t)(ic - 1) + (size_t)nx * (jv - 1)];
-                    fy[fr][c] = weno_face_up(tile[fr][c + 2], tile[fr + 1][c + 2], tile[fr + 2][c + 2],
-                                             tile[fr + 3][c + 2], vv, true);
-                }
+        }
+#pragma unroll
+        for (int s = 0; s < FS; ++s) {
+            const int e = tid + s * NT;
+            f_base[s] = 0;
+            f_step[s] = 0;
+            f_off[s] = 0;
+            if (e < NXF) {
+                // x face f + 1/2 (velocity u(f+1)), f = i0-1+fl, of tile row r
+                const int r = e / (TX + 1), fl = e - r * (TX + 1);
+                const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
+                f_base[s] = (THREE_D ? r + 2 : 0) * LW + fl;
+                f_step[s] = 1;
+                f_off[s] = (size_t)(min(i0 + fl, nx) - 1) + (size_t)nx * (jr - 1);
+            } else if (e < NF) {
+                // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
+                const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
+                f_base[s] = fr * LW + c + 2;
+                f_step[s] = LW;
+                f_off[s] = (size_t)(min(i0 + c, nx - 2) - 1) + (size_t)nx * (min(j0 + fr, ny) - 1);
             }
-            __syncthreads();
+        }
+        // A chunk above the floor first evaluates its bottom face k0-1/2 in a
+        // z-only pass, so that every z face comes from the same code.
+        const int kstart = (k0 > 1 && zdir) ? k0 - 1 : k0;
+        double z0 = Qz(kstart - 1), z1 = Qz(kstart), z2 = Qz(kstart + 1), z3 = Qz(kstart + 2);
+        double wz = kstart < nz ? a.w[col + (size_t)kstart * plane] : 0.0;
+        double fzb = 0.0;
+        // operands of level k, requested one level ahead
+        double tv[TS], vv[FS], wv = 0.0, zn = 0.0, q0v = 0.0, tdv = 0.0, xsv = 0.0;
+        auto fetch = [&](int k) {
+            const size_t kb = (size_t)(k - 1) * plane;
+            const double *qk = a.q + kb;
+#pragma unroll
+            for (int s = 0; s < TS; ++s)
+                if (t_off[s] >= 0) tv[s] = qk[t_off[s]];
+#pragma unroll
+            for (int s = 0; s < FS; ++s)
+                vv[s] = f_step[s] == 1 ? a.u[kb + f_off[s]] : (f_step[s] != 0 ? a.v[kb + f_off[s]] : 0.0);
+            wv = k < nz ? a.w[col + kb + plane] : 0.0;
+            zn = Qz(k + 2);
+            if (valid) {
+                if (need_q0) q0v = a.q0[col + kb];
+                if (need_tend) tdv = a.tend[col + kb];
+                if (accum) xsv = a.xs[col + kb];
+            }
+        };
+        fetch(k0);
+#pragma unroll 1
+        for (int k = kstart; k <= k1; ++k) {
+            if (k > kstart) {
+                z0 = z1;
+                z1 = z2;
+                z2 = z3;
+                z3 = zn;
+            }
+            const double wc = k < k0 ? wz : wv;
             double fzt = 0.0;
-            const bool need_top = k <= nz - 1 && (a.dirmask & 4);
-            if (need_top) fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, Wz(k + 1));
+            if (zdir && k <= nz - 1) fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
+            if (k < k0) {
+                fzb = fzt;
+                continue;
+            }
+            const size_t kb = (size_t)(k - 1) * plane;
+#pragma unroll
+            for (int s = 0; s < TS; ++s)
+                if (t_off[s] >= 0) tile[tid + s * NT] = tv[s];
+            double vc[FS];
+#pragma unroll
+            for (int s = 0; s < FS; ++s) vc[s] = vv[s];
+            const double q0c = q0v, tdc = tdv, xsc = xsv;
+            __syncthreads();
+            if (k < k1) fetch(k + 1);
+            // one face at a time: the exact FP64 face is register-hungry
+#pragma unroll 1
+            for (int s = 0; s < FS; ++s) {
+                const int st = f_step[s];
+                if (st == 0) continue;
+                const int b = f_base[s];
+                face[tid + s * NT] = weno_face_up(tile[b], tile[b + st], tile[b + 2 * st], tile[b + 3 * st], vc[s], true);
+            }
+            __syncthreads();
             if (valid) {
                 const size_t id = col + kb;
                 double X = 0.0, Y = 0.0, Z = 0.0;
                 if (a.dirmask & 1) {
-                    X = (fx[ty][tx + 1] - fx[ty][tx]) * dx1;
+                    const int e = ty * (TX + 1) + tx;
+                    X = (face[e + 1] - face[e]) * dx1;
                     if (adv) {
                         const double *urow = a.u + kb + (size_t)nx * (j - 1);
                         X = X - z1 * (urow[i] - urow[i - 1]) * dx1;
                     }
                 }
                 if (THREE_D && (a.dirmask & 2)) {
-                    Y = (fy[ty + 1][tx] - fy[ty][tx]) * dy1;
+                    const int e = NXF + ty * TX + tx;
+                    Y = (face[e + TX] - face[e]) * dy1;
                     if (adv) {
                         const double *vcol = a.v + kb + (size_t)(i - 1);
                         Y = Y - z1 * (vcol[(size_t)nx * j] - vcol[(size_t)nx * (j - 1)]) * dy1;
                     }
                 }
-                if (a.dirmask & 4) {
+                if (zdir) {
                     // z_assemble (no-flux 'nnf' floor/lid, zero z tendency at other ends)
                     if (k >= 2 && k <= nz - 1) {
                         Z = (fzt - fzb) * dz1w[k - 1];
-                        if (adv) Z = Z - z1 * (Wz(k + 1) - Wz(k)) * dz1w[k - 1];
+                        if (adv) Z = Z - z1 * (wc - a.w[col + kb]) * dz1w[k - 1];
                     } else {
                         double t = 0.0;
                         if (k == 1 && bc5 == 1) {
                             t = t + (fzt - 0.0) * dz1w[0];
-                            if (adv) t = t - z1 * (Wz(2) - 0.0) * dz1w[0];
+                            if (adv) t = t - z1 * (wc - 0.0) * dz1w[0];
                         }
                         if (k == nz && bc6 == 1) {
                             t = t + (0.0 - fzb) * dz1w[nz - 1];
-                            if (adv) t = t - z1 * (0.0 - Wz(nz)) * dz1w[nz - 1];
+                            if (adv) t = t - z1 * (0.0 - a.w[col + kb]) * dz1w[nz - 1];
                         }
                         Z = t;
                     }
                 }
-                if (a.epi == EPI_ACCUM || a.epi == EPI_OVERWRITE) {
-                    double r = (a.epi == EPI_ACCUM) ? a.xs[id] : 0.0;
+                if (accum || a.epi == EPI_OVERWRITE) {
+                    double r = accum ? xsc : 0.0;
                     if (a.dirmask & 1) r = r - X;
                     if (THREE_D && (a.dirmask & 2)) r = r - Y;
-                    if (a.dirmask & 4) r = r - Z;
+                    if (zdir) r = r - Z;
                     a.xs[id] = r;
                 } else {
                     double A = 0.0;
                     if (a.dirmask & 1) A = A - X;
                     if (THREE_D && (a.dirmask & 2)) A = A - Y;
-                    if (a.dirmask & 4) A = A - Z;
+                    if (zdir) A = A - Z;
                     double qn;
-                    if (a.epi == EPI_HEUN) {
-                        const double T = (a.b_lm != 0.0) ? A + a.b_lm * a.tend[id] : A + 0.0;
+                    if (heun) {
+                        const double T = need_tend ? A + a.b_lm * tdc : A + 0.0;
                         a.tend[id] = T;
-                        qn = a.q0[id] + a.adt * T;
+                        qn = q0c + a.adt * T;
                     } else {
-                        qn = a.c0 * a.q0[id] + a.c1 * (z1 + a.dt * A);
+                        qn = a.c0 * q0c + a.c1 * (z1 + a.dt * A);
                     }
                     a.qn[id] = qn;
                     if (a.fill) {
@@ -249,11 +322,6 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
                 }
             }
             fzb = fzt;
-            z0 = z1;
-            z1 = z2;
-            z2 = z3;
-            z3 = Qz(k + 3);
-            __syncthreads();
         }
     }
 }
