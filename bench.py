@@ -36,8 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # (--copy-probe 256: FETCH_SIZE counts half the bytes read, WRITE_SIZE all
 # bytes written).  Reported only for the exact configuration measured.
 HBM_TRAFFIC = {
-    ("plain", "256,256,128", "weno", "heun", 2): 554.3e6,
+    ("plain", "256,256,128", "weno", "heun", 2): 533.8e6,
     ("fused", "512,512,160", "weno", "ssprk3", 17): 13.79e9,
+    ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1063.6e6,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
 
@@ -298,7 +299,8 @@ def adjoint(args):
                    "self_contained_ratio": t_adj_self / t_fwd,
                    "device_arrays": 3 + nseg + 3 * kk},
         "roofline": {"bound": "hbm", "kernel": "adjoint_kernel<64,8> (one stage adjoint)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": HBM_TRAFFIC.get(("adjoint", args.size, "weno", "ssprk3", 1)),
                      "alg_bytes_per_launch": adj_bytes, "avg_launch_us": adj_kernel_s * 1e6},
         "gradient_norms": {"q0_lognormal": q_grad_norm, "q0_theta": theta_grad_norm},
         "cpu_baseline": None,
