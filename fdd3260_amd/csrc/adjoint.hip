@@ -41,17 +41,30 @@
 #include <algorithm>
 
 #include "mimadv_internal.hpp"
+#include "weno_ratio.hpp"
 
 namespace mimadv {
 
 namespace {
+
+// 1/x from the hardware estimate and two Newton steps (within an ulp or
+// two; 0 and infinities give NaN, which the callers treat like the
+// reference's infinite or undefined quotient).
+__device__ __forceinline__ double rcp_nr(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
 
 // Face value R and Rb * dR/d(a, b, c, d).  kind: 0 loop body (limiter and,
 // when nonlin, the ratio-20 weights), 2 top closure (no limiter).
 __device__ __forceinline__ double face_vjp(double a, double b, double c, double d, bool upos, int kind, bool nonlin,
                                            double Rb, double &ga, double &gb, double &gc, double &gd)
 {
-    const double eps = 1.e-15, g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, thres = 20.0;
+    const double eps = 1.e-15, g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
     const double f1 = -0.5 * a + 1.5 * b, f2 = 0.5 * b + 0.5 * c, f4 = 1.5 * c - 0.5 * d;
     double f1b = 0.0, f2b = 0.0, f4b = 0.0;
     double ab = 0.0, bb = 0.0, cb = 0.0, db = 0.0;
@@ -77,7 +90,7 @@ __device__ __forceinline__ double face_vjp(double a, double b, double c, double 
             sel = 4;
             m = f4;
         }
-        const double rden = 1.0 / (m - b + eps);
+        const double rden = rcp_nr(m - b + eps);
         const double r = -b * rden;
         const double ar = r < 0.0 ? -r : r;
         const bool clamped = !(ar < 1.0);
@@ -89,19 +102,19 @@ __device__ __forceinline__ double face_vjp(double a, double b, double c, double 
             const double to = upos ? b - a : d - c, ti = c - b;
             const double eo = to * to, ei = ti * ti;
             const double mx = eo > ei ? eo : ei, mn = eo < ei ? eo : ei;
-            if (mx / (mn + eps) > thres) {
+            if (ratio_above_20(mx, mn + eps)) {
+                // wo = (g1/Q)/(g1/Q + g2/P) = P/(P + 2Q), wi = 2Q/(P + 2Q)
+                // with Q = (eps + eo)^2, P = (eps + ei)^2
                 const double po = eps + eo, pi = eps + ei;
-                const double wO = g1 / (po * po), wI = g2 / (pi * pi);
-                const double rs = 1.0 / (wO + wI);
-                wo = wO * rs;
-                wi = wI * rs;
-                // wo = wO/(wO+wI), wi = wI/(wO+wI)
-                const double wob = Rb * fo_r, wib = Rb * f2_r;
-                const double t = wob * wo + wib * wi;
-                const double wOb = (wob - t) * rs, wIb = (wib - t) * rs;
-                // wX = gX/(eps + tX^2)^2  =>  dwX/dtX = -4 wX tX/(eps + tX^2)
-                const double tob = -4.0 * wOb * wO * to / po;
-                const double tib = -4.0 * wIb * wI * ti / pi;
+                const double Q = po * po, P = pi * pi;
+                const double rS = rcp_nr(P + 2.0 * Q);
+                wo = P * rS;
+                wi = 2.0 * Q * rS;
+                // dR/dwo = fo_r, dR/dwi = f2_r; dwo/dP = -dwi/dP = 2Q/S^2,
+                // dwo/dQ = -dwi/dQ = -2P/S^2; dP/dti = 4 pi ti, dQ/dto = 4 po to
+                const double t = 2.0 * Rb * (fo_r - f2_r) * rS * rS;
+                const double tob = -4.0 * t * P * po * to;
+                const double tib = 4.0 * t * Q * pi * ti;
                 if (upos) {
                     bb += tob;
                     ab -= tob;
@@ -113,7 +126,7 @@ __device__ __forceinline__ double face_vjp(double a, double b, double c, double 
                 bb -= tib;
             }
         }
-        R = upos ? wo * fo_r + wi * f2_r : wi * f2_r + wo * fo_r;
+        R = wo * fo_r + wi * f2_r;
         const double fo_rb = Rb * wo, f2_rb = Rb * wi;
         if (upos) f1b += fo_rb * phi;
         else f4b += fo_rb * phi;
@@ -238,7 +251,9 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
                     xy += cf[0][ey + 3 * TX] + cf[1][ey + 2 * TX] + cf[2][ey + TX] + cf[3][ey];
                 }
                 acc1 += xy;
-                __syncthreads();
+                // no barrier before the next level's tile writes: cf is
+                // rewritten only after the next barrier, which every thread
+                // reaches after its gather
             }
             // z face k+1/2 (velocity w(k+1)); no flux through floor and lid
             if (k <= nz - 1) {
