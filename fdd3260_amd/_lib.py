@@ -29,6 +29,7 @@ DIR_X, DIR_Y, DIR_Z, DIR_ALL = 1, 2, 4, 7
 OVERWRITE = 8
 HOST_PTRS = 16
 GRAPH = 32
+MOMENTUM = 64
 F64, F32 = 0, 1
 OWN_STREAM = C.c_void_p(-1 & ((1 << 64) - 1)).value
 

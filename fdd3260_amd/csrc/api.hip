@@ -569,6 +569,16 @@ int check_batch(mimadv_ctx *c, const float *u, const float *v, const float *w)
     return MIMADV_OK;
 }
 
+// MIMADV_MOMENTUM: the first three fields are FP32 momenta
+int check_momentum(mimadv_ctx *c, unsigned flags, int nf, const int *precision)
+{
+    if (!(flags & MIMADV_MOMENTUM)) return MIMADV_OK;
+    if (nf < 3) return fail(c, MIMADV_ERR_ARG, "MIMADV_MOMENTUM needs the three momenta as the first fields");
+    for (int n = 0; n < 3; ++n)
+        if (precision[n] != MIMADV_F32) return fail(c, MIMADV_ERR_ARG, "the batched momenta are MIMADV_F32");
+    return MIMADV_OK;
+}
+
 FusedArgs base_args(mimadv_ctx *c, const float *u, const float *v, const float *w)
 {
     FusedArgs A{};
@@ -640,23 +650,42 @@ int mimadv_batch_tendency(mimadv_ctx *c, const float *u, const float *v, const f
     if (rc) return rc;
     if (nf <= 0) return nf == 0 ? MIMADV_OK : fail(c, MIMADV_ERR_ARG, "nf < 0");
     if (!precision || !q || !xs) return fail(c, MIMADV_ERR_ARG, "NULL field list");
-    std::vector<BatchGroup> groups;
-    rc = plan_groups(c, nf, precision, groups);
+    const int nm = (flags & MIMADV_MOMENTUM) ? 3 : 0;
+    rc = check_momentum(c, flags, nf, precision);
     if (rc) return rc;
-    std::vector<const void *> in(nf), in0(nf);
-    std::vector<void *> out(nf);
+    if (nm && (flags & MIMADV_DIR_ALL) && (flags & MIMADV_DIR_ALL) != MIMADV_DIR_ALL)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched momentum tendency covers all directions");
+    std::vector<BatchGroup> groups;
+    rc = plan_groups(c, nf - nm, precision + nm, groups);
+    if (rc) return rc;
+    std::vector<const void *> in(nf - nm), in0(nf - nm);
+    std::vector<void *> out(nf - nm);
     for (int f = 0; f < nf; ++f) {
         if (!q[f] || !xs[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
-        in[f] = q[f];
-        in0[f] = q[f];
-        out[f] = xs[f];
+        if (f < nm) continue;
+        in[f - nm] = q[f];
+        in0[f - nm] = q[f];
+        out[f - nm] = xs[f];
     }
     FusedArgs A = base_args(c, u, v, w);
     A.epi = 0;
     A.fill = 0;
     A.dirmask = (flags & MIMADV_DIR_ALL) ? (int)(flags & MIMADV_DIR_ALL) : 7;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launch_groups(c, A, groups, in, in0, out));
+    if (nf > nm) HIPCHK(c, launch_groups(c, A, groups, in, in0, out));
+    if (nm) {
+        MomBatchArgs M{};
+        for (int n = 0; n < 3; ++n) {
+            M.m[n] = (const float *)q[n];
+            M.m0[n] = (const float *)q[n];
+            M.out[n] = (float *)xs[n];
+        }
+        M.ut = u;
+        M.vt = v ? v : u;
+        M.wt = w;
+        M.epi = 0;
+        HIPCHK(c, launch_momentum_batch(c->g, c->opts.mom_ord, c->opts.lavisc, M, c->stream));
+    }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
     return MIMADV_OK;
@@ -674,8 +703,11 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
     if (nsteps < 0 || nf < 0) return fail(c, MIMADV_ERR_ARG, "negative count");
     if (nsteps == 0 || nf == 0) return MIMADV_OK;
     if (!precision || !q) return fail(c, MIMADV_ERR_ARG, "NULL field list");
+    const int nm = (flags & MIMADV_MOMENTUM) ? 3 : 0;
+    rc = check_momentum(c, flags, nf, precision);
+    if (rc) return rc;
     std::vector<BatchGroup> groups;
-    rc = plan_groups(c, nf, precision, groups);
+    rc = plan_groups(c, nf - nm, precision + nm, groups);
     if (rc) return rc;
     size_t n3 = c->n3();
     // two stage buffers per field, in the field's precision
@@ -725,15 +757,35 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
             FusedArgs B = A;
             B.c0 = s.c0;
             B.c1 = s.c1;
-            std::vector<const void *> in(nf), in0(nf);
-            std::vector<void *> out(nf);
-            for (int f = 0; f < nf; ++f) {
-                in[f] = pick(s.in, f);
-                in0[f] = q[f];
-                out[f] = pick(s.out, f);
+            std::vector<const void *> in(nf - nm), in0(nf - nm);
+            std::vector<void *> out(nf - nm);
+            for (int f = nm; f < nf; ++f) {
+                in[f - nm] = pick(s.in, f);
+                in0[f - nm] = q[f];
+                out[f - nm] = pick(s.out, f);
             }
-            hipError_t e = launch_groups(c, B, groups, in, in0, out);
-            if (e != hipSuccess) return e;
+            if (nf > nm) {
+                hipError_t e = launch_groups(c, B, groups, in, in0, out);
+                if (e != hipSuccess) return e;
+            }
+            if (nm) {
+                MomBatchArgs M{};
+                for (int n = 0; n < 3; ++n) {
+                    M.m[n] = (const float *)pick(s.in, n);
+                    M.m0[n] = (const float *)q[n];
+                    M.out[n] = (float *)pick(s.out, n);
+                }
+                M.ut = u;
+                M.vt = v ? v : u;
+                M.wt = w;
+                M.c0 = s.c0;
+                M.c1 = s.c1;
+                M.dt = dt;
+                M.epi = 1;
+                M.fill = 1;
+                hipError_t e = launch_momentum_batch(c->g, c->opts.mom_ord, c->opts.lavisc, M, c->stream);
+                if (e != hipSuccess) return e;
+            }
         }
         return hipSuccess;
     };
@@ -744,6 +796,8 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         for (int f = 0; f < nf; ++f) key.push_back(q[f]);
         key.push_back((const void *)(size_t)integrator);
         key.push_back((const void *)(size_t)nf);
+        key.push_back((const void *)(size_t)nm);
+        for (int f = 0; f < nf; ++f) key.push_back((const void *)(size_t)precision[f]);
         double dtk = dt;
         size_t dbits;
         std::memcpy(&dbits, &dtk, sizeof dbits);

@@ -70,6 +70,19 @@ hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const Sca
 hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
                                 const double *mv, const double *mw, const double *ut, const double *vt,
                                 const double *wt, double *ua, double *va, double *wa, hipStream_t s);
+
+// Batched momentum stage (FP32 storage): m = (rho0 u, rho0 v, rho0 w) stage
+// input, m0 the step-start state, out the result; ut/vt/wt the advecting
+// velocities.  epi 0: tendency, 1: Shu-Osher stage c0*m0 + c1*(m + dt*L).
+struct MomBatchArgs {
+    const float *m[3];
+    const float *m0[3];
+    float *out[3];
+    const float *ut, *vt, *wt;
+    double c0, c1, dt;
+    int epi, fill;
+};
+hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s);
 hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
 hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
                              double *tend, double *x, hipStream_t s);

@@ -20,6 +20,7 @@ namespace {
 
 constexpr double AVISC = 1.0 / 16.0;
 
+
 struct MomCoefs {
     double c1s, c2s;  // self lines (4/3, 1/3)
     double c1c, c2c;  // cross lines (9/8, 1/8)
@@ -173,6 +174,86 @@ __device__ __forceinline__ double w_z(double t, int k, int nz, const GridDev &g,
     return t;
 }
 
+// The three momentum increments of cell (i, j, k), as qnadv_uvw accumulates
+// them (uadv_fd, vadv_fd, wadv_fd, windadv.f90:82-492): directions in the
+// order x, y, z (even ntime) or z, y, x (odd).  M* and T* read the momenta
+// and the advecting velocity at (ii, jj, kk) as doubles.
+template <class MU, class MV, class MW, class UT, class VT, class WT>
+__device__ __forceinline__ void mom_cell(const GridDev &g, const MomCoefs &c, int lavisc, bool even, int i, int j,
+                                         int k, MU mu, MV mv, MW mw, UT ut, VT vt, WT wt, double &ua, double &va,
+                                         double &wa)
+{
+    const int nz = g.nz;
+    const bool three_d = g.ny > 1;
+    int order[3];
+    if (even) { order[0] = 0; order[1] = 1; order[2] = 2; }
+    else { order[0] = 2; order[1] = 1; order[2] = 0; }
+
+    // ---- u (uadv_fd, windadv.f90:82-220)
+    {
+        double t = 0.0;
+        for (int s = 0; s < 3; ++s) {
+            int d = order[s];
+            if (d == 0) {
+                t = self_h(t, i, g.dx, c, lavisc, [&](int ii) { return mu(ii, j, k); },
+                           [&](int ii) { return ut(ii, j, k); });
+            } else if (d == 1) {
+                if (!three_d) continue;
+                t = cross_h(t, j, g.dy, c, lavisc, [&](int jj) { return mu(i, jj, k); },
+                            [&](int jj) { return vt(i - 1, jj, k); }, [&](int jj) { return vt(i, jj, k); },
+                            [&](int jj) { return vt(i - 2, jj, k); }, [&](int jj) { return vt(i + 1, jj, k); });
+            } else {
+                t = cross_z(t, k, nz, g, c, lavisc, 1, [&](int kk) { return mu(i, j, kk); },
+                            [&](int kk) { return wt(i - 1, j, kk); }, [&](int kk) { return wt(i, j, kk); },
+                            [&](int kk) { return wt(i - 2, j, kk); }, [&](int kk) { return wt(i + 1, j, kk); });
+            }
+        }
+        ua = t;
+    }
+    // ---- v (vadv_fd, windadv.f90:223-357), 3D only
+    va = 0.0;
+    if (three_d) {
+        double t = 0.0;
+        for (int s = 0; s < 3; ++s) {
+            int d = order[s];
+            if (d == 0) {
+                t = cross_h(t, i, g.dx, c, lavisc, [&](int ii) { return mv(ii, j, k); },
+                            [&](int ii) { return ut(ii, j - 1, k); }, [&](int ii) { return ut(ii, j, k); },
+                            [&](int ii) { return ut(ii, j - 2, k); }, [&](int ii) { return ut(ii, j + 1, k); });
+            } else if (d == 1) {
+                t = self_h(t, j, g.dy, c, lavisc, [&](int jj) { return mv(i, jj, k); },
+                           [&](int jj) { return vt(i, jj, k); });
+            } else {
+                t = cross_z(t, k, nz, g, c, lavisc, 2, [&](int kk) { return mv(i, j, kk); },
+                            [&](int kk) { return wt(i, j - 1, kk); }, [&](int kk) { return wt(i, j, kk); },
+                            [&](int kk) { return wt(i, j - 2, kk); }, [&](int kk) { return wt(i, j + 1, kk); });
+            }
+        }
+        va = t;
+    }
+    // ---- w (wadv_fd, windadv.f90:360-492): levels 2..nz
+    {
+        double t = 0.0;
+        if (k >= 2) {
+            for (int s = 0; s < 3; ++s) {
+                int d = order[s];
+                if (d == 0) {
+                    t = w_h(t, i, k, nz, g.dx, c, lavisc, [&](int ii, int kk) { return mw(ii, j, kk); },
+                            [&](int ii, int kk) { return ut(ii, j, kk); });
+                } else if (d == 1) {
+                    if (!three_d) continue;
+                    t = w_h(t, j, k, nz, g.dy, c, lavisc, [&](int jj, int kk) { return mw(i, jj, kk); },
+                            [&](int jj, int kk) { return vt(i, jj, kk); });
+                } else {
+                    t = w_z(t, k, nz, g, c, lavisc, [&](int kk) { return mw(i, j, kk); },
+                            [&](int kk) { return wt(i, j, kk); });
+                }
+            }
+        }
+        wa = t;
+    }
+}
+
 __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs c, int lavisc, int ntime,
                                                             const double *__restrict__ mu,
                                                             const double *__restrict__ mv,
@@ -182,7 +263,7 @@ __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs 
                                                             const double *__restrict__ wt, double *__restrict__ ua,
                                                             double *__restrict__ va, double *__restrict__ wa)
 {
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nx = g.nx, ny = g.ny;
     const bool three_d = ny > 1;
     const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
     const int j = blockIdx.y + 1;
@@ -197,83 +278,146 @@ __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs 
         wa[id] = 0.0;
         return;
     }
-    auto at = [&](const double *p, int ii, int jj, int kk) {
-        return p[(size_t)(ii - 1) + sy * (jj - 1) + sz * (kk - 1)];
+    auto acc = [&](const double *p) {
+        return [=](int ii, int jj, int kk) { return p[(size_t)(ii - 1) + sy * (jj - 1) + sz * (kk - 1)]; };
     };
-    const bool even = (ntime % 2) == 0;
-    int order[3];
-    if (even) { order[0] = 0; order[1] = 1; order[2] = 2; }
-    else { order[0] = 2; order[1] = 1; order[2] = 0; }
+    double tu, tv, tw;
+    mom_cell(g, c, lavisc, (ntime % 2) == 0, i, j, k, acc(mu), acc(mv), acc(mw), acc(ut), acc(vt), acc(wt), tu, tv,
+             tw);
+    ua[id] = tu;
+    if (three_d) va[id] = tv;
+    wa[id] = tw;
+}
 
-    // ---- u (uadv_fd, windadv.f90:82-220)
-    {
-        double t = 0.0;
-        for (int s = 0; s < 3; ++s) {
-            int d = order[s];
-            if (d == 0) {
-                t = self_h(t, i, g.dx, c, lavisc, [&](int ii) { return at(mu, ii, j, k); },
-                           [&](int ii) { return at(ut, ii, j, k); });
-            } else if (d == 1) {
-                if (!three_d) continue;
-                t = cross_h(t, j, g.dy, c, lavisc, [&](int jj) { return at(mu, i, jj, k); },
-                            [&](int jj) { return at(vt, i - 1, jj, k); }, [&](int jj) { return at(vt, i, jj, k); },
-                            [&](int jj) { return at(vt, i - 2, jj, k); }, [&](int jj) { return at(vt, i + 1, jj, k); });
-            } else {
-                t = cross_z(t, k, nz, g, c, lavisc, 1, [&](int kk) { return at(mu, i, j, kk); },
-                            [&](int kk) { return at(wt, i - 1, j, kk); }, [&](int kk) { return at(wt, i, j, kk); },
-                            [&](int kk) { return at(wt, i - 2, j, kk); }, [&](int kk) { return at(wt, i + 1, j, kk); });
+// Batched momentum stage: FP32 momenta advected by the FP32 velocities of
+// the batch, FP64 arithmetic on the loaded values.  epi 0: out = -(div), the
+// tendency the caller of qnadv_uvw forms (momentum.f90 subtracts winda);
+// epi 1: out = c0*m0 + c1*(m + dt*(-div)) rounded once to FP32, with the
+// periodic images of the cell.
+//
+// Column-marching form: a workgroup owns a TX x TY tile of columns and a
+// chunk of levels.  Every operand the stencils of level k read (the three
+// momenta and the three advecting velocities at levels k-2..k+2, columns
+// within two cells of the tile) sits in a five-level LDS ring of FP32
+// tiles with a 2-cell halo; each level is loaded once, coalesced,
+// and serves five output levels.  The per-cell formulas are those of the
+// FP64 kernel (mom_cell), so results do not depend on the mapping.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void momentum_march_kernel(GridDev g, MomCoefs c, int lavisc, MomBatchArgs a,
+                                                                int kchunk, int ntx, int nwork)
+{
+    constexpr bool THREE_D = TY > 1;
+    constexpr int NT = TX * TY;
+    constexpr int H = 2, LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1, PL = LW * LH;
+    constexpr int NR = 5, TS = (PL + NT - 1) / NT;
+    __shared__ float ring[6][NR][PL];  // mu, mv, mw, ut, vt, wt
+
+    const int nx = g.nx, ny = g.ny, nz = g.nz, mx = nx - 5, my = ny - 5;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int nchunk = (nz + kchunk - 1) / kchunk;
+    const float *src[6] = {a.m[0], a.m[1], a.m[2], a.ut, a.vt, a.wt};
+
+    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+        const int tile = wi / nchunk, kc = wi - tile * nchunk;
+        const int i0 = 4 + (tile % ntx) * TX;
+        const int j0 = THREE_D ? 4 + (tile / ntx) * TY : 1;
+        const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
+        const int i = i0 + tx, j = j0 + ty;
+        const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
+        // plane offsets of this thread's ring entries (periodic images for
+        // positions outside the interior; -1: no entry)
+        int t_off[TS];
+#pragma unroll
+        for (int s = 0; s < TS; ++s) {
+            const int e = tid + s * NT;
+            t_off[s] = -1;
+            if (e < PL) {
+                const int r = e / LW, cc = e - r * LW;
+                // ghost columns are read as the caller filled them, like the
+                // reference; positions past the array (padding of a partial
+                // tile) take a periodic image that feeds no interior cell
+                int ii = i0 - H + cc;
+                if (ii > nx) ii = (ii - 4) % mx + 4;
+                int jj = 1;
+                if (THREE_D) {
+                    jj = j0 - H + r;
+                    if (jj > ny) jj = (jj - 4) % my + 4;
+                }
+                t_off[s] = (ii - 1) + nx * (jj - 1);
             }
         }
-        ua[id] = t;
-    }
-    // ---- v (vadv_fd, windadv.f90:223-357), 3D only
-    if (three_d) {
-        double t = 0.0;
-        for (int s = 0; s < 3; ++s) {
-            int d = order[s];
-            if (d == 0) {
-                t = cross_h(t, i, g.dx, c, lavisc, [&](int ii) { return at(mv, ii, j, k); },
-                            [&](int ii) { return at(ut, ii, j - 1, k); }, [&](int ii) { return at(ut, ii, j, k); },
-                            [&](int ii) { return at(ut, ii, j - 2, k); }, [&](int ii) { return at(ut, ii, j + 1, k); });
-            } else if (d == 1) {
-                t = self_h(t, j, g.dy, c, lavisc, [&](int jj) { return at(mv, i, jj, k); },
-                           [&](int jj) { return at(vt, i, jj, k); });
-            } else {
-                t = cross_z(t, k, nz, g, c, lavisc, 2, [&](int kk) { return at(mv, i, j, kk); },
-                            [&](int kk) { return at(wt, i, j - 1, kk); }, [&](int kk) { return at(wt, i, j, kk); },
-                            [&](int kk) { return at(wt, i, j - 2, kk); }, [&](int kk) { return at(wt, i, j + 1, kk); });
+        auto load_level = [&](int kk) {
+            const int slot = (kk + 2 * NR) % NR;
+            const bool in = kk >= 1 && kk <= nz;
+            const size_t kb = in ? sz * (kk - 1) : 0;
+#pragma unroll
+            for (int s = 0; s < TS; ++s) {
+                if (t_off[s] < 0) continue;
+#pragma unroll
+                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = in ? src[n][kb + t_off[s]] : 0.0f;
             }
-        }
-        va[id] = t;
-    }
-    // ---- w (wadv_fd, windadv.f90:360-492): levels 2..nz
-    {
-        double t = 0.0;
-        if (k >= 2) {
-            for (int s = 0; s < 3; ++s) {
-                int d = order[s];
-                if (d == 0) {
-                    t = w_h(t, i, k, nz, g.dx, c, lavisc, [&](int ii, int kk) { return at(mw, ii, j, kk); },
-                            [&](int ii, int kk) { return at(ut, ii, j, kk); });
-                } else if (d == 1) {
-                    if (!three_d) continue;
-                    t = w_h(t, j, k, nz, g.dy, c, lavisc, [&](int jj, int kk) { return at(mw, i, jj, kk); },
-                            [&](int jj, int kk) { return at(vt, i, jj, kk); });
-                } else {
-                    t = w_z(t, k, nz, g, c, lavisc, [&](int kk) { return at(mw, i, j, kk); },
-                            [&](int kk) { return at(wt, i, j, kk); });
+        };
+#pragma unroll 1
+        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) load_level(kk);
+#pragma unroll 1
+        for (int k = k0; k <= k1; ++k) {
+            load_level(k + 2);
+            __syncthreads();
+            if (valid) {
+                // ring offset of level kk (k-2 <= kk <= k+2)
+                auto lev = [&](int kk) { return ((kk + 2 * NR) % NR) * PL; };
+                auto acc = [&](int n) {
+                    return [&, n](int ii, int jj, int kk) {
+                        const int e = lev(kk) + (THREE_D ? (jj - j0 + H) * LW : 0) + (ii - i0 + H);
+                        return (double)ring[n][0][e];
+                    };
+                };
+                double d[3];
+                mom_cell(g, c, lavisc, true, i, j, k, acc(0), acc(1), acc(2), acc(3), acc(4), acc(5), d[0], d[1],
+                         d[2]);
+                const size_t kb = sz * (k - 1);
+                const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
+                const int ce = lev(k) + (THREE_D ? (ty + H) * LW : 0) + tx + H;
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    if (n == 1 && !THREE_D) continue;
+                    const double L = -d[n];
+                    float qv;
+                    if (a.epi == 0) {
+                        qv = (float)L;
+                    } else {
+                        const double m = (double)ring[n][0][ce];
+                        const double qn = a.c0 != 0.0 ? a.c0 * (double)a.m0[n][id] + a.c1 * (m + a.dt * L)
+                                                      : a.c1 * (m + a.dt * L);
+                        qv = (float)qn;
+                    }
+                    float *o = a.out[n];
+                    o[id] = qv;
+                    if (a.epi != 0 && a.fill) {
+                        // periodic images (qnbc_p4 / lbcu_p4 for the periodic case)
+                        int ix = 0, jy = 0;
+                        if (i >= nx - 4) ix = i - mx;
+                        else if (i <= 5) ix = i + mx;
+                        if (THREE_D) {
+                            if (j >= ny - 4) jy = j - my;
+                            else if (j <= 5) jy = j + my;
+                        }
+                        if (ix) o[kb + (size_t)(ix - 1) + sy * (j - 1)] = qv;
+                        if (jy) {
+                            o[kb + (size_t)(i - 1) + sy * (jy - 1)] = qv;
+                            if (ix) o[kb + (size_t)(ix - 1) + sy * (jy - 1)] = qv;
+                        }
+                    }
                 }
             }
+            // the next level's load overwrites the ring slot of level k-2
+            __syncthreads();
         }
-        wa[id] = t;
     }
 }
 
-}  // namespace
-
-hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
-                                const double *mv, const double *mw, const double *ut, const double *vt,
-                                const double *wt, double *ua, double *va, double *wa, hipStream_t s)
+MomCoefs mom_coefs(int mom_ord)
 {
     MomCoefs c;
     if (mom_ord == 4) {
@@ -283,10 +427,41 @@ hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int n
         c.c1s = 1.0; c.c2s = 0.0;
         c.c1c = 1.0; c.c2c = 0.0;
     }
+    return c;
+}
+
+}  // namespace
+
+hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
+                                const double *mv, const double *mw, const double *ut, const double *vt,
+                                const double *wt, double *ua, double *va, double *wa, hipStream_t s)
+{
+    const MomCoefs c = mom_coefs(mom_ord);
     dim3 block(256);
     dim3 grid((g.nx + 255) / 256, g.ny, g.nz);
     hipLaunchKernelGGL(momentum_fp64_kernel, grid, block, 0, s, g, c, lavisc, ntime, mu, mv, mw, ut, vt, wt, ua, va,
                        wa);
+    return hipGetLastError();
+}
+
+hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s)
+{
+    const MomCoefs c = mom_coefs(mom_ord);
+    const int kchunk = g.nz < 16 ? g.nz : 16;
+    const int nchunk = (g.nz + kchunk - 1) / kchunk;
+    if (g.ny > 1) {
+        constexpr int TX = 64, TY = 4;
+        const int ntx = (g.nx - 5 + TX - 1) / TX, nty = (g.ny - 5 + TY - 1) / TY;
+        const int nwork = ntx * nty * nchunk;
+        hipLaunchKernelGGL((momentum_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, c, lavisc, a, kchunk,
+                           ntx, nwork);
+    } else {
+        constexpr int TX = 64, TY = 1;
+        const int ntx = (g.nx - 5 + TX - 1) / TX;
+        const int nwork = ntx * nchunk;
+        hipLaunchKernelGGL((momentum_march_kernel<TX, TY>), dim3(nwork), dim3(TX), 0, s, g, c, lavisc, a, kchunk, ntx,
+                           nwork);
+    }
     return hipGetLastError();
 }
 

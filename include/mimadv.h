@@ -192,11 +192,17 @@ int mimadv_get_conserved(mimadv_ctx *ctx, const double *dens, int nf, double *co
 #define MIMADV_F64 0
 #define MIMADV_F32 1
 #define MIMADV_GRAPH 32u /* capture one step in a hipGraph and replay it */
+/* q[0], q[1], q[2] (MIMADV_F32) are the momenta rho0*u, rho0*v, rho0*w,
+ * advanced with the momentum operator of qnadv_uvw (windadv.f90:46-1127;
+ * the context's mom_ord and lavisc, directions in the even-ntime order) by
+ * the batch's velocities instead of with the scalar scheme; FP64 arithmetic
+ * on the FP32 values, one rounding per stage.  In 2D q[1] is not touched. */
+#define MIMADV_MOMENTUM 64u
 
 /* Tuning of the batched launches; 0 selects the default of each field. */
 typedef struct mimadv_tuning {
     int group32;        /* FP32 fields per launch (default and maximum 18) */
-    int kchunk;         /* levels per work item (default 32) */
+    int kchunk;         /* levels per work item (default and maximum 8) */
     int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
     int tile_y;         /* rows of the 64-wide column tile in 3D: 2, 4 or 8 (default 8) */
 } mimadv_tuning;

@@ -84,3 +84,16 @@ def test_null_context_is_an_argument_error():
     assert lib.mimadv_synchronize(None) == _lib.ERR_ARG
     assert lib.mimadv_last_error(None) == b"NULL context"
     assert lib.mimadv_scalar_tendency(None, 1.0, None, None, None, 0, None, None, None, 0) == _lib.ERR_ARG
+
+
+def test_header_constants_match_binding():
+    """Every integer #define of the header that the Python binding mirrors
+    (MIMADV_X -> _lib.X) has the same value."""
+    text = open(HEADER).read()
+    defs = dict(re.findall(r"#define\s+MIMADV_([A-Z0-9_]+)\s+\(?(-?\d+)u?\)?", text))
+    checked = 0
+    for name, val in defs.items():
+        if hasattr(_lib, name):
+            assert getattr(_lib, name) == int(val), name
+            checked += 1
+    assert checked >= 20

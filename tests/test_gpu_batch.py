@@ -158,3 +158,153 @@ def test_batch_tendency_per_direction(cuda, dirs):
             print(f"dirs={dirs} field {n} ({'fp64' if n < 2 else 'fp32'}): rel rms {rr:.3e} rel max {rm:.3e}",
                   flush=True)
             assert rr <= (1e-10 if n < 2 else 1e-5)
+
+
+def _momenta(g, u, v, w, seed=41):
+    """rho0*(u, v, w) with a level profile, perturbed, rounded to FP32."""
+    rng = np.random.default_rng(seed)
+    den = 1.2 * np.exp(-np.cumsum(g.dzw) / 8000.0)[:, None, None]
+    return [(a * den * (1.0 + 0.1 * rng.standard_normal(a.shape))).astype(np.float32).astype(np.float64)
+            for a in (u, v, w)]
+
+
+@pytest.mark.parametrize("mom_ord,lavisc", [(4, True), (2, False)])
+def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
+    """The three FP32 momenta in the batch (MIMADV_MOMENTUM) against the
+    oracle's qnadv_uvw on the same FP32-rounded inputs: the kernel evaluates
+    the reference formulas in FP64, so only the final rounding to FP32
+    separates them (held to 1e-7 of the field maximum).  The scalar members
+    of the same batch keep their own bounds."""
+    print(f"case batch_momentum_tendency ord={mom_ord} lavisc={lavisc}", flush=True)
+    g, u, v, w, f64, f32 = _case(nspec=3)
+    mom = _momenta(g, u, v, w)
+    ora = Oracle(g, scal_adv="weno", mom_ord=mom_ord, lavisc=lavisc)
+    ua, va, wa = ora.qnadv_uvw(0, *mom, u, v, w)
+    srefs = []
+    for q in f64 + f32:
+        xs = np.zeros(g.shape)
+        ora.scalar_tendency(0.0, u, v, w, q, xs)
+        srefs.append(xs)
+    with Advection(g, scal_adv="weno", mom_ord=mom_ord, lavisc=lavisc) as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = ([torch.from_numpy(m.astype(np.float32)).to(cuda) for m in mom] +
+                  [torch.from_numpy(q).to(cuda) for q in f64] +
+                  [torch.from_numpy(q.astype(np.float32)).to(cuda) for q in f32])
+        outs = [torch.zeros_like(f) for f in fields]
+        a.batch_tendency(du, dv, dw, fields, outs, momentum=True)
+        a.stream_sync()
+        for c, r in enumerate((ua, va, wa)):
+            got = outs[c].double().cpu().numpy()
+            assert _relmax(got, -r, g) <= 1e-7, f"momentum component {c}"
+        for n, (o, r) in enumerate(zip(outs[3:], srefs)):
+            got = o.double().cpu().numpy()
+            if n < len(f64):
+                assert _relmax(got, r, g) <= 1e-10, f"fp64 field {n}"
+            else:
+                assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
+
+
+@pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
+def test_batch_momentum_advance(cuda, integrator):
+    """Three steps of the momenta (advecting velocity held fixed) with θ, qv
+    and species in one batch, eager and from a hipGraph, against the FP64
+    oracle trajectory: FP32 storage, 1e-5 relative RMS; graph replay equal
+    to the eager launches bit for bit."""
+    print(f"case batch_momentum_advance {integrator}", flush=True)
+    g, u, v, w, f64, f32 = _case(nspec=2, zero_frac=0.0)
+    mom = _momenta(g, u, v, w)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    nsteps = 3
+    ora = Oracle(g, scal_adv="weno")
+    stages = ([(0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)] if integrator == "ssprk3"
+              else [(0.0, 1.0), (0.5, 0.5)])
+    mref = [m.copy() for m in mom]
+    for _ in range(nsteps):
+        m0 = [m.copy() for m in mref]
+        cur = [m.copy() for m in mref]
+        for c0, c1 in stages:
+            L = ora.qnadv_uvw(0, *cur, u, v, w)
+            cur = [c0 * a0 + c1 * (x - dt * l) for a0, x, l in zip(m0, cur, L)]
+            for x in cur:
+                ora.fill_scalar(x)
+        mref = cur
+    code = _lib.INT_SSPRK3 if integrator == "ssprk3" else _lib.INT_HEUN
+    results = []
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        for graph in (False, True):
+            fields = ([torch.from_numpy(m.astype(np.float32)).to(cuda) for m in mom] +
+                      [torch.from_numpy(q).to(cuda) for q in f64] +
+                      [torch.from_numpy(q.astype(np.float32)).to(cuda) for q in f32])
+            a.batch_advance(code, nsteps, dt, du, dv, dw, fields, graph=graph, momentum=True)
+            a.stream_sync()
+            results.append([f.double().cpu().numpy() for f in fields])
+            for c in range(3):
+                assert _rel_rms(results[-1][c], mref[c], g) <= 1e-5, f"momentum component {c}"
+                assert _relmax(results[-1][c], mref[c], g) <= 1e-5, f"momentum component {c}"
+    for x, y in zip(*results):
+        assert np.array_equal(x, y), "graph replay differs from eager launches"
+
+
+def test_batch_momentum_rejects(cuda):
+    g, u, v, w, f64, f32 = _case(nspec=1)
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        few = [torch.from_numpy(q.astype(np.float32)).to(cuda) for q in f32 * 2]
+        with pytest.raises(Exception, match="three momenta"):
+            a.batch_advance(_lib.INT_SSPRK3, 1, 1.0, du, dv, dw, few, momentum=True)
+        mixed = [torch.from_numpy(f64[0]).to(cuda)] + few + few
+        with pytest.raises(Exception, match="MIMADV_F32"):
+            a.batch_advance(_lib.INT_SSPRK3, 1, 1.0, du, dv, dw, mixed, momentum=True)
+
+
+def test_batch_config3_field_errors(cuda):
+    """The 20-field batch of the fused benchmark at reduced size (128x128x64:
+    3 FP32 momenta, θ and qv in FP64, 15 FP32 log-normal species with 30 %
+    exact zeros, 3 m/s wind), one SSP-RK3 step against the FP64 oracle from
+    the same FP32-rounded state.  Prints max and RMS error per field
+    (relative to the field's max and RMS) and holds the momenta and the FP64
+    members to their bounds; species with exact zeros carry the reference
+    scheme's discontinuity (see test_batch_advance_and_graph)."""
+    print("case batch_config3_field_errors", flush=True)
+    g = gm.stretched(128, 128, 64)
+    u, v, w = cases.turbulent_wind(g, seed=2, rms=3.0)
+    u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    den0 = 1.2 * np.exp(-g.zc() / 8000.0)[:, None, None]
+    state = [(a * den0).astype(np.float32).astype(np.float64) for a in (u, v, w)]
+    state += [cases.theta_profile(g, seed=5), cases.lognormal(g, seed=6, mu=-5.0, sigma=0.3)]
+    for s in range(15):
+        q = cases.lognormal(g, seed=10 + s, mu=-20 + 14 * s / 14, sigma=1.5, zero_frac=0.3)
+        state.append(q.astype(np.float32).astype(np.float64))
+    ora = Oracle(g, scal_adv="weno")
+    ora.set_threads(8)
+    m0 = state[:3]
+    cur = [m.copy() for m in m0]
+    for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
+        L = ora.qnadv_uvw(0, *cur, u, v, w)
+        cur = [c0 * a0 + c1 * (x - dt * l) for a0, x, l in zip(m0, cur, L)]
+        for x in cur:
+            ora.fill_scalar(x)
+    refs = cur
+    for q in state[3:]:
+        r = q.copy()
+        ora.ssprk3_step(dt, u, v, w, r)
+        refs.append(r)
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = [torch.from_numpy(x.astype(np.float64 if n in (3, 4) else np.float32)).to(cuda)
+                  for n, x in enumerate(state)]
+        a.batch_advance(_lib.INT_SSPRK3, 1, dt, du, dv, dw, fields, graph=True, momentum=True)
+        a.stream_sync()
+        got = [f.double().cpu().numpy() for f in fields]
+    names = ["rho0u", "rho0v", "rho0w", "theta", "qv"] + [f"species{s}" for s in range(15)]
+    for n, (x, r) in enumerate(zip(got, refs)):
+        mx, rms = _relmax(x, r, g), _rel_rms(x, r, g)
+        print(f"  {names[n]:10s} max {mx:.3e}  rms {rms:.3e}", flush=True)
+        if n < 3:
+            assert mx <= 1e-5 and rms <= 1e-6, names[n]
+        elif n < 5:
+            assert mx <= 1e-9, names[n]
+        else:
+            assert rms <= 1e-2, names[n]
