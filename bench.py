@@ -441,6 +441,7 @@ def main():
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of the column tile (2, 4, 8)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="plain workload: skip the scheme variants")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="after the run, copy this many MiB on the device (a known-size kernel that scales "
                          "hardware byte counters)")
@@ -499,6 +500,31 @@ def main():
     alg_bytes = sum(sb) / len(sb)
     achieved = alg_bytes / avg_launch_s / 1e9
 
+    # the other scheme variants of the same workload (and the reference scheme
+    # with SSP-RK3), timed the same way over a short run
+    variants = {}
+    if not args.no_variants and args.scheme == "weno" and args.integrator == "heun":
+        vsteps = max(1, min(args.steps, 20))
+        for name, scheme, lim, integ in (("weno/ssprk3", "weno", "MC", "ssprk3"), ("ppm/heun", "ppm", "MC", "heun"),
+                                         ("muscl-MC/heun", "muscl", "MC", "heun"),
+                                         ("weno5/ssprk3", "weno5", "MC", "ssprk3")):
+            va = Advection(g, scal_adv=scheme, muscllimiter=lim)
+            vq = [torch.from_numpy(f).to(dev) for f in fields]
+            vcode = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
+            vn = 2 if integ == "heun" else 3
+            va.advance(vcode, 2, dt, du, dv, dw, vq)
+            torch.cuda.synchronize()
+            ts = time.perf_counter()
+            va.advance(vcode, vsteps, dt, du, dv, dw, vq)
+            torch.cuda.synchronize()
+            vw = time.perf_counter() - ts
+            vk = va.last_kernel_ms() * 1e-3 / (nfield * vn * vsteps)
+            vb = sum(stage_bytes(integ, ncell)) / vn
+            variants[name] = {"value": ncell * nfield * vn * vsteps / vw, "avg_launch_us": vk * 1e6,
+                              "frac": vb / vk / 1e9 / HBM_PEAK_GBS, "steps": vsteps}
+            va.close()
+            del vq
+
     cpu = None
     if args.cpu_steps > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -536,6 +562,7 @@ def main():
             "fields": nfield,
             "stages_per_step": nst,
             "dt_s": dt,
+            "variants": variants,
         },
         "roofline": {
             "bound": "hbm",

@@ -347,23 +347,40 @@ __global__ __launch_bounds__(TX *TY) void momentum_march_kernel(GridDev g, MomCo
                 t_off[s] = (ii - 1) + nx * (jj - 1);
             }
         }
-        auto load_level = [&](int kk) {
-            const int slot = (kk + 2 * NR) % NR;
+        // level kk of the six operands into registers (zeros outside 1..nz),
+        // then into its ring slot
+        float pre[6][TS];
+        auto fetch = [&](int kk) {
             const bool in = kk >= 1 && kk <= nz;
             const size_t kb = in ? sz * (kk - 1) : 0;
+#pragma unroll
+            for (int s = 0; s < TS; ++s)
+#pragma unroll
+                for (int n = 0; n < 6; ++n) pre[n][s] = (in && t_off[s] >= 0) ? src[n][kb + t_off[s]] : 0.0f;
+        };
+        auto store = [&](int kk) {
+            const int slot = (kk + 2 * NR) % NR;
 #pragma unroll
             for (int s = 0; s < TS; ++s) {
                 if (t_off[s] < 0) continue;
 #pragma unroll
-                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = in ? src[n][kb + t_off[s]] : 0.0f;
+                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = pre[n][s];
             }
         };
 #pragma unroll 1
-        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) load_level(kk);
+        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) {
+            fetch(kk);
+            store(kk);
+        }
+        fetch(k0 + 2);
 #pragma unroll 1
         for (int k = k0; k <= k1; ++k) {
-            load_level(k + 2);
+            // level k+2 arrived during the previous level's arithmetic; the
+            // slot it takes (level k-3) was last read before the barrier
+            // that ended that level
+            store(k + 2);
             __syncthreads();
+            if (k < k1) fetch(k + 3);
             if (valid) {
                 // ring offset of level kk (k-2 <= kk <= k+2)
                 auto lev = [&](int kk) { return ((kk + 2 * NR) % NR) * PL; };
