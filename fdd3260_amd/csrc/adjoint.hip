@@ -186,10 +186,9 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
         const int ip = perio(i, nx, mx), jp = THREE_D ? perio(j, ny, my) : 1;
         const size_t col = (size_t)(ip - 1) + (size_t)nx * (jp - 1);
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;  // levels k-1 .. k+2
-        auto finalize = [&](int m, double val) {
+        auto finalize = [&](int m, double val, double qb) {
             if (!valid || m < k0 || m > k1) return;
             const size_t id = col + (size_t)(m - 1) * plane;
-            const double qb = A.qbn[id];
             const double o = A.ic * qb + val;
             if (A.accumulate) A.out[id] += o;
             else A.out[id] = o;
@@ -197,8 +196,24 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
             else if (A.acc0_mode == 2) A.acc0[id] += A.c0 * qb;
         };
         const int ks = max(1, k0 - 2), ke = min(nz, k1 + 1);
+        // column values at level kk (clamped to 1..nz; q(0) := q(1) is the
+        // floor convention of the first face)
+        auto Qz = [&](int kk) { return A.q[col + (size_t)(min(max(kk, 1), nz) - 1) * plane]; };
+        auto QBz = [&](int kk) { return A.qbn[col + (size_t)(min(max(kk, 1), nz) - 1) * plane]; };
+        // register windows of iteration k: q(k-1..k+2), qbn(k-1..k+1), w(k+1)
+        double z0 = Qz(ks - 1), z1 = Qz(ks), z2 = Qz(ks + 1), z3 = Qz(ks + 2);
+        double qbm = QBz(ks - 1), qb0 = QBz(ks), qb1 = QBz(ks + 1);
+        double wz = ks <= nz - 1 ? A.w[col + (size_t)ks * plane] : 0.0;
+#pragma unroll 1
         for (int k = ks; k <= ke; ++k) {
             const size_t kb = (size_t)(k - 1) * plane;
+            // the next iteration's column values, requested now
+            double zn = 0.0, qbn1 = 0.0, wn = 0.0;
+            if (k < ke) {
+                zn = Qz(k + 3);
+                qbn1 = QBz(k + 2);
+                wn = k + 1 <= nz - 1 ? A.w[col + (size_t)(k + 1) * plane] : 0.0;
+            }
             if (k >= k0 && k <= k1) {
                 for (int t = tid; t < QH * QW; t += NT) {
                     const int r = t / QW, c = t - r * QW;
@@ -257,18 +272,12 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
             }
             // z face k+1/2 (velocity w(k+1)); no flux through floor and lid
             if (k <= nz - 1) {
-                auto Qz = [&](int kk) { return A.q[col + (size_t)(min(max(kk, 1), nz) - 1) * plane]; };
-                const double wf = A.w[col + (size_t)k * plane];
-                const double Fb = A.ls * (A.qbn[col + (size_t)k * plane] * A.dz1w[k] - A.qbn[col + kb] * A.dz1w[k - 1]);
-                const bool upos = !(wf < 0.0);
+                const double Fb = A.ls * (qb1 * A.dz1w[k] - qb0 * A.dz1w[k - 1]);
+                const bool upos = !(wz < 0.0);
                 double ga, gb, gc, gd, R;
-                if (k == nz - 1) {
-                    R = face_vjp(Qz(k - 1), Qz(k), Qz(k + 1), 0.0, upos, 2, false, Fb * wf, ga, gb, gc, gd);
-                } else {
-                    // the first face reads x(0) as x(1)
-                    R = face_vjp(Qz(k == 1 ? 1 : k - 1), Qz(k), Qz(k + 1), Qz(k + 2), upos, 0, k >= 3 && k <= nz - 3,
-                                 Fb * wf, ga, gb, gc, gd);
-                }
+                if (k == nz - 1) R = face_vjp(z0, z1, z2, 0.0, upos, 2, false, Fb * wz, ga, gb, gc, gd);
+                else R = face_vjp(z0, z1, z2, z3, upos, 0, k >= 3 && k <= nz - 3, Fb * wz, ga, gb, gc, gd);
+                // the first face reads x(0) as x(1)
                 if (k == 1) acc1 += ga;
                 else acc0 += ga;
                 acc1 += gb;
@@ -276,13 +285,21 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
                 acc3 += gd;
                 if (valid && k >= k0 && k <= k1) A.wb[col + (size_t)k * plane] += Fb * R;
             }
-            finalize(k - 1, acc0);
+            finalize(k - 1, acc0, qbm);
             acc0 = acc1;
             acc1 = acc2;
             acc2 = acc3;
             acc3 = 0.0;
+            z0 = z1;
+            z1 = z2;
+            z2 = z3;
+            z3 = zn;
+            qbm = qb0;
+            qb0 = qb1;
+            qb1 = qbn1;
+            wz = wn;
         }
-        if (ke == nz) finalize(nz, acc0);
+        if (ke == nz) finalize(nz, acc0, qbm);
     }
 }
 
