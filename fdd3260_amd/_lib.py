@@ -20,7 +20,7 @@ ERR_UNSUPPORTED = -3
 ERR_NOMEM = -4
 
 BC_PER, BC_NNF, BC_OPE = 0, 1, 2
-WENO, PPM, MUSCL, WENO5 = 0, 1, 2, 3
+WENO, PPM, MUSCL, WENO5, QUICK, LW = 0, 1, 2, 3, 4, 5
 LIM_MINMOD, LIM_MC, LIM_VL, LIM_FROMM = 0, 1, 2, 3
 WENO_BOTTOM_X0_IS_X1, WENO_BOTTOM_CLOSURE = 0, 1
 INT_EULER, INT_HEUN, INT_SSPRK3 = 1, 2, 3
@@ -28,6 +28,8 @@ INT_EULER, INT_HEUN, INT_SSPRK3 = 1, 2, 3
 DIR_X, DIR_Y, DIR_Z, DIR_ALL = 1, 2, 4, 7
 OVERWRITE = 8
 HOST_PTRS = 16
+UNLIMITED = 128
+SORDER1 = 256
 GRAPH = 32
 MOMENTUM = 64
 F64, F32 = 0, 1

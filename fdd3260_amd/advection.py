@@ -23,7 +23,8 @@ from . import _lib
 from ._lib import MimadvError
 from .grid import Grid
 
-SCHEMES = {"weno": _lib.WENO, "ppm": _lib.PPM, "muscl": _lib.MUSCL, "weno5": _lib.WENO5}
+SCHEMES = {"weno": _lib.WENO, "ppm": _lib.PPM, "muscl": _lib.MUSCL, "weno5": _lib.WENO5, "quick": _lib.QUICK,
+           "lw": _lib.LW}
 LIMITERS = {"minmod": _lib.LIM_MINMOD, "MC": _lib.LIM_MC, "VL": _lib.LIM_VL, "Fromm": _lib.LIM_FROMM}
 
 
@@ -153,8 +154,10 @@ class Advection:
         return v.value
 
     # -- scalar advection (advection.f90) -----------------------------------
-    def scalar_tendency(self, dt0, u, v, w, q_list, xs_list, adv=None, dirs=_lib.DIR_ALL, overwrite=False):
-        """xs_f = ((xs_f - X) - Y) - Z for every field (advection_x/y/z)."""
+    def scalar_tendency(self, dt0, u, v, w, q_list, xs_list, adv=None, dirs=_lib.DIR_ALL, overwrite=False,
+                        limit=True, sorder=2):
+        """xs_f = ((xs_f - X) - Y) - Z for every field (advection_x/y/z); limit and sorder select the line
+        kernel as the reference does (unlimited: QUICK or Lax-Wendroff; sorder 1: Lax-Wendroff)."""
         nf = len(q_list)
         if len(xs_list) != nf:
             raise MimadvError("q and xs lists differ in length")
@@ -165,7 +168,8 @@ class Advection:
         ad = None
         if adv is not None:
             ad = (C.c_int * nf)(*[1 if a else 0 for a in adv])
-        flags = int(dirs) | hflag | (_lib.OVERWRITE if overwrite else 0)
+        flags = (int(dirs) | hflag | (_lib.OVERWRITE if overwrite else 0) | (0 if limit else _lib.UNLIMITED)
+                 | (_lib.SORDER1 if sorder == 1 else 0))
         rc = self.lib.mimadv_scalar_tendency(self._h, float(dt0), pu, pv, pw, nf, pq, px, ad, flags)
         self._check(rc, "mimadv_scalar_tendency")
 
@@ -211,7 +215,7 @@ class Advection:
         self._check(rc, "mimadv_totalav")
         return mean.value, wsum.value
 
-    def advance(self, integrator, nsteps, dt, u, v, w, fields, adv=None):
+    def advance(self, integrator, nsteps, dt, u, v, w, fields, adv=None, limit=True):
         ptrs, hflag = self._ptrs([u, v, w] + list(fields), writable=fields)
         if hflag:
             raise MimadvError("advance works on device tensors")
@@ -220,7 +224,7 @@ class Advection:
         if adv is not None:
             ad = (C.c_int * nf)(*[1 if a else 0 for a in adv])
         rc = self.lib.mimadv_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1], ptrs[2],
-                                     nf, _lib.ptr_array(ptrs[3:]), ad, 0)
+                                     nf, _lib.ptr_array(ptrs[3:]), ad, 0 if limit else _lib.UNLIMITED)
         self._check(rc, "mimadv_advance")
 
     # -- batched mixed-precision path ---------------------------------------

@@ -70,13 +70,26 @@ int hipfail(mimadv_ctx *c, hipError_t e, const char *what)
 int check_opts(mimadv_ctx *c, const mimadv_opts *o)
 {
     if (!o) return fail(c, MIMADV_ERR_ARG, "opts is NULL");
-    if (o->scheme < MIMADV_WENO || o->scheme > MIMADV_WENO5) return fail(c, MIMADV_ERR_ARG, "unknown scheme");
+    if (o->scheme < MIMADV_WENO || o->scheme > MIMADV_LW) return fail(c, MIMADV_ERR_ARG, "unknown scheme");
     if (o->muscl_limiter < 0 || o->muscl_limiter > 3) return fail(c, MIMADV_ERR_ARG, "unknown MUSCL limiter");
     if (o->weno_bottom != 0 && o->weno_bottom != 1) return fail(c, MIMADV_ERR_ARG, "unknown weno_bottom convention");
     if (o->mom_ord != 2 && o->mom_ord != 4) return fail(c, MIMADV_ERR_ARG, "mom_ord must be 2 or 4");
     if (!(o->lim_tol > 0.0)) return fail(c, MIMADV_ERR_ARG, "lim_tol must be positive");
     return MIMADV_OK;
 }
+
+// The line kernel advection_{x,y,z} runs (advection.f90:580-620): limit =
+// .false. gives unlimited QUICK for 'quick' and unlimited Lax-Wendroff for
+// every other scheme; sorder = 1 gives limited Lax-Wendroff.  Kernel codes
+// 6 and 7 are the unlimited QUICK and Lax-Wendroff.
+int line_scheme(int scheme, unsigned flags)
+{
+    if (flags & MIMADV_UNLIMITED) return scheme == MIMADV_QUICK ? 6 : 7;
+    if (flags & MIMADV_SORDER1) return MIMADV_LW;
+    return scheme;
+}
+
+bool uses_dt0(int line) { return line == MIMADV_PPM || line == MIMADV_LW || line == 7; }
 
 void apply_opts(mimadv_ctx *c, const mimadv_opts *o)
 {
@@ -126,12 +139,13 @@ int check_ptr_dev(mimadv_ctx *c, const void *p, const char *name)
 // without its checks and timing): per stage, tendency of every field,
 // stage combine and periodic ghost refresh in one launch per field.
 int advance_steps(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
-                  const double *w, int nf, double *const *q, const int *adv_form)
+                  const double *w, int nf, double *const *q, const int *adv_form, unsigned flags = 0)
 {
     int rc = ensure_work(c, (size_t)2 * nf);
     if (rc) return rc;
     size_t n3 = c->n3();
     SchemeParams sp = c->sp;
+    sp.scheme = line_scheme(c->sp.scheme, flags | (integrator == MIMADV_INT_EULER ? MIMADV_SORDER1 : 0u));
     sp.dt0 = dt;
     // the scratch arrays start from the state so their ghosts are defined
     for (int f = 0; f < nf; ++f) {
@@ -139,6 +153,29 @@ int advance_steps(mimadv_ctx *c, int integrator, int nsteps, double dt, const do
         HIPCHK(c, hipMemcpyAsync(c->work + (2 * f + 1) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     }
     for (int s = 0; s < nsteps; ++s) {
+        if (integrator == MIMADV_INT_EULER) {
+            // sorder = 1 (integ.h): x = xold + dt*a, into w1, then back
+            for (int f = 0; f < nf; ++f) {
+                double *w1 = c->work + (2 * f) * n3, *w2 = c->work + (2 * f + 1) * n3;
+                ScalarArgs a{};
+                a.u = u;
+                a.v = v;
+                a.w = w;
+                a.adv = adv_form ? adv_form[f] : 0;
+                a.dirmask = 7;
+                a.fill = 1;
+                a.epi = EPI_HEUN;
+                a.tend = w2;
+                a.q0 = q[f];
+                a.q = q[f];
+                a.qn = w1;
+                a.b_lm = 0.0;
+                a.adt = dt;
+                HIPCHK(c, launch_scalar_fp64(c->g, sp, a, c->stream));
+                HIPCHK(c, hipMemcpyAsync(q[f], w1, n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            }
+            continue;
+        }
         int nst = integrator == MIMADV_INT_HEUN ? 2 : 3;
         for (int st = 1; st <= nst; ++st) {
             for (int f = 0; f < nf; ++f) {
@@ -313,11 +350,13 @@ int mimadv_scalar_tendency(mimadv_ctx *c, double dt0, const double *u, const dou
     if ((dirs & MIMADV_DIR_X) && !u) return fail(c, MIMADV_ERR_ARG, "u is NULL");
     if ((dirs & MIMADV_DIR_Y) && three_d && !v) return fail(c, MIMADV_ERR_ARG, "v is NULL");
     if ((dirs & MIMADV_DIR_Z) && !w) return fail(c, MIMADV_ERR_ARG, "w is NULL");
-    if (c->sp.scheme == MIMADV_PPM && !(std::isfinite(dt0))) return fail(c, MIMADV_ERR_ARG, "PPM needs a finite dt0");
+    const int line = line_scheme(c->sp.scheme, flags);
+    if (uses_dt0(line) && !(std::isfinite(dt0))) return fail(c, MIMADV_ERR_ARG, "PPM and Lax-Wendroff need a finite dt0");
     for (int f = 0; f < nf; ++f) {
         if (!q[f] || !xs[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
     }
     SchemeParams sp = c->sp;
+    sp.scheme = line;
     sp.dt0 = dt0;
     size_t n3 = c->n3();
     size_t bytes = n3 * sizeof(double);
@@ -489,9 +528,8 @@ int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const d
 {
     if (!c) return MIMADV_ERR_ARG;
     if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "mimadv_advance works on device arrays");
-    if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
-        return fail(c, MIMADV_ERR_UNSUPPORTED,
-                    "only Heun (sorder=2) and SSP-RK3 are supported; sorder=1 runs Lax-Wendroff in the reference");
+    if (integrator != MIMADV_INT_EULER && integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "integrator must be MIMADV_INT_EULER, MIMADV_INT_HEUN or MIMADV_INT_SSPRK3");
     for (int b = 0; b < 4; ++b)
         if (c->g.bcl[b] != MIMADV_BC_PER && !(c->g.ny == 1 && b >= 2))
             return fail(c, MIMADV_ERR_UNSUPPORTED, "mimadv_advance needs periodic lateral boundaries");
@@ -503,7 +541,7 @@ int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const d
     for (int f = 0; f < nf; ++f)
         if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    int rc = advance_steps(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form);
+    int rc = advance_steps(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form, flags & MIMADV_UNLIMITED);
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));

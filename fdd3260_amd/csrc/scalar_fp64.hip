@@ -56,6 +56,17 @@ __device__ __forceinline__ double h_contrib(int i, int n, double d, bool adv, in
         return ppm_h(i, n, d1, adv, bc_hi, sp, Q, U);
     } else if constexpr (SCHEME == 2) {
         return muscl_h(i, d, d1, adv, sp, Q, U);
+    } else if constexpr (SCHEME >= 4) {
+        // QUICK (4), Lax-Wendroff (5) and their unlimited forms (6, 7)
+        auto face = [&](int f) -> double {
+            if constexpr (SCHEME == 4) return quick_hface<false>(f, sp, Q, U);
+            else if constexpr (SCHEME == 6) return quick_hface<true>(f, sp, Q, U);
+            else if constexpr (SCHEME == 5) return lw_hface<false>(f, d1, sp, Q, U);
+            else return lw_hface<true>(f, d1, sp, Q, U);
+        };
+        double t = (face(i) - face(i - 1)) * d1;
+        if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
+        return t;
     } else {
         double t = (weno5_hface(i, Q, U) - weno5_hface(i - 1, Q, U)) * d1;
         if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
@@ -94,6 +105,14 @@ __device__ __forceinline__ double z_contrib(int k, int nz, bool adv, const GridD
         return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
     } else if constexpr (SCHEME == 2) {
         auto face = [&](int f) -> double { return muscl_zface(f, nz, zm, sp, bc51, bc61, Q, W); };
+        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
+    } else if constexpr (SCHEME >= 4) {
+        auto face = [&](int f) -> double {
+            if constexpr (SCHEME == 4) return quick_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
+            else if constexpr (SCHEME == 6) return quick_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
+            else if constexpr (SCHEME == 5) return lw_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
+            else return lw_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
+        };
         return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
     } else {
         auto face = [&](int f) -> double { return weno3_zface(f, nz, sp.weno_bottom, Q, W); };
@@ -233,6 +252,10 @@ hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const Sc
     case 1: return launch_scheme<1>(g, sp, a, s);
     case 2: return launch_scheme<2>(g, sp, a, s);
     case 3: return launch_scheme<3>(g, sp, a, s);
+    case 4: return launch_scheme<4>(g, sp, a, s);
+    case 5: return launch_scheme<5>(g, sp, a, s);
+    case 6: return launch_scheme<6>(g, sp, a, s);
+    case 7: return launch_scheme<7>(g, sp, a, s);
     default: return hipErrorInvalidValue;
     }
 }

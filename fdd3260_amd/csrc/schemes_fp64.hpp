@@ -248,6 +248,185 @@ __device__ __forceinline__ double muscl_zface(int f, int nz, const ZMetrics &zm,
 }
 
 // ---------------------------------------------------------------------------
+// QUICK (advq_quick.f90) and its unlimited form (advq_quick_nolim.f90)
+// ---------------------------------------------------------------------------
+
+// limiter_quick (advq_quick.f90:313-352); phic: x in x/y, x/dz in z
+__device__ __forceinline__ void quick_limiter(double lt, double phic, double ffip1, double ffip2, double ffim1,
+                                              double &phil, double &phir)
+{
+    const double eps = 1.e-18, c1 = 1.0 / 3.0;
+    double rp1 = 1.0, rp2 = 1.0, rp11 = 1.0, rp21 = 1.0;
+    if (fabs(ffim1) > lt * fabs(phic) + eps) rp1 = ffip1 / ffim1;
+    if (fabs(ffip2) > lt * fabs(phic) + eps) rp2 = ffip1 / ffip2;
+    if (fabs(ffip1) > lt * fabs(phic) + eps) {
+        rp11 = ffim1 / ffip1;
+        rp21 = ffip2 / ffip1;
+    }
+    phil = rp1 * (1.0 - c1 * (1.0 - rp11));
+    phir = rp2 * (1.0 - c1 * (1.0 - rp21));
+    if (fabs(ffim1) > lt * fabs(phic) + eps || fabs(ffip2) > lt * fabs(phic) + eps) {
+        phil = dmaxd(0.0, dmind(2.0 * rp1, dmind(phil, 2.0)));
+        phir = dmaxd(0.0, dmind(2.0 * rp2, dmind(phir, 2.0)));
+    }
+}
+
+__device__ __forceinline__ double half_upwind(double uu, double fl, double fr)
+{
+    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
+}
+
+// face flux i+1/2, x/y (advq_quick.f90:75-88, advq_quick_nolim.f90:77-90)
+template <bool NL, class QF, class UF>
+__device__ __forceinline__ double quick_hface(int i, const SchemeParams &sp, QF Q, UF U)
+{
+    double uu = U(i + 1), xx = Q(i), xp = Q(i + 1);
+    double ffip1 = xp - xx;
+    double ffip2 = Q(i + 2) - xp;
+    double ffim1 = xx - Q(i - 1);
+    double phil, phir, fl, fr;
+    if constexpr (NL) {
+        const double cc = 1.0 / 4.0;
+        phil = ffip1 - cc * (ffip1 - ffim1);
+        phir = ffip1 + cc * (ffip2 - ffip1);
+        fl = xx + 0.5 * phil;
+        fr = xp - 0.5 * phir;
+    } else {
+        quick_limiter(sp.lim_tol, xx, ffip1, ffip2, ffim1, phil, phir);
+        fl = xx + 0.5 * phil * ffim1;
+        fr = xp - 0.5 * phir * ffip2;
+    }
+    return half_upwind(uu, fl, fr);
+}
+
+// bottom/core/top slopes of a z face (per metre), with the periodic-floor
+// and periodic-lid substitutions of advsz_quick/advsz_lw
+template <class QF>
+__device__ __forceinline__ void z_slopes(int f, int nz, const ZMetrics &zm, double bc51, double bc61, QF Q,
+                                         double ffip1, double &ffip2, double &ffim1)
+{
+    const double *dz1 = zm.dz1;
+    if (f == 1) {
+        ffip2 = (Q(3) - Q(2)) * dz1[1];
+        ffim1 = bc51 * (Q(nz - 4) - Q(nz - 5)) * dz1[nz - 6] + (1.0 - bc51) * ffip1;
+    } else if (f == nz - 1) {
+        ffim1 = (Q(f) - Q(f - 1)) * dz1[f - 2];
+        ffip2 = bc61 * (Q(6) - Q(5)) * dz1[4] + (1.0 - bc61) * ffip1;
+    } else {
+        ffip2 = (Q(f + 2) - Q(f + 1)) * dz1[f];
+        ffim1 = (Q(f) - Q(f - 1)) * dz1[f - 2];
+    }
+}
+
+// z face flux k+1/2 (advq_quick.f90:233-284, advq_quick_nolim.f90:239-286)
+template <bool NL, class QF, class WF>
+__device__ __forceinline__ double quick_zface(int f, int nz, const ZMetrics &zm, const SchemeParams &sp,
+                                              double bc51, double bc61, QF Q, WF W)
+{
+    double xx = Q(f), ww = W(f + 1), xp = Q(f + 1);
+    double ffip1 = (xp - xx) * zm.dz1[f - 1];
+    double ffip2, ffim1;
+    z_slopes(f, nz, zm, bc51, bc61, Q, ffip1, ffip2, ffim1);
+    const double m = zm.dz1w[f - 1];
+    double phil, phir, fl, fr;
+    if constexpr (NL) {
+        const double cc = 1.0 / 4.0;
+        phil = ffip1 - cc * (ffip1 - ffim1);
+        phir = ffip1 + cc * (ffip2 - ffip1);
+        fl = xx + 0.5 * phil / m;
+        fr = xp - 0.5 * phir / m;
+    } else {
+        quick_limiter(sp.lim_tol, xx / zm.dz, ffip1, ffip2, ffim1, phil, phir);
+        fl = xx + 0.5 * phil * ffim1 / m;
+        fr = xp - 0.5 * phir * ffip2 / m;
+    }
+    return half_upwind(ww, fl, fr);
+}
+
+// ---------------------------------------------------------------------------
+// Lax-Wendroff (advq_lw.f90, limiter 'MC': the module's lwlimiter, which
+// nothing in the model changes) and its unlimited form (advq_lw_nolim.f90)
+// ---------------------------------------------------------------------------
+
+// limiter_lw, 'MC' (advq_lw.f90:349-373,389-391)
+__device__ __forceinline__ void lw_limiter(double lt, double xx, double ffip1, double ffip2, double ffim1,
+                                           double &phir, double &phil)
+{
+    const double eps = 1.e-18;
+    double rp1 = 1.0, rp2 = 1.0;
+    if (fabs(ffim1) > lt * fabs(xx) + eps) rp1 = ffip1 / ffim1;
+    if (fabs(ffip2) > lt * fabs(xx) + eps) rp2 = ffip1 / ffip2;
+    phil = dmaxd(dmind(dmind(2.0 * rp1, 0.5 * (rp1 + 1.0)), 2.0), 0.0);
+    phir = dmaxd(dmind(dmind(2.0 * rp2, 0.5 * (rp2 + 1.0)), 2.0), 0.0);
+}
+
+__device__ __forceinline__ double lw_flux(double uu, double fl, double fr)
+{
+    return 0.5 * uu * (fl * (sgn1(uu) + 1.0) - fr * (sgn1(uu) - 1.0));
+}
+
+// face flux i+1/2, x/y (advq_lw.f90:81-98, advq_lw_nolim.f90:79-88);
+// Courant number from the full step dt0
+template <bool NL, class QF, class UF>
+__device__ __forceinline__ double lw_hface(int i, double d1, const SchemeParams &sp, QF Q, UF U)
+{
+    const double eps = 1.e-18;
+    double uu = U(i + 1), xx = Q(i), xp = Q(i + 1);
+    double ffip1 = xp - xx;
+    double mu = fabs(uu) * sp.dt0 * d1;
+    double fl, fr;
+    if constexpr (NL) {
+        fl = xx + 0.5 * ffip1 * (1.0 - mu);
+        fr = xp - 0.5 * ffip1 * (1.0 - mu);
+        return half_upwind(uu, fl, fr);
+    }
+    double ffim1 = xx - Q(i - 1);
+    double ffip2 = Q(i + 2) - xp;
+    double lt = sp.lim_tol;
+    if (fabs(ffim1) > lt * fabs(xx) + eps || fabs(ffip2) > lt * fabs(xx) + eps) {
+        double phir, phil;
+        lw_limiter(lt, xx, ffip1, ffip2, ffim1, phir, phil);
+        fl = xx + 0.5 * phil * ffim1 * (1.0 - mu);
+        fr = xp - 0.5 * phir * ffip2 * (1.0 - mu);
+    } else {
+        fl = xx + 0.5 * ffip1 * (1.0 - mu);
+        fr = xp - 0.5 * ffip1 * (1.0 - mu);
+    }
+    return lw_flux(uu, fl, fr);
+}
+
+// z face flux k+1/2 (advq_lw.f90:245-316, advq_lw_nolim.f90:226-235)
+template <bool NL, class QF, class WF>
+__device__ __forceinline__ double lw_zface(int f, int nz, const ZMetrics &zm, const SchemeParams &sp, double bc51,
+                                           double bc61, QF Q, WF W)
+{
+    const double eps = 1.e-18;
+    double xx = Q(f), ww = W(f + 1), xp = Q(f + 1);
+    double mu = fabs(ww) * sp.dt0 * zm.dz1[f - 1];
+    double ffip1 = (xp - xx) * zm.dz1[f - 1];
+    const double m = zm.dz1w[f - 1];
+    double fl, fr;
+    if constexpr (NL) {
+        fl = xx + 0.5 * ffip1 * (1.0 - mu) / m;
+        fr = xp - 0.5 * ffip1 * (1.0 - mu) / m;
+        return half_upwind(ww, fl, fr);
+    }
+    double ffip2, ffim1;
+    z_slopes(f, nz, zm, bc51, bc61, Q, ffip1, ffip2, ffim1);
+    double lt = sp.lim_tol, dz = zm.dz;
+    if (dz * fabs(ffim1) > lt * fabs(xx) + eps || dz * fabs(ffip2) > lt * fabs(xx) + eps) {
+        double phir, phil;
+        lw_limiter(lt, xx, ffip1, ffip2, ffim1, phir, phil);
+        fl = xx + 0.5 * phil * ffim1 * (1.0 - mu) / m;
+        fr = xp - 0.5 * phir * ffip2 * (1.0 - mu) / m;
+    } else {
+        fl = xx + 0.5 * ffip1 * (1.0 - mu) / m;
+        fr = xp - 0.5 * ffip1 * (1.0 - mu) / m;
+    }
+    return lw_flux(ww, fl, fr);
+}
+
+// ---------------------------------------------------------------------------
 // PPM (advq_ppm.f90)
 // ---------------------------------------------------------------------------
 

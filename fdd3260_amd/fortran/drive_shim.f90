@@ -3,7 +3,7 @@
 ! does, on fields read from a stream file, and writes the results.
 !
 ! usage: drive_shim <input> <output>
-!   input  (stream, native endian): nx ny nz scheme limiter (int32),
+!   input  (stream, native endian): nx ny nz scheme limiter limit sorder (int32),
 !          dx dy dz dt0 (real64), fdz0 dzw dz1w dz1 (nz each),
 !          u v w q mu mv mw (nx*ny*nz each, x fastest)
 !   output (stream): xs after advection_x; xs after advection_x, _y, _z in
@@ -12,7 +12,8 @@
 program drive_shim
   use mimadv_advection
   implicit none
-  integer :: nx, ny, nz, scheme, limiter, ios
+  integer :: nx, ny, nz, scheme, limiter, ilimit, sorder, ios
+  logical :: lim
   real(8) :: dx, dy, dz, dt0
   real(8), allocatable :: fdz0(:), dzw(:), dz1w(:), dz1(:)
   real(8), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), q(:,:,:), mu(:,:,:), mv(:,:,:), mw(:,:,:)
@@ -26,7 +27,7 @@ program drive_shim
   call get_command_argument(2, fout)
   open(10, file=trim(fin), access='stream', form='unformatted', status='old', iostat=ios)
   if (ios /= 0) error stop 2
-  read(10) nx, ny, nz, scheme, limiter
+  read(10) nx, ny, nz, scheme, limiter, ilimit, sorder
   read(10) dx, dy, dz, dt0
   allocate(fdz0(nz), dzw(nz), dz1w(nz), dz1(nz))
   read(10) fdz0, dzw, dz1w, dz1
@@ -39,9 +40,14 @@ program drive_shim
     sname = 'weno'
   case (1)
     sname = 'ppm'
+  case (4)
+    sname = 'quick'
+  case (5)
+    sname = 'lw'
   case default
     sname = 'muscl'
   end select
+  lim = ilimit /= 0
   select case (limiter)
   case (1)
     lname = 'MC'
@@ -54,15 +60,15 @@ program drive_shim
   end select
   bcl = (/ 'per', 'per', 'per', 'per', 'nnf', 'nnf' /)
   call mimadv_init(nx, ny, nz, dx, dy, dz, fdz0, dzw, dz1w, dz1, bcl, sname, lname, 'PPM', 1.d-9, 4, &
-                   .true., .true.)
+                   .true., .true., sorder=sorder, limit=lim)
 
   allocate(xs(nx,ny,nz), dummy(nx,ny,nz), ua(nx,ny,nz), va(nx,ny,nz), wa(nx,ny,nz))
   open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
   xs = 0.d0
-  call advection_x('q1', dt0, dummy, q, xs, u, .true.)
+  call advection_x('q1', dt0, dummy, q, xs, u, lim)
   write(11) xs
-  call advection_y('q1', dt0, dummy, q, xs, v, .true.)
-  call advection_z('q1', dt0, dummy, q, xs, w, .true.)
+  call advection_y('q1', dt0, dummy, q, xs, v, lim)
+  call advection_z('q1', dt0, dummy, q, xs, w, lim)
   write(11) xs
   xs = 0.d0
   call advection_ac_arrays(dt0, u, v, w, q, xs)

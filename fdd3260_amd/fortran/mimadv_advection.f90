@@ -29,6 +29,7 @@ module mimadv_advection
 
   integer(c_int), parameter :: MIMADV_DIR_X = 1, MIMADV_DIR_Y = 2, MIMADV_DIR_Z = 4
   integer(c_int), parameter :: MIMADV_HOST_PTRS = 16
+  integer(c_int), parameter :: MIMADV_UNLIMITED = 128, MIMADV_SORDER1 = 256
 
   type, bind(c) :: mimadv_grid_t
     integer(c_int) :: nx, ny, nz
@@ -106,6 +107,8 @@ module mimadv_advection
   type(c_ptr), save :: ctx = c_null_ptr
   integer, save :: g_nx = 0, g_ny = 0, g_nz = 0
   logical, save :: g_conservative = .false.
+  logical, save :: g_lim_def = .true.
+  integer, save :: g_sorder = 2
   real(c_double), allocatable, target, save :: m_fdz0(:), m_dzw(:), m_dz1w(:), m_dz1(:)
 
 contains
@@ -139,14 +142,19 @@ contains
   end function
 
   ! One-time set-up (advection_init plus the module globals it relies on).
+  ! sorder (default 2) and limit (default .true., the lim_def that
+  ! advection_ac uses when ilim is absent) are the namelist values of
+  ! cm_num (mimica.f90:74).
   subroutine mimadv_init(nx, ny, nz, dx, dy, dz, fdz0, dzw, dz1w, dz1, bcl, scal_adv, muscllimiter, &
-                         ppmlimiter, lim_tol, mom_ord, lavisc, conservative)
+                         ppmlimiter, lim_tol, mom_ord, lavisc, conservative, sorder, limit)
     integer, intent(in) :: nx, ny, nz, mom_ord
     real(8), intent(in) :: dx, dy, dz, lim_tol
     real(8), intent(in) :: fdz0(nz), dzw(nz), dz1w(nz), dz1(nz)
     character(len=3), intent(in) :: bcl(6)
     character(len=*), intent(in) :: scal_adv, muscllimiter, ppmlimiter
     logical, intent(in) :: lavisc, conservative
+    integer, optional, intent(in) :: sorder
+    logical, optional, intent(in) :: limit
     type(mimadv_grid_t) :: g
     type(mimadv_opts_t) :: o
     integer :: b
@@ -166,9 +174,11 @@ contains
       o%scheme = 1
     case ('muscl')
       o%scheme = 2
+    case ('quick')
+      o%scheme = 4
     case default
-      write(*, '(a,a)') 'mimadv: scal_adv not provided by the GPU operator: ', trim(scal_adv)
-      error stop 1
+      ! every other value runs Lax-Wendroff (advection.f90:614-616)
+      o%scheme = 5
     end select
     select case (trim(muscllimiter))
     case ('MC')
@@ -187,6 +197,10 @@ contains
     o%lavisc = merge(1, 0, lavisc)
     g_nx = nx; g_ny = ny; g_nz = nz
     g_conservative = conservative
+    g_sorder = 2
+    if (present(sorder)) g_sorder = sorder
+    g_lim_def = .true.
+    if (present(limit)) g_lim_def = limit
     call check(c_create(g, o, ctx), 'mimadv_create')
   end subroutine
 
@@ -195,6 +209,14 @@ contains
     ctx = c_null_ptr
     if (allocated(m_fdz0)) deallocate(m_fdz0, m_dzw, m_dz1w, m_dz1)
   end subroutine
+
+  ! the line-kernel selection of advection_{x,y,z} (advection.f90:580-620)
+  integer(c_int) function line_flags(lim)
+    logical, intent(in) :: lim
+    line_flags = 0
+    if (.not. lim) line_flags = MIMADV_UNLIMITED
+    if (g_sorder == 1) line_flags = ior(line_flags, MIMADV_SORDER1)
+  end function
 
   subroutine scalar_dir(dirflag, dtl, xtmp, xs, vel, lim, adv)
     integer(c_int), intent(in) :: dirflag
@@ -206,10 +228,6 @@ contains
     type(c_ptr), target :: qp(1), xp(1)
     integer(c_int), target :: af(1)
     type(c_ptr) :: pu, pv, pw
-    if (.not. lim) then
-      write(*, '(a)') 'mimadv: unlimited advection (LW/QUICK without limiter) is not provided by the GPU operator'
-      error stop 1
-    endif
     af(1) = merge(1, 0, .not. g_conservative)
     if (present(adv)) af(1) = merge(1, 0, adv)
     qp(1) = c_loc(xtmp)
@@ -219,7 +237,7 @@ contains
     if (dirflag == MIMADV_DIR_Y) pv = c_loc(vel)
     if (dirflag == MIMADV_DIR_Z) pw = c_loc(vel)
     call check(c_scalar_tendency(ctx, dtl, pu, pv, pw, 1_c_int, c_loc(qp), c_loc(xp), c_loc(af), &
-                                 ior(dirflag, MIMADV_HOST_PTRS)), 'advection')
+                                 ior(ior(dirflag, MIMADV_HOST_PTRS), line_flags(lim))), 'advection')
   end subroutine
 
   ! advection_x (advection.f90:556): xs = xs - (x flux divergence of xtmp)
@@ -268,17 +286,14 @@ contains
     logical, optional, intent(in) :: ilim
     type(c_ptr), target :: qp(1), xp(1)
     integer(c_int), target :: af(1)
-    if (present(ilim)) then
-      if (.not. ilim) then
-        write(*, '(a)') 'mimadv: unlimited advection is not provided by the GPU operator'
-        error stop 1
-      endif
-    endif
+    logical :: lim
+    lim = g_lim_def
+    if (present(ilim)) lim = ilim
     af(1) = merge(1, 0, .not. g_conservative)
     qp(1) = c_loc(xx)
     xp(1) = c_loc(xs)
     call check(c_scalar_tendency(ctx, dt0, c_loc(u), c_loc(v), c_loc(w), 1_c_int, c_loc(qp), c_loc(xp), &
-                                 c_loc(af), ior(7_c_int, MIMADV_HOST_PTRS)), 'advection_ac')
+                                 c_loc(af), ior(ior(7_c_int, MIMADV_HOST_PTRS), line_flags(lim))), 'advection_ac')
   end subroutine
 
   ! qnadv_uvw (windadv.f90:46) on arrays: (ua, va, wa) OVERWRITTEN with

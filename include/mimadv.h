@@ -50,6 +50,8 @@ extern "C" {
 #define MIMADV_PPM 1   /* 'ppm' */
 #define MIMADV_MUSCL 2 /* 'muscl' */
 #define MIMADV_WENO5 3 /* WENO5-JS in x/y (no reference counterpart) */
+#define MIMADV_QUICK 4 /* 'quick' (advq_quick.f90) */
+#define MIMADV_LW 5    /* Lax-Wendroff with the MC limiter (advq_lw.f90; also any other scal_adv) */
 
 /* ---- MUSCL limiters: muscllimiter (advq_muscl.f90:49) ---- */
 #define MIMADV_LIM_MINMOD 0
@@ -73,6 +75,12 @@ extern "C" {
 #define MIMADV_DIR_ALL 7u
 #define MIMADV_OVERWRITE 8u    /* xs = -div q (xs is not read) */
 #define MIMADV_HOST_PTRS 16u   /* array arguments are host memory */
+/* limit = .false. of advection_{x,y,z}: unlimited QUICK for MIMADV_QUICK,
+ * unlimited Lax-Wendroff for every other scheme (advection.f90:580-592,
+ * advq_quick_nolim.f90, advq_lw_nolim.f90) */
+#define MIMADV_UNLIMITED 128u
+/* sorder = 1: limited Lax-Wendroff whatever the scheme (advection.f90:598-601) */
+#define MIMADV_SORDER1 256u
 
 /* Grid: extents, spacings and vertical metrics (define_grid.f90:179-187).
  * Metric arrays are host arrays of length nz; they are copied at create. */
@@ -89,7 +97,7 @@ typedef struct mimadv_grid {
 /* Options: the namelist values the reference reads as module globals
  * (shared_data.f90:251-349, advq_muscl.f90:49, advq_ppm.f90:49). */
 typedef struct mimadv_opts {
-    int scheme;         /* MIMADV_WENO / PPM / MUSCL / WENO5 */
+    int scheme;         /* MIMADV_WENO / PPM / MUSCL / WENO5 / QUICK / LW */
     int muscl_limiter;  /* MIMADV_LIM_* (reference default minmod) */
     int ppm_limiter;    /* 1 = 'PPM' selective limiter (default), 0 = none */
     double lim_tol;     /* reference default 1e-9 */
@@ -121,9 +129,12 @@ int mimadv_set_opts(mimadv_ctx *ctx, const mimadv_opts *opts);
  * in advection_x/y/z (advection.f90:556-798; `xs = xs - xx` at :625, :706,
  * :788), restricted to the directions in flags & MIMADV_DIR_ALL.  adv_form[f]
  * (NULL = all 0) selects the advective form (`adv` argument of advs*_*).
- * dt0 is the full step; only PPM uses it (advq_ppm.f90:81-82).  For ny == 1
- * the y direction is skipped as in a 2D build.  Ghost values are read as
- * given by the caller; ghost tendencies are left untouched.
+ * dt0 is the full step; PPM and Lax-Wendroff use it (advq_ppm.f90:81-82,
+ * advq_lw.f90:88).  flags may add MIMADV_UNLIMITED (limit = .false.) and
+ * MIMADV_SORDER1 (sorder = 1), which select the line kernel as
+ * advection_{x,y,z} does.  For ny == 1 the y direction is skipped as in a
+ * 2D build.  Ghost values are read as given by the caller; ghost tendencies
+ * are left untouched.
  * Replaces: advection_x/y/z (advection.f90:556,637,719), advection_ac
  * (advection.f90:801, xs += xsx+xsy+xsz), and the loops of advect /
  * qnadv_scal_{x,y,z} (advection.f90:84-553). */
@@ -157,8 +168,11 @@ int mimadv_totalav(mimadv_ctx *ctx, const double *x, double *mean, double *dzw_s
  * loop (time_step.f90:238-284 with integrate_all :337-444 and the ghost
  * refresh statebc :405-413): per stage, tendency of every field, stage
  * combine and periodic ghost refresh, all device-resident.  integrator is
- * MIMADV_INT_*.  Lateral boundaries must be periodic.  q holds the fields
- * (in place); dt is also the PPM dt0. */
+ * MIMADV_INT_*: MIMADV_INT_EULER (sorder = 1, which runs the limited
+ * Lax-Wendroff scheme, or the unlimited one with MIMADV_UNLIMITED),
+ * MIMADV_INT_HEUN or MIMADV_INT_SSPRK3.  Lateral boundaries must be
+ * periodic.  q holds the fields (in place); dt is also the dt0 of PPM and
+ * Lax-Wendroff.  flags may contain MIMADV_UNLIMITED. */
 int mimadv_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
                    const double *v, const double *w, int nf, double *const *q, const int *adv_form,
                    unsigned flags);

@@ -591,6 +591,275 @@ static void muscl_z(const orc_grid *g, const orc_opts *o, const double *x, const
 }
 
 /* ======================================================================
+ * QUICK: advq_quick.f90 (limited) and advq_quick_nolim.f90 (unlimited)
+ * ====================================================================== */
+
+/* limiter_quick (advq_quick.f90:313-352); phic is the value the guards
+ * compare with (x in x/y, x/dz in z) */
+static void quick_limiter(double lt, double phic, double ffip1, double ffip2, double ffim1, double *phil,
+                          double *phir)
+{
+    const double eps = 1.e-18, c1 = 1.0 / 3.0;
+    double rp1 = 1.0, rp2 = 1.0, rp11 = 1.0, rp21 = 1.0;
+    if (fabs(ffim1) > lt * fabs(phic) + eps) rp1 = ffip1 / ffim1;
+    if (fabs(ffip2) > lt * fabs(phic) + eps) rp2 = ffip1 / ffip2;
+    if (fabs(ffip1) > lt * fabs(phic) + eps) {
+        rp11 = ffim1 / ffip1;
+        rp21 = ffip2 / ffip1;
+    }
+    *phil = rp1 * (1.0 - c1 * (1.0 - rp11));
+    *phir = rp2 * (1.0 - c1 * (1.0 - rp21));
+    if (fabs(ffim1) > lt * fabs(phic) + eps || fabs(ffip2) > lt * fabs(phic) + eps) {
+        *phil = dmax(0.0, dmin(2.0 * rp1, dmin(*phil, 2.0)));
+        *phir = dmax(0.0, dmin(2.0 * rp2, dmin(*phir, 2.0)));
+    }
+}
+
+static double half_upwind(double uu, double fl, double fr)
+{
+    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
+}
+
+/* divergence of line fluxes flux(i1-1..i2) into xa(i1..i2), plus the
+ * advective-form correction (the common tail of the x/y line kernels) */
+static void h_divergence(int n, double d1, const double *x, const double *u, const double *flux, double *xa,
+                         int adv)
+{
+    int i1 = 4, i2 = n - 2;
+    double fold = L(flux, i1 - 1);
+    for (int i = i1; i <= i2; ++i) {
+        L(xa, i) = L(xa, i) + (L(flux, i) - fold) * d1;
+        fold = L(flux, i);
+    }
+    if (adv) {
+        double uold = L(u, i2 + 1);
+        for (int i = i2; i >= i1; --i) {
+            L(xa, i) = L(xa, i) - L(x, i) * (uold - L(u, i)) * d1;
+            uold = L(u, i);
+        }
+    }
+}
+
+/* z divergence of flux(1..nz-1) with the 'nnf' closures (the common tail of
+ * the QUICK/LW/MUSCL z kernels, e.g. advq_quick.f90:286-305) */
+static void z_divergence(const orc_grid *g, const double *x, const double *w, const double *flux, double *xa,
+                         int adv)
+{
+    int nz = g->nz;
+    const double *dz1w = g->dz1w;
+    double fold = L(flux, 1);
+    for (int k = 2; k <= nz - 1; ++k) {
+        L(xa, k) = L(xa, k) + (L(flux, k) - fold) * L(dz1w, k);
+        fold = L(flux, k);
+    }
+    if (adv) {
+        double wold = L(w, nz);
+        for (int k = nz - 1; k >= 2; --k) {
+            L(xa, k) = L(xa, k) - L(x, k) * (wold - L(w, k)) * L(dz1w, k);
+            wold = L(w, k);
+        }
+    }
+    if (g->bcl[4] == ORC_BC_NNF) L(xa, 1) = L(xa, 1) + (L(flux, 1) - 0.0) * L(dz1w, 1);
+    if (g->bcl[4] == ORC_BC_NNF && adv) L(xa, 1) = L(xa, 1) - L(x, 1) * (L(w, 2) - 0.0) * L(dz1w, 1);
+    if (g->bcl[5] == ORC_BC_NNF) L(xa, nz) = L(xa, nz) + (0.0 - L(flux, nz - 1)) * L(dz1w, nz);
+    if (g->bcl[5] == ORC_BC_NNF && adv) L(xa, nz) = L(xa, nz) - L(x, nz) * (0.0 - L(w, nz)) * L(dz1w, nz);
+}
+
+/* advsx_quick / advsy_quick (advq_quick.f90:54-122) and the unlimited
+ * advsx_quick_nl (advq_quick_nolim.f90:54-111) */
+static void quick_h(const orc_opts *o, int unlimited, int n, double d, const double *x, const double *u,
+                    double *xa, int adv)
+{
+    int i1 = 4, i2 = n - 2;
+    double d1 = 1.0 / d;
+    double *flux = (double *)calloc((size_t)n + 1, sizeof(double));
+    for (int i = i1 - 1; i <= i2; ++i) {
+        double uu = L(u, i + 1), xx = L(x, i);
+        double ffip1 = L(x, i + 1) - xx;
+        double ffip2 = L(x, i + 2) - L(x, i + 1);
+        double ffim1 = xx - L(x, i - 1);
+        double phil, phir, fl, fr;
+        if (unlimited) {
+            const double cc = 1.0 / 4.0;
+            phil = ffip1 - cc * (ffip1 - ffim1);
+            phir = ffip1 + cc * (ffip2 - ffip1);
+            fl = xx + 0.5 * phil;
+            fr = L(x, i + 1) - 0.5 * phir;
+        } else {
+            quick_limiter(o->lim_tol, xx, ffip1, ffip2, ffim1, &phil, &phir);
+            fl = xx + 0.5 * phil * ffim1;
+            fr = L(x, i + 1) - 0.5 * phir * ffip2;
+        }
+        L(flux, i) = half_upwind(uu, fl, fr);
+    }
+    h_divergence(n, d1, x, u, flux, xa, adv);
+    free(flux);
+}
+
+/* advsz_quick (advq_quick.f90:197-311), advsz_quick_nl
+ * (advq_quick_nolim.f90:212-318): per-metre slopes, face offsets by
+ * dzw(k) on both sides, guards against x/dz */
+static void quick_z(const orc_grid *g, const orc_opts *o, int unlimited, const double *x, const double *w,
+                    double *xa, int adv)
+{
+    int nz = g->nz;
+    double dz = g->dz;
+    const double *dz1 = g->dz1, *dz1w = g->dz1w;
+    double bc51 = (g->bcl[4] == ORC_BC_PER) ? 1.0 : 0.0;
+    double bc61 = (g->bcl[5] == ORC_BC_PER) ? 1.0 : 0.0;
+    double *flux = (double *)calloc((size_t)nz + 1, sizeof(double));
+    for (int k = 1; k <= nz - 1; ++k) {
+        double xx = L(x, k), ww = L(w, k + 1);
+        double ffip1 = (L(x, k + 1) - xx) * L(dz1, k);
+        double ffip2, ffim1;
+        if (k == 1) {
+            ffip2 = (L(x, 3) - L(x, 2)) * L(dz1, 2);
+            ffim1 = bc51 * (L(x, nz - 4) - L(x, nz - 5)) * L(dz1, nz - 5) + (1.0 - bc51) * ffip1;
+        } else if (k == nz - 1) {
+            ffim1 = (xx - L(x, k - 1)) * L(dz1, k - 1);
+            ffip2 = bc61 * (L(x, 6) - L(x, 5)) * L(dz1, 5) + (1.0 - bc61) * ffip1;
+        } else {
+            ffip2 = (L(x, k + 2) - L(x, k + 1)) * L(dz1, k + 1);
+            ffim1 = (xx - L(x, k - 1)) * L(dz1, k - 1);
+        }
+        double phil, phir, fl, fr;
+        if (unlimited) {
+            const double cc = 1.0 / 4.0;
+            phil = ffip1 - cc * (ffip1 - ffim1);
+            phir = ffip1 + cc * (ffip2 - ffip1);
+            fl = xx + 0.5 * phil / L(dz1w, k);
+            fr = L(x, k + 1) - 0.5 * phir / L(dz1w, k);
+        } else {
+            quick_limiter(o->lim_tol, xx / dz, ffip1, ffip2, ffim1, &phil, &phir);
+            fl = xx + 0.5 * phil * ffim1 / L(dz1w, k);
+            fr = L(x, k + 1) - 0.5 * phir * ffip2 / L(dz1w, k);
+        }
+        L(flux, k) = half_upwind(ww, fl, fr);
+    }
+    z_divergence(g, x, w, flux, xa, adv);
+    free(flux);
+}
+
+/* ======================================================================
+ * Lax-Wendroff: advq_lw.f90 (flux-limited, lwlimiter = 'MC', the module's
+ * value, which nothing in the model changes) and advq_lw_nolim.f90
+ * ====================================================================== */
+
+/* limiter_lw with lwlimiter = 'MC' (advq_lw.f90:349-373,389-391) */
+static void lw_limiter(double lt, double xx, double ffip1, double ffip2, double ffim1, double *phir,
+                       double *phil)
+{
+    const double eps = 1.e-18;
+    double rp1 = 1.0, rp2 = 1.0;
+    if (fabs(ffim1) > lt * fabs(xx) + eps) rp1 = ffip1 / ffim1;
+    if (fabs(ffip2) > lt * fabs(xx) + eps) rp2 = ffip1 / ffip2;
+    *phil = dmax(dmin(dmin(2.0 * rp1, 0.5 * (rp1 + 1.0)), 2.0), 0.0);
+    *phir = dmax(dmin(dmin(2.0 * rp2, 0.5 * (rp2 + 1.0)), 2.0), 0.0);
+}
+
+static double lw_flux(double uu, double fl, double fr)
+{
+    return 0.5 * uu * (fl * (sgn1(uu) + 1.0) - fr * (sgn1(uu) - 1.0));
+}
+
+/* advsx_lw / advsy_lw (advq_lw.f90:57-126), advsx_lw_nl
+ * (advq_lw_nolim.f90:54-111); Courant number from the full step dt0 */
+static void lw_h(const orc_opts *o, int unlimited, int n, double d, const double *x, const double *u,
+                 double *xa, int adv)
+{
+    const double eps = 1.e-18;
+    int i1 = 4, i2 = n - 2;
+    double d1 = 1.0 / d, lt = o->lim_tol;
+    double *flux = (double *)calloc((size_t)n + 1, sizeof(double));
+    for (int i = i1 - 1; i <= i2; ++i) {
+        double uu = L(u, i + 1), xx = L(x, i);
+        double ffip1 = L(x, i + 1) - xx;
+        double mu = fabs(uu) * o->dt0 * d1;
+        double fl, fr;
+        if (unlimited) {
+            fl = xx + 0.5 * ffip1 * (1.0 - mu);
+            fr = L(x, i + 1) - 0.5 * ffip1 * (1.0 - mu);
+            L(flux, i) = half_upwind(uu, fl, fr);
+            continue;
+        }
+        double ffim1 = xx - L(x, i - 1);
+        double ffip2 = L(x, i + 2) - L(x, i + 1);
+        if (fabs(ffim1) > lt * fabs(xx) + eps || fabs(ffip2) > lt * fabs(xx) + eps) {
+            double phir, phil;
+            lw_limiter(lt, xx, ffip1, ffip2, ffim1, &phir, &phil);
+            fl = xx + 0.5 * phil * ffim1 * (1.0 - mu);
+            fr = L(x, i + 1) - 0.5 * phir * ffip2 * (1.0 - mu);
+        } else {
+            fl = xx + 0.5 * ffip1 * (1.0 - mu);
+            fr = L(x, i + 1) - 0.5 * ffip1 * (1.0 - mu);
+        }
+        L(flux, i) = lw_flux(uu, fl, fr);
+    }
+    h_divergence(n, d1, x, u, flux, xa, adv);
+    free(flux);
+}
+
+/* advsz_lw (advq_lw.f90:210-332), advsz_lw_nl (advq_lw_nolim.f90:188-265) */
+static void lw_z(const orc_grid *g, const orc_opts *o, int unlimited, const double *x, const double *w,
+                 double *xa, int adv)
+{
+    const double eps = 1.e-18;
+    int nz = g->nz;
+    double dz = g->dz, lt = o->lim_tol;
+    const double *dz1 = g->dz1, *dz1w = g->dz1w;
+    double bc51 = (g->bcl[4] == ORC_BC_PER) ? 1.0 : 0.0;
+    double bc61 = (g->bcl[5] == ORC_BC_PER) ? 1.0 : 0.0;
+    double *flux = (double *)calloc((size_t)nz + 1, sizeof(double));
+    for (int k = 1; k <= nz - 1; ++k) {
+        double xx = L(x, k), ww = L(w, k + 1);
+        double mu = fabs(ww) * o->dt0 * L(dz1, k);
+        double ffip1 = (L(x, k + 1) - xx) * L(dz1, k);
+        double fl, fr;
+        if (unlimited) {
+            fl = xx + 0.5 * ffip1 * (1.0 - mu) / L(dz1w, k);
+            fr = L(x, k + 1) - 0.5 * ffip1 * (1.0 - mu) / L(dz1w, k);
+            L(flux, k) = half_upwind(ww, fl, fr);
+            continue;
+        }
+        double ffip2, ffim1;
+        if (k == 1) {
+            ffip2 = (L(x, 3) - L(x, 2)) * L(dz1, 2);
+            ffim1 = bc51 * (L(x, nz - 4) - L(x, nz - 5)) * L(dz1, nz - 5) + (1.0 - bc51) * ffip1;
+        } else if (k == nz - 1) {
+            ffim1 = (xx - L(x, k - 1)) * L(dz1, k - 1);
+            ffip2 = bc61 * (L(x, 6) - L(x, 5)) * L(dz1, 5) + (1.0 - bc61) * ffip1;
+        } else {
+            ffim1 = (xx - L(x, k - 1)) * L(dz1, k - 1);
+            ffip2 = (L(x, k + 2) - L(x, k + 1)) * L(dz1, k + 1);
+        }
+        if (dz * fabs(ffim1) > lt * fabs(xx) + eps || dz * fabs(ffip2) > lt * fabs(xx) + eps) {
+            double phir, phil;
+            lw_limiter(lt, xx, ffip1, ffip2, ffim1, &phir, &phil);
+            fl = xx + 0.5 * phil * ffim1 * (1.0 - mu) / L(dz1w, k);
+            fr = L(x, k + 1) - 0.5 * phir * ffip2 * (1.0 - mu) / L(dz1w, k);
+        } else {
+            fl = xx + 0.5 * ffip1 * (1.0 - mu) / L(dz1w, k);
+            fr = L(x, k + 1) - 0.5 * ffip1 * (1.0 - mu) / L(dz1w, k);
+        }
+        L(flux, k) = lw_flux(ww, fl, fr);
+    }
+    z_divergence(g, x, w, flux, xa, adv);
+    free(flux);
+}
+
+/* The line kernel advection_{x,y,z} selects (advection.f90:580-620,
+ * :661-701, :743-782): unlimited ('limit' false) runs unlimited QUICK for
+ * scal_adv = 'quick' and unlimited Lax-Wendroff for every other value;
+ * limited with sorder = 1 runs the limited Lax-Wendroff whatever scal_adv
+ * is; otherwise scal_adv's own scheme. */
+int orc_line_scheme(int scheme, int unlimited, int sorder)
+{
+    if (unlimited) return scheme == ORC_QUICK ? ORC_QUICK_NL : ORC_LW_NL;
+    if (sorder == 1) return ORC_LW;
+    return scheme;
+}
+
+/* ======================================================================
  * Extension (not in the reference): WENO5-JS on x/y lines, periodic data
  * with the reference ghost layout (3 west, 2 east).  Jiang & Shu (1996):
  * candidate stencils (2,-7,11)/6, (-1,5,2)/6, (2,5,-1)/6; linear weights
@@ -655,6 +924,10 @@ void orc_advs_h(const orc_opts *o, int n, double d, int bc_lo, int bc_hi,
     case ORC_PPM: ppm_h(o, n, d, bc_hi, x, u, xa, adv); break;
     case ORC_MUSCL: muscl_h(o, n, d, x, u, xa, adv); break;
     case ORC_WENO5: weno5_h(n, d, x, u, xa, adv); break;
+    case ORC_QUICK: quick_h(o, 0, n, d, x, u, xa, adv); break;
+    case ORC_QUICK_NL: quick_h(o, 1, n, d, x, u, xa, adv); break;
+    case ORC_LW: lw_h(o, 0, n, d, x, u, xa, adv); break;
+    case ORC_LW_NL: lw_h(o, 1, n, d, x, u, xa, adv); break;
     default: break;
     }
 }
@@ -669,6 +942,10 @@ void orc_advs_z(const orc_grid *g, const orc_opts *o, const double *x, double x0
         break;
     case ORC_PPM: ppm_z(g, o, x, w, xa, adv); break;
     case ORC_MUSCL: muscl_z(g, o, x, w, xa, adv); break;
+    case ORC_QUICK: quick_z(g, o, 0, x, w, xa, adv); break;
+    case ORC_QUICK_NL: quick_z(g, o, 1, x, w, xa, adv); break;
+    case ORC_LW: lw_z(g, o, 0, x, w, xa, adv); break;
+    case ORC_LW_NL: lw_z(g, o, 1, x, w, xa, adv); break;
     default: break;
     }
 }

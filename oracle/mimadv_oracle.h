@@ -41,6 +41,9 @@ extern "C" {
 
 enum { ORC_BC_PER = 0, ORC_BC_NNF = 1, ORC_BC_OPE = 2 };
 enum { ORC_WENO = 0, ORC_PPM = 1, ORC_MUSCL = 2, ORC_WENO5 = 3 };
+/* advq_quick.f90, advq_lw.f90 and their unlimited forms (advq_quick_nolim.f90,
+ * advq_lw_nolim.f90) */
+enum { ORC_QUICK = 4, ORC_LW = 5, ORC_QUICK_NL = 6, ORC_LW_NL = 7 };
 enum { ORC_LIM_MINMOD = 0, ORC_LIM_MC = 1, ORC_LIM_VL = 2, ORC_LIM_FROMM = 3 };
 /* weno_bottom: what advsz_weno reads for x(0) in its first loop pass. */
 enum { ORC_WENO_BOTTOM_X0_IS_X1 = 0, ORC_WENO_BOTTOM_CLOSURE = 1 };
@@ -56,7 +59,7 @@ typedef struct {
 } orc_grid;
 
 typedef struct {
-    int scheme;              /* ORC_WENO / ORC_PPM / ORC_MUSCL / ORC_WENO5 */
+    int scheme;              /* ORC_WENO .. ORC_LW_NL: the line kernel to run */
     int muscl_limiter;       /* ORC_LIM_* (reference default: minmod) */
     int ppm_limiter;         /* 1: ppmlimiter='PPM' (reference default), 0: none */
     double lim_tol;          /* reference default 1e-9 */
@@ -65,6 +68,10 @@ typedef struct {
     int mom_ord;             /* 2 or 4 */
     int lavisc;              /* artificial viscosity on momentum */
 } orc_opts;
+
+/* The line kernel advection_{x,y,z} runs for scal_adv = scheme (ORC_WENO,
+ * ORC_PPM, ORC_MUSCL, ORC_QUICK, ORC_LW), limit = !unlimited and sorder. */
+int orc_line_scheme(int scheme, int unlimited, int sorder);
 
 /* ---- 1D line kernels (pointers to element 1 of the line, stride 1) ---- */
 void orc_advs_h(const orc_opts *o, int n, double d, int bc_lo, int bc_hi,

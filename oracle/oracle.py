@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 BC = {"per": 0, "nnf": 1, "ope": 2}
-SCHEMES = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
+SCHEMES = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5, "quick_nl": 6, "lw_nl": 7}
 LIMITERS = {"minmod": 0, "MC": 1, "VL": 2, "Fromm": 3}
 
 
@@ -74,6 +74,8 @@ def load():
     lib.orc_get_advective.argtypes = [G, dp, dp, d, d, dp, dp, dp, dp, dp, dp]
     lib.orc_get_conserved.argtypes = [G, dp, dp, i]
     lib.orc_set_threads.argtypes = [i]
+    lib.orc_line_scheme.argtypes = [i, i, i]
+    lib.orc_line_scheme.restype = i
     lib.orc_set_threads.restype = i
     _lib = lib
     return lib

@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from oracle.oracle import Oracle
+from oracle.oracle import SCHEMES, Oracle, load
 
 from helpers import case3d, relmax
 
@@ -17,8 +17,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "fdd3260_amd", "fortran", "drive_shim")
 
 
-@pytest.mark.parametrize("scheme,code,lim", [("weno", 0, "minmod"), ("ppm", 1, "minmod"), ("muscl", 2, "MC")])
-def test_fortran_shim_matches_oracle(cuda, tmp_path, scheme, code, lim):
+@pytest.mark.parametrize("scheme,code,lim,limit,sorder", [
+    ("weno", 0, "minmod", 1, 2), ("ppm", 1, "minmod", 1, 2), ("muscl", 2, "MC", 1, 2), ("quick", 4, "minmod", 1, 2),
+    ("lw", 5, "minmod", 1, 2), ("quick", 4, "minmod", 0, 2), ("muscl", 2, "MC", 0, 2), ("weno", 0, "minmod", 1, 1)])
+def test_fortran_shim_matches_oracle(cuda, tmp_path, scheme, code, lim, limit, sorder):
+    """limit = .false. and sorder = 1 select the line kernel as the reference
+    does: unlimited QUICK for 'quick', unlimited Lax-Wendroff otherwise,
+    limited Lax-Wendroff for sorder = 1."""
     if not os.path.exists(DRIVER):
         pytest.fail("the Fortran driver was not built (flang missing at build time)")
     g, u, v, w, q = case3d(seed=41)
@@ -28,14 +33,15 @@ def test_fortran_shim_matches_oracle(cuda, tmp_path, scheme, code, lim):
     limcode = {"minmod": 0, "MC": 1, "VL": 2, "Fromm": 3}[lim]
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
-        np.array([g.nx, g.ny, g.nz, code, limcode], dtype=np.int32).tofile(f)
+        np.array([g.nx, g.ny, g.nz, code, limcode, limit, sorder], dtype=np.int32).tofile(f)
         np.array([g.dx, g.dy, g.dz, dt0]).tofile(f)
         for a in (g.fdz, g.dzw, g.dz1w, g.dz1, u, v, w, q, mu, mv, mw):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
     r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     out = np.fromfile(fout, dtype=np.float64).reshape(6, *g.shape)
-    ora = Oracle(g, scal_adv=scheme, muscllimiter=lim)
+    line = load().orc_line_scheme(SCHEMES[scheme], 0 if limit else 1, sorder)
+    ora = Oracle(g, scal_adv={v: k for k, v in SCHEMES.items()}[line], muscllimiter=lim)
     x_only = np.zeros(g.shape)
     ora.scalar_tendency(dt0, u, v, w, q, x_only, False, 1)
     full = np.zeros(g.shape)

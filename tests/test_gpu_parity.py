@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 SCHEMES = [("weno", "minmod"), ("ppm", "minmod"), ("muscl", "minmod"), ("muscl", "MC"), ("muscl", "VL"),
-           ("muscl", "Fromm"), ("weno5", "minmod")]
+           ("muscl", "Fromm"), ("weno5", "minmod"), ("quick", "minmod"), ("lw", "minmod")]
 
 
 def _dev(cuda, *arrs):
@@ -47,7 +47,7 @@ def test_scalar_tendency(cuda, scheme, lim, adv, dims):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl", "quick", "lw"])
 @pytest.mark.parametrize("bc", ["nnf", "ope", "per"])
 def test_vertical_boundaries(cuda, scheme, bc):
     print(f"case vertical {scheme} {bc}", flush=True)
@@ -180,3 +180,53 @@ def test_cone_2d_500_steps(cuda, scheme, integrator):
     I = (slice(None), slice(None), slice(3, g.nx - 2))
     m0, m1 = float(q[I].sum()), float(got[I].sum())
     assert abs(m1 - m0) <= 1e-12 * abs(m0)
+
+
+@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl", "quick", "lw"])
+@pytest.mark.parametrize("limit,sorder", [(False, 2), (True, 1), (False, 1)])
+@pytest.mark.parametrize("dims", ["3d", "2d"])
+def test_line_kernel_selection(cuda, scheme, limit, sorder, dims):
+    """limit = .false. and sorder = 1 (MIMADV_UNLIMITED, MIMADV_SORDER1) run
+    the line kernel advection_{x,y,z} selects: unlimited QUICK for 'quick',
+    unlimited Lax-Wendroff otherwise, limited Lax-Wendroff for sorder = 1;
+    bit-identical to the oracle's kernel of that choice."""
+    from oracle.oracle import SCHEMES as OS, load
+    print(f"case line_kernel {scheme} limit={limit} sorder={sorder} {dims}", flush=True)
+    g, u, v, w, q = case3d() if dims == "3d" else case2d()
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    line = load().orc_line_scheme(OS[scheme], 0 if limit else 1, sorder)
+    ora = Oracle(g, scal_adv={c: n for n, c in OS.items()}[line])
+    ref = np.zeros(g.shape)
+    ora.scalar_tendency(dt, u, v, w, q, ref)
+    with Advection(g, scal_adv=scheme) as a:
+        du, dv, dw, dq, dx = _dev(cuda, u, v, w, q, np.zeros(g.shape))
+        a.scalar_tendency(dt, du, dv, dw, [dq], [dx], limit=limit, sorder=sorder)
+        a.stream_sync()
+        assert relmax(dx.cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("scheme,limit", [("weno", True), ("quick", False), ("muscl", False)])
+def test_euler_runs_lax_wendroff(cuda, scheme, limit):
+    """MIMADV_INT_EULER (sorder = 1): x = xold + dt*a per step (integ.h) with
+    the limited Lax-Wendroff scheme whatever scal_adv is, or the unlimited
+    line kernel with limit = .false.; four steps against the oracle."""
+    from fdd3260_amd import _lib
+    from oracle.oracle import SCHEMES as OS, load
+    print(f"case euler {scheme} limit={limit}", flush=True)
+    g, u, v, w, q = case3d()
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    line = load().orc_line_scheme(OS[scheme], 0 if limit else 1, 1)
+    ora = Oracle(g, scal_adv={c: n for n, c in OS.items()}[line])
+    ref = q.copy()
+    for _ in range(4):
+        a = np.zeros(g.shape)
+        ora.scalar_tendency(dt, u, v, w, ref, a)
+        nxt = np.empty_like(ref)
+        ora.integ_scal(1, 1, dt, ref, a, np.zeros(g.shape), nxt)
+        ora.fill_scalar(nxt)
+        ref = nxt
+    with Advection(g, scal_adv=scheme) as a:
+        du, dv, dw, dq = _dev(cuda, u, v, w, q)
+        a.advance(_lib.INT_EULER, 4, dt, du, dv, dw, [dq], limit=limit)
+        got = dq.cpu().numpy()
+    assert relmax(got, ref) <= TOL

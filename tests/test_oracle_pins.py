@@ -14,6 +14,11 @@ by hand from the reference text, each chosen so that a plausible slip fails:
 * MUSCL (advq_muscl.f90): centred flux when unlimited, textbook
   minmod/MC/van Leer/Fromm limiters, TVD property, vertical metrics on a
   stretched grid (exact face height for linear data, the dz1w(k) quirk);
+* QUICK and Lax-Wendroff (advq_quick*.f90, advq_lw*.f90): Leonard's
+  quadratic face value (unlimited), the kappa = 1/3 face and extremum
+  clipping (limited), the textbook LW face and the exact one-cell shift at
+  Courant number 1, TVD with the MC limiter, vertical metrics, and the
+  scheme dispatch of advection_{x,y,z};
 * flux form: conservation of sum(dzw q); advective form: constants stay;
 * momentum (windadv.f90): exactness for linear fields, 4th/2nd-order
   convergence, direction-order independence;
@@ -327,10 +332,133 @@ def test_muscl_vertical_metrics_linear_in_height():
             assert F[k - 1] == pytest.approx(w[0] * exp, rel=1e-12)
 
 
+# ------------------------------------------------------ QUICK and LW -----
+
+def _face_value(ora, x, f, uf, d=50.0, dt0=0.0):
+    return _face_flux(ora, x, f, uf, d, dt0) / uf
+
+
+def test_quick_unlimited_interpolates_quadratics():
+    """Unlimited QUICK (advq_quick_nolim.f90) is Leonard's face value
+    -x(i-1)/8 + 3x(i)/4 + 3x(i+1)/8 (u > 0; mirrored for u < 0): exact for
+    point values of a quadratic."""
+    n = 30
+    i = np.arange(1, n + 1, dtype=float)
+    q = lambda t: 2.0 + 0.3 * t - 0.02 * t * t
+    x = q(i)
+    ora = _line_oracle("quick_nl")
+    for f in range(5, n - 3):
+        for uf in (1.3, -0.7):
+            assert _face_value(ora, x, f, uf) == pytest.approx(q(f + 0.5), rel=1e-13)
+
+
+def test_quick_limited_smooth_is_third_order_and_clips_extrema():
+    """Limited QUICK (advq_quick.f90): on smooth monotone data the limiter
+    gives phi(r) r = (2/3) r + 1/3 in units of the upwind slope, i.e. the
+    kappa = 1/3 face value -x(i-1)/6 + 5x(i)/6 + x(i+1)/3 (u > 0) or
+    x(i)/3 + 5x(i+1)/6 - x(i+2)/6 (u < 0); at an extremum phi = 0 and the
+    face takes the upwind cell value."""
+    n = 30
+    i = np.arange(1, n + 1, dtype=float)
+    x = 10.0 + 0.1 * i + 0.001 * i ** 2
+    ora = _line_oracle("quick")
+    X = lambda k: x[k - 1]
+    for f in range(5, n - 3):
+        exp_p = -X(f - 1) / 6 + 5 * X(f) / 6 + X(f + 1) / 3
+        exp_m = X(f) / 3 + 5 * X(f + 1) / 6 - X(f + 2) / 6
+        assert _face_value(ora, x, f, 1.1) == pytest.approx(exp_p, rel=1e-13)
+        assert _face_value(ora, x, f, -0.9) == pytest.approx(exp_m, rel=1e-13)
+    spike = np.ones(n)
+    spike[14] = 2.0  # cell 15
+    assert _face_value(ora, spike, 15, 1.0) == 2.0
+    assert _face_value(ora, spike, 14, -1.0) == 2.0
+
+
+@pytest.mark.parametrize("scheme", ["lw_nl", "lw"])
+def test_lax_wendroff_face_and_unit_courant_shift(scheme):
+    """Lax-Wendroff (advq_lw*.f90) face value x(i) + (1 - mu)(x(i+1) -
+    x(i))/2 for u > 0 on smooth data (the limiter is inactive: MC gives
+    phi(1) = 1), mu = |u| dt0/dx; and at Courant number 1 a forward-Euler
+    step moves periodic data by exactly one cell, limited or not."""
+    n, d = 30, 50.0
+    i = np.arange(1, n + 1, dtype=float)
+    x = 5.0 + 0.2 * i
+    ora = _line_oracle(scheme)
+    for f in range(5, n - 3):
+        for mu in (0.0, 0.3, 0.8):
+            uf = 1.5
+            dt0 = mu * d / uf
+            exp = x[f - 1] + 0.5 * (1.0 - mu) * (x[f] - x[f - 1])
+            assert _face_value(ora, x, f, uf, d, dt0) == pytest.approx(exp, rel=1e-13)
+    m = 40
+    rng = np.random.default_rng(8)
+    q = rng.uniform(0.0, 1.0, m)
+    for uc in (2.0, -2.0):
+        dt = d / abs(uc)
+        full = np.concatenate([q[-3:], q, q[:2]])
+        xa = ora.line_h(dt, full, np.full(m + 5, uc), d)
+        qn = q - dt * xa[3:m + 3]
+        assert np.allclose(qn, np.roll(q, 1 if uc > 0 else -1), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("sgn", [1.0, -1.0])
+def test_lax_wendroff_mc_tvd_forward_euler(sgn):
+    """Flux-limited Lax-Wendroff with the MC limiter and forward Euler at
+    Courant number 0.8 does not increase the total variation (Sweby's TVD
+    region: 0 <= phi(r) <= min(2r, 2))."""
+    m, d = 60, 1.0
+    rng = np.random.default_rng(9)
+    q = np.where(np.arange(m) % 19 < 9, 1.0, 0.2) + 0.05 * rng.standard_normal(m)
+    ora = _line_oracle("lw")
+    uc = sgn * 1.0
+    dt = 0.8 * d / abs(uc)
+    tv = lambda a: np.abs(np.diff(np.concatenate([a, a[:1]]))).sum()
+    for _ in range(12):
+        full = np.concatenate([q[-3:], q, q[:2]])
+        xa = ora.line_h(dt, full, np.full(m + 5, uc), d)
+        qn = q - dt * xa[3:m + 3]
+        assert tv(qn) <= tv(q) * (1 + 1e-12)
+        q = qn
+
+
+@pytest.mark.parametrize("scheme", ["quick", "quick_nl", "lw", "lw_nl"])
+def test_quick_lw_vertical_linear_in_height(scheme):
+    """Linear-in-height data on a stretched grid with dt0 = 0 and the
+    limiters inactive: per-metre slopes are 1, so the w > 0 face value is the
+    exact face height z_c(k) + dzw(k)/2; the w < 0 value x(k+1) - dzw(k)/2
+    keeps the lower cell's thickness (the /dz1w(k) on both sides of
+    advq_quick.f90:263-264, advq_lw.f90:293-294, as in MUSCL)."""
+    g = gm.stretched(10, 0, 16)
+    z = g.zc()
+    ora = Oracle(g, scal_adv=scheme, lim_tol=1e6)
+    for sgn in (1.0, -1.0):
+        w = np.full(g.nz, sgn * 0.7)
+        F = _zcol_flux(ora, g, z.copy(), w)
+        for k in range(2, g.nz - 1):
+            exp = z[k - 1] + 0.5 * g.dzw[k - 1] if sgn > 0 else z[k] - 0.5 * g.dzw[k - 1]
+            assert F[k - 1] == pytest.approx(w[0] * exp, rel=1e-12)
+
+
+def test_line_scheme_dispatch():
+    """advection_{x,y,z} (advection.f90:580-620): limit = .false. runs
+    unlimited QUICK for 'quick' and unlimited Lax-Wendroff for every other
+    scheme; sorder = 1 runs limited Lax-Wendroff whatever scal_adv is."""
+    from oracle.oracle import SCHEMES, load
+    lib = load()
+    S = SCHEMES
+    for name in ("weno", "ppm", "muscl", "quick", "lw", "weno5"):
+        sc = S[name]
+        assert lib.orc_line_scheme(sc, 1, 2) == (S["quick_nl"] if name == "quick" else S["lw_nl"])
+        assert lib.orc_line_scheme(sc, 1, 1) == (S["quick_nl"] if name == "quick" else S["lw_nl"])
+        assert lib.orc_line_scheme(sc, 0, 1) == S["lw"]
+        assert lib.orc_line_scheme(sc, 0, 2) == sc
+
+
 # ------------------------------------------------- operator invariants -----
 
 @pytest.mark.parametrize("scheme,lim,dirs", [("weno", "minmod", 7), ("muscl", "minmod", 7), ("muscl", "MC", 7),
-                                             ("muscl", "VL", 7), ("ppm", "minmod", 4)])
+                                             ("muscl", "VL", 7), ("ppm", "minmod", 4), ("quick", "minmod", 7),
+                                             ("quick_nl", "minmod", 7), ("lw", "minmod", 7), ("lw_nl", "minmod", 7)])
 def test_flux_form_conserves_mass(scheme, lim, dirs):
     """Flux form with periodic sides and no-flux floor/lid: sum(dzw * xs) is
     zero to round-off (PPM only in z: its tile-edge closures, F6, make x/y
@@ -344,7 +472,7 @@ def test_flux_form_conserves_mass(scheme, lim, dirs):
     assert abs(wx.sum()) <= 1e-13 * np.abs(wx).sum()
 
 
-@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl", "quick", "quick_nl", "lw", "lw_nl"])
 def test_advective_form_preserves_constant(scheme):
     g, u, v, w, _ = case3d()
     q = np.full(g.shape, 3.7)
