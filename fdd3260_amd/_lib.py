@@ -30,6 +30,7 @@ OVERWRITE = 8
 HOST_PTRS = 16
 UNLIMITED = 128
 SORDER1 = 256
+DREF_DEN0 = 512
 GRAPH = 32
 MOMENTUM = 64
 F64, F32 = 0, 1
@@ -61,6 +62,7 @@ EXPORTED = (
     "mimadv_get_momentum",
     "mimadv_get_advective",
     "mimadv_get_conserved",
+    "mimadv_advection_1d",
     "mimadv_multi_create",
     "mimadv_multi_destroy",
     "mimadv_multi_last_error",
@@ -176,6 +178,8 @@ def load() -> C.CDLL:
     lib.mimadv_get_advective.restype = i
     lib.mimadv_get_conserved.argtypes = [vp, vp, i, pp, i, u]
     lib.mimadv_get_conserved.restype = i
+    lib.mimadv_advection_1d.argtypes = [vp, C.POINTER(C.c_double), vp, i, pp, pp, u]
+    lib.mimadv_advection_1d.restype = i
     lib.mimadv_multi_create.argtypes = [i, pp, pp, pp, pp, ip, C.POINTER(pp), C.POINTER(vp)]
     lib.mimadv_multi_create.restype = i
     lib.mimadv_multi_destroy.argtypes = [vp]

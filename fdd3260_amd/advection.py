@@ -350,6 +350,23 @@ class Advection:
         rc = self.lib.mimadv_get_advective(self._h, float(u0shift), float(v0shift), *ptrs, hflag)
         self._check(rc, "mimadv_get_advective")
 
+    def advection_1d(self, w0, q_list, xs_list, dens=None, den0=False):
+        """advection_ac_1d: xs_f -= dref*w0*grad_1d(q_f/dref); dref = den0 profile (den0=True), the 3D
+        array dens, or 1.  w0 is a host profile of nz values."""
+        nf = len(q_list)
+        if len(xs_list) != nf:
+            raise MimadvError("q and xs lists differ in length")
+        w = np.ascontiguousarray(w0, dtype=np.float64)
+        if w.shape != (self.grid.nz,):
+            raise MimadvError(f"w0 must hold nz = {self.grid.nz} values")
+        arrs = list(q_list) + list(xs_list) + ([dens] if dens is not None else [])
+        ptrs, hflag = self._ptrs(arrs, writable=xs_list)
+        pd = ptrs[2 * nf] if dens is not None else None
+        flags = hflag | (_lib.DREF_DEN0 if den0 else 0)
+        rc = self.lib.mimadv_advection_1d(self._h, w.ctypes.data_as(C.POINTER(C.c_double)), pd, nf,
+                                          _lib.ptr_array(ptrs[:nf]), _lib.ptr_array(ptrs[nf:2 * nf]), flags)
+        self._check(rc, "mimadv_advection_1d")
+
     def get_conserved(self, dens, fields, inverse=False):
         """In place: q *= dens (inverse: q /= dens, get_scalars) for every field."""
         ptrs, hflag = self._ptrs([dens] + list(fields), writable=fields)

@@ -42,6 +42,7 @@ struct mimadv_ctx {
     size_t adj_elems = 0;
     // density weighting: den0, avden0 on the device (2*nz), or null
     double *d_den0 = nullptr;
+    double *d_w0 = nullptr;  // subsidence profile of mimadv_advection_1d (nz)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
@@ -301,6 +302,7 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->bwork) hipFree(c->bwork);
     if (c->adj) hipFree(c->adj);
     if (c->d_den0) hipFree(c->d_den0);
+    if (c->d_w0) hipFree(c->d_w0);
     if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -1175,6 +1177,46 @@ int mimadv_set_reference_density(mimadv_ctx *c, const double *den0, const double
     }
     HIPCHK(c, hipMemcpyAsync(c->d_den0, h.data(), 2 * (size_t)nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MIMADV_OK;
+}
+
+int mimadv_advection_1d(mimadv_ctx *c, const double *w0, const double *dens, int nf, const double *const *q,
+                        double *const *xs, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0) return fail(c, MIMADV_ERR_ARG, "nf < 0");
+    if (nf == 0) return MIMADV_OK;
+    if (!w0 || !q || !xs) return fail(c, MIMADV_ERR_ARG, "NULL w0 or field list");
+    for (int f = 0; f < nf; ++f)
+        if (!q[f] || !xs[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+    const bool den0 = (flags & MIMADV_DREF_DEN0) != 0;
+    if (den0 && !c->d_den0) return fail(c, MIMADV_ERR_ARG, "no reference density: call mimadv_set_reference_density");
+    const int nz = c->g.nz;
+    if (!c->d_w0 && hipMalloc(&c->d_w0, (size_t)nz * sizeof(double)) != hipSuccess)
+        return fail(c, MIMADV_ERR_NOMEM, "w0 allocation failed");
+    HIPCHK(c, hipMemcpyAsync(c->d_w0, w0, (size_t)nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const double *dprof = den0 ? c->d_den0 : nullptr;
+    const double *d3 = den0 ? nullptr : dens;
+    const bool host = (flags & MIMADV_HOST_PTRS) != 0;
+    const size_t bytes = c->n3() * sizeof(double);
+    if (host) {
+        int rc = ensure_stage(c, 3);
+        if (rc) return rc;
+        double *sq = c->stage, *sx = c->stage + c->n3(), *sd = c->stage + 2 * c->n3();
+        if (d3) {
+            HIPCHK(c, hipMemcpyAsync(sd, d3, bytes, hipMemcpyHostToDevice, c->stream));
+            d3 = sd;
+        }
+        for (int f = 0; f < nf; ++f) {
+            HIPCHK(c, hipMemcpyAsync(sq, q[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(sx, xs[f], bytes, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, launch_advection_1d(c->g, c->d_w0, dprof, d3, sq, sx, c->stream));
+            HIPCHK(c, hipMemcpyAsync(xs[f], sx, bytes, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MIMADV_OK;
+    }
+    for (int f = 0; f < nf; ++f) HIPCHK(c, launch_advection_1d(c->g, c->d_w0, dprof, d3, q[f], xs[f], c->stream));
     return MIMADV_OK;
 }
 

@@ -84,6 +84,10 @@ struct MomBatchArgs {
 };
 hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s);
 hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
+// xs -= dref*w0*grad_1d(q/dref) on interior columns; dref = d3 (3D) if not
+// NULL, else dprof (levels) if not NULL, else 1 (util.hip)
+hipError_t launch_advection_1d(const GridDev &g, const double *w0, const double *dprof, const double *d3,
+                               const double *q, double *xs, hipStream_t s);
 hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,
                              double *tend, double *x, hipStream_t s);
 hipError_t launch_totalav(const GridDev &g, const double *x, double *d_out2, hipStream_t s);

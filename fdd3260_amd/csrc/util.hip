@@ -275,4 +275,58 @@ hipError_t launch_field_scale(const GridDev &g, double *q, const double *d, int 
     return hipGetLastError();
 }
 
+namespace {
+
+// advection_ac_1d (advection.f90:993-1042) with grad_1d (gradients.f90:
+// 400-473): xs(i,j,k) -= dref(k)*w0(k)*gz(k), gz(k) the derivative at level
+// k of the quadratic through the values x/dref of levels k-1, k, k+1 (the
+// floor reads x(1) at distance dz*fdz0(1); gz(nz) = 0).  One thread per
+// interior column, marching up with a three-level register window.
+__global__ __launch_bounds__(64) void adv1d_kernel(GridDev g, const double *__restrict__ w0,
+                                                   const double *__restrict__ dprof, const double *__restrict__ d3,
+                                                   const double *__restrict__ q, double *__restrict__ xs)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool three_d = ny > 1;
+    const int i = 4 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = three_d ? 4 + (int)blockIdx.y : 1;
+    if (i > nx - 2) return;
+    const size_t plane = (size_t)nx * ny, col = (size_t)(i - 1) + (size_t)nx * (j - 1);
+    const double *fdz0 = g.zm.fdz0;
+    const double dz = g.zm.dz;
+    auto dref = [&](int k) { return d3 ? d3[col + (size_t)(k - 1) * plane] : (dprof ? dprof[k - 1] : 1.0); };
+    double dk = dref(1);
+    double x0 = q[col] / dk, xm = 0.0;
+    for (int k = 1; k <= nz; ++k) {
+        double gz = 0.0, dn = 0.0, xp = 0.0;
+        if (k <= nz - 1) {
+            dn = dref(k + 1);
+            xp = q[col + (size_t)k * plane] / dn;
+            const double deltad = 0.5 * dz * (fdz0[k] + fdz0[k - 1]);
+            const double deltau = k == 1 ? dz * fdz0[0] : 0.5 * dz * (fdz0[k - 1] + fdz0[k - 2]);
+            const double deltac = (deltad + deltau) * (deltad * deltad + deltau * deltau) -
+                                  (deltad - deltau) * (deltad * deltad - deltau * deltau);
+            const double ffip1 = xp - x0;
+            const double ffim1 = k == 1 ? 0.0 : x0 - xm;
+            gz = ((ffip1 + ffim1) * (deltad * deltad + deltau * deltau) -
+                  (ffip1 - ffim1) * (deltad * deltad - deltau * deltau)) / deltac;
+        }
+        const size_t id = col + (size_t)(k - 1) * plane;
+        xs[id] = xs[id] - dk * w0[k - 1] * gz;
+        xm = x0;
+        x0 = xp;
+        dk = dn;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_advection_1d(const GridDev &g, const double *w0, const double *dprof, const double *d3,
+                               const double *q, double *xs, hipStream_t s)
+{
+    const int mx = g.nx - 5, my = g.ny > 1 ? g.ny - 5 : 1;
+    hipLaunchKernelGGL(adv1d_kernel, dim3((mx + 63) / 64, my, 1), dim3(64), 0, s, g, w0, dprof, d3, q, xs);
+    return hipGetLastError();
+}
+
 }  // namespace mimadv

@@ -7,7 +7,9 @@
 !          dx dy dz dt0 (real64), fdz0 dzw dz1w dz1 (nz each),
 !          u v w q mu mv mw (nx*ny*nz each, x fastest)
 !   output (stream): xs after advection_x; xs after advection_x, _y, _z in
-!          sequence; xs of advection_ac_arrays; ua va wa of qnadv_uvw_arrays
+!          sequence; xs of advection_ac_arrays; ua va wa of qnadv_uvw_arrays;
+!          xs of advection_ac_1d_arrays with w0(k) = -1e-3*k and dref =
+!          den0(k) = 1.2 - 0.01*k (the conservative anelastic build)
 !
 program drive_shim
   use mimadv_advection
@@ -15,7 +17,8 @@ program drive_shim
   integer :: nx, ny, nz, scheme, limiter, ilimit, sorder, ios
   logical :: lim
   real(8) :: dx, dy, dz, dt0
-  real(8), allocatable :: fdz0(:), dzw(:), dz1w(:), dz1(:)
+  real(8), allocatable :: fdz0(:), dzw(:), dz1w(:), dz1(:), w0(:), den0(:)
+  integer :: k
   real(8), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), q(:,:,:), mu(:,:,:), mv(:,:,:), mw(:,:,:)
   real(8), allocatable :: xs(:,:,:), dummy(:,:,:), ua(:,:,:), va(:,:,:), wa(:,:,:)
   character(len=512) :: fin, fout
@@ -75,6 +78,15 @@ program drive_shim
   write(11) xs
   call qnadv_uvw_arrays(2, ua, va, wa, mu, mv, mw, u, v, w)
   write(11) ua, va, wa
+  allocate(w0(nz), den0(nz))
+  do k = 1, nz
+    w0(k) = -1.d-3 * k
+    den0(k) = 1.2d0 - 0.01d0 * k
+  enddo
+  call mimadv_set_density(den0, den0)
+  xs = 0.d0
+  call advection_ac_1d_arrays(w0, q, xs)
+  write(11) xs
   close(11)
   call mimadv_finalize()
 end program drive_shim

@@ -26,10 +26,12 @@ module mimadv_advection
 
   public :: mimadv_init, mimadv_finalize
   public :: advection_x, advection_y, advection_z, advection_ac_arrays, qnadv_uvw_arrays
+  public :: advection_ac_1d_arrays
+  public :: mimadv_set_density, get_momentum_arrays, get_velocities_arrays, get_advective_arrays, get_conserved_array
 
   integer(c_int), parameter :: MIMADV_DIR_X = 1, MIMADV_DIR_Y = 2, MIMADV_DIR_Z = 4
   integer(c_int), parameter :: MIMADV_HOST_PTRS = 16
-  integer(c_int), parameter :: MIMADV_UNLIMITED = 128, MIMADV_SORDER1 = 256
+  integer(c_int), parameter :: MIMADV_UNLIMITED = 128, MIMADV_SORDER1 = 256, MIMADV_DREF_DEN0 = 512
 
   type, bind(c) :: mimadv_grid_t
     integer(c_int) :: nx, ny, nz
@@ -93,6 +95,13 @@ module mimadv_advection
       type(c_ptr), value :: ctx
       real(c_double), value :: u0shift, v0shift
       type(c_ptr), value :: u, v, w, ua, va, wa
+      integer(c_int), value :: flags
+    end function
+    integer(c_int) function c_advection_1d(ctx, w0, dens, nf, q, xs, flags) bind(c, name='mimadv_advection_1d')
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx, w0, dens
+      integer(c_int), value :: nf
+      type(c_ptr), value :: q, xs
       integer(c_int), value :: flags
     end function
     integer(c_int) function c_get_conserved(ctx, dens, nf, q, inverse, flags) bind(c, name='mimadv_get_conserved')
@@ -306,6 +315,32 @@ contains
     call check(c_momentum_tendency(ctx, int(ntime, c_int), c_loc(mu), c_loc(mv), c_loc(mw), c_loc(ut), &
                                    c_loc(vt), c_loc(wt), c_loc(ua), c_loc(va), c_loc(wa), MIMADV_HOST_PTRS), &
                'qnadv_uvw')
+  end subroutine
+
+  ! advection_ac_1d (advection.f90:993-1042) on arrays: xs = xs -
+  ! dref*w0*grad_1d(xx/dref) on the interior columns.  dref is the 3D
+  ! density dens when present (pressure%dens of the compressible
+  ! CONSERVATIVE build), den0 of mimadv_set_density in a conservative build
+  ! (CONSERVATIVE + ANELASTIC), 1 otherwise.  advection_1d (:918-990) is this
+  ! call on state%es (and state%qt with with_lssub).
+  subroutine advection_ac_1d_arrays(w0, xx, xs, dens)
+    real(8), target, intent(in) :: w0(g_nz)
+    real(8), target, intent(in) :: xx(g_nx, g_ny, g_nz)
+    real(8), target, intent(inout) :: xs(g_nx, g_ny, g_nz)
+    real(8), target, optional, intent(in) :: dens(g_nx, g_ny, g_nz)
+    type(c_ptr), target :: qp(1), xp(1)
+    type(c_ptr) :: pd
+    integer(c_int) :: flags
+    qp(1) = c_loc(xx)
+    xp(1) = c_loc(xs)
+    pd = c_null_ptr
+    flags = MIMADV_HOST_PTRS
+    if (present(dens)) then
+      pd = c_loc(dens)
+    else if (g_conservative) then
+      flags = ior(flags, MIMADV_DREF_DEN0)
+    endif
+    call check(c_advection_1d(ctx, c_loc(w0), pd, 1_c_int, c_loc(qp), c_loc(xp), flags), 'advection_ac_1d')
   end subroutine
 
   ! Reference density profiles den0, avden0 (shared_data globals set in

@@ -201,6 +201,20 @@ int mimadv_get_advective(mimadv_ctx *ctx, double u0shift, double v0shift, const 
 int mimadv_get_conserved(mimadv_ctx *ctx, const double *dens, int nf, double *const *q, int inverse,
                          unsigned flags);
 
+/* ---- large-scale vertical advection (advection.f90:918-1042) ----
+ * advection_ac_1d: for nf fields, xs_f -= dref*w0*grad_1d(q_f/dref) on the
+ * interior columns, with grad_1d (gradients.f90:400-473) the derivative of
+ * the quadratic through levels k-1..k+1 (the floor reads x(1) at distance
+ * dz*fdz0(1); level nz gets no tendency).  w0 is a host array of nz values
+ * (the subsidence profile w0).  dref is the context's den0 profile with
+ * MIMADV_DREF_DEN0 (CONSERVATIVE + ANELASTIC build; needs
+ * mimadv_set_reference_density), else the 3D array dens when not NULL
+ * (pressure%dens of the CONSERVATIVE compressible build), else 1.
+ * advection_1d (:918-990) is the same call on es (and qt with with_lssub). */
+#define MIMADV_DREF_DEN0 512u
+int mimadv_advection_1d(mimadv_ctx *ctx, const double *w0, const double *dens, int nf, const double *const *q,
+                        double *const *xs, unsigned flags);
+
 /* ---- batched mixed-precision path (no reference counterpart: the
  * reference advects field by field in FP64) ---- */
 #define MIMADV_F64 0

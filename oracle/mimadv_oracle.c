@@ -1022,6 +1022,64 @@ void orc_scalar_tendency(const orc_grid *g, const orc_opts *o, const double *q,
 }
 
 /* ======================================================================
+ * Large-scale vertical advection: advection_1d / advection_ac_1d
+ * (advection.f90:918-1042) with grad_1d (gradients.f90:400-473)
+ * ====================================================================== */
+
+/* grad_1d: gz(k) for k = 1..nz-1 from the quadratic through levels k-1, k,
+ * k+1 (the floor reads the value of level 1 at distance dz*fdz0(1));
+ * gz(nz) = 0 */
+void orc_grad_1d(const orc_grid *g, const double *xz, double *gz)
+{
+    int nz = g->nz;
+    double dz = g->dz;
+    const double *fdz0 = g->fdz0;
+    for (int k = 1; k <= nz; ++k) L(gz, k) = 0.0;
+    for (int k = 1; k <= nz - 1; ++k) {
+        double deltad = 0.5 * dz * (L(fdz0, k + 1) + L(fdz0, k));
+        double deltau = k == 1 ? dz * L(fdz0, 1) : 0.5 * dz * (L(fdz0, k) + L(fdz0, k - 1));
+        double deltac = (deltad + deltau) * (deltad * deltad + deltau * deltau) -
+                        (deltad - deltau) * (deltad * deltad - deltau * deltau);
+        double ffip1 = L(xz, k + 1) - L(xz, k);
+        double ffim1 = k == 1 ? 0.0 : L(xz, k) - L(xz, k - 1);
+        L(gz, k) = ((ffip1 + ffim1) * (deltad * deltad + deltau * deltau) -
+                    (ffip1 - ffim1) * (deltad * deltad - deltau * deltau)) / deltac;
+    }
+}
+
+/* advection_ac_1d: xs(i,j,:) = xs(i,j,:) - dref*w*grad_1d(xx/dref) on the
+ * interior columns; dref is the level profile dref_prof[nz] (den0 of the
+ * anelastic conservative build), the 3D array dref3 (pressure%dens of the
+ * compressible conservative build), or 1 when both are NULL */
+void orc_advection_1d(const orc_grid *g, const double *w0, const double *dref_prof, const double *dref3,
+                      const double *xx, double *xs)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    int three_d = ny > 1;
+    int jt0 = three_d ? 4 : 1, jt1 = three_d ? ny - 2 : 1;
+#pragma omp parallel num_threads(g_threads)
+    {
+        double *xz = (double *)malloc(sizeof(double) * (size_t)nz * 3);
+        double *gz = xz + nz, *dr = xz + 2 * nz;
+#pragma omp for collapse(2) schedule(static)
+        for (int j = jt0; j <= jt1; ++j)
+            for (int i = 4; i <= nx - 2; ++i) {
+                for (int k = 1; k <= nz; ++k) {
+                    size_t id = IDX(g, i, j, k);
+                    L(dr, k) = dref3 ? dref3[id] : (dref_prof ? L(dref_prof, k) : 1.0);
+                    L(xz, k) = xx[id] / L(dr, k);
+                }
+                orc_grad_1d(g, xz, gz);
+                for (int k = 1; k <= nz; ++k) {
+                    size_t id = IDX(g, i, j, k);
+                    xs[id] = xs[id] - L(dr, k) * L(w0, k) * L(gz, k);
+                }
+            }
+        free(xz);
+    }
+}
+
+/* ======================================================================
  * Momentum: windadv.f90
  * ====================================================================== */
 

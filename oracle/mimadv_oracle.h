@@ -81,6 +81,14 @@ void orc_advs_h(const orc_opts *o, int n, double d, int bc_lo, int bc_hi,
 void orc_advs_z(const orc_grid *g, const orc_opts *o, const double *x, double x0,
                 const double *w, double *xa, int adv);
 
+/* ---- large-scale vertical advection (advection.f90:918-1042) ---- */
+/* grad_1d (gradients.f90:400-473) of one column xz[nz] into gz[nz] */
+void orc_grad_1d(const orc_grid *g, const double *xz, double *gz);
+/* xs -= dref*w0*grad_1d(xx/dref) on interior columns; dref from dref3 (3D),
+ * else dref_prof (levels), else 1 */
+void orc_advection_1d(const orc_grid *g, const double *w0, const double *dref_prof, const double *dref3,
+                      const double *xx, double *xs);
+
 /* ---- 3D operator ---- */
 /* xs = xs - div_dir(q) for dir = 0 (x), 1 (y), 2 (z); like advection_{x,y,z}. */
 void orc_advection_dir(const orc_grid *g, const orc_opts *o, int dir,

@@ -75,6 +75,8 @@ def load():
     lib.orc_get_conserved.argtypes = [G, dp, dp, i]
     lib.orc_set_threads.argtypes = [i]
     lib.orc_line_scheme.argtypes = [i, i, i]
+    lib.orc_grad_1d.argtypes = [G, dp, dp]
+    lib.orc_advection_1d.argtypes = [G, dp, dp, dp, dp, dp]
     lib.orc_line_scheme.restype = i
     lib.orc_set_threads.restype = i
     _lib = lib
@@ -146,6 +148,20 @@ class Oracle:
         self.lib.orc_advs_z(C.byref(self.g), C.byref(self.o), _dp(x), float(x[0] if x0 is None else x0), _dp(w),
                             _dp(xa), 1 if adv else 0)
         return xa
+
+    def grad_1d(self, xz):
+        xz = np.ascontiguousarray(xz, dtype=np.float64)
+        gz = np.zeros_like(xz)
+        self.lib.orc_grad_1d(C.byref(self.g), _dp(xz), _dp(gz))
+        return gz
+
+    def advection_1d(self, w0, xx, xs, dref_prof=None, dref3=None):
+        """xs -= dref*w0*grad_1d(xx/dref) on interior columns (in place)."""
+        w0 = np.ascontiguousarray(w0, dtype=np.float64)
+        prof = None if dref_prof is None else np.ascontiguousarray(dref_prof, dtype=np.float64)
+        d3 = None if dref3 is None else np.ascontiguousarray(dref3, dtype=np.float64)
+        self.lib.orc_advection_1d(C.byref(self.g), _dp(w0), None if prof is None else _dp(prof),
+                                  None if d3 is None else _dp(d3), _dp(xx), _dp(xs))
 
     def qnadv_uvw(self, ntime, mu, mv, mw, ut, vt, wt):
         ua, va, wa = (np.zeros_like(mu) for _ in range(3))

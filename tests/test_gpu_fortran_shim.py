@@ -39,7 +39,7 @@ def test_fortran_shim_matches_oracle(cuda, tmp_path, scheme, code, lim, limit, s
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
     r = subprocess.run([DRIVER, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    out = np.fromfile(fout, dtype=np.float64).reshape(6, *g.shape)
+    out = np.fromfile(fout, dtype=np.float64).reshape(7, *g.shape)
     line = load().orc_line_scheme(SCHEMES[scheme], 0 if limit else 1, sorder)
     ora = Oracle(g, scal_adv={v: k for k, v in SCHEMES.items()}[line], muscllimiter=lim)
     x_only = np.zeros(g.shape)
@@ -50,5 +50,9 @@ def test_fortran_shim_matches_oracle(cuda, tmp_path, scheme, code, lim, limit, s
     assert relmax(out[1], full) <= 1e-12
     assert relmax(out[2], full) <= 1e-12
     ua, va, wa = ora.qnadv_uvw(2, mu, mv, mw, u, v, w)
-    for got, ref in zip(out[3:], (ua, va, wa)):
+    for got, ref in zip(out[3:6], (ua, va, wa)):
         assert relmax(got, ref) <= 1e-12
+    k = np.arange(1, g.nz + 1, dtype=float)
+    x1d = np.zeros(g.shape)
+    ora.advection_1d(-1e-3 * k, q, x1d, dref_prof=1.2 - 0.01 * k)
+    assert np.array_equal(out[6], x1d)

@@ -230,3 +230,38 @@ def test_euler_runs_lax_wendroff(cuda, scheme, limit):
         a.advance(_lib.INT_EULER, 4, dt, du, dv, dw, [dq], limit=limit)
         got = dq.cpu().numpy()
     assert relmax(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("dref", ["one", "den0", "dens3d"])
+@pytest.mark.parametrize("dims", ["3d", "2d"])
+@pytest.mark.parametrize("host", [False, True])
+def test_advection_1d(cuda, dref, dims, host):
+    """advection_ac_1d (large-scale subsidence, advection.f90:993-1042 with
+    grad_1d): bit-identical to the oracle for dref = 1, the den0 profile and
+    a 3D density, on device and host arrays."""
+    print(f"case advection_1d {dref} {dims} host={host}", flush=True)
+    g, u, v, w, q = case3d() if dims == "3d" else case2d()
+    rng = np.random.default_rng(12)
+    z = g.zc()
+    w0 = -0.005 * (1.0 + z / z.max()) + 1e-4 * rng.standard_normal(g.nz)
+    den0 = 1.2 * np.exp(-z / 8000.0)
+    dens = np.broadcast_to(den0[:, None, None], g.shape) * (1.0 + 0.01 * rng.standard_normal(g.shape))
+    dens = np.ascontiguousarray(dens)
+    xs0 = rng.standard_normal(g.shape) * 1e-6
+    ref = xs0.copy()
+    ora = Oracle(g)
+    ora.advection_1d(w0, q, ref, dref_prof=den0 if dref == "den0" else None,
+                     dref3=dens if dref == "dens3d" else None)
+    with Advection(g) as a:
+        if dref == "den0":
+            a.set_reference_density(den0)
+        if host:
+            xs = xs0.copy()
+            a.advection_1d(w0, [q], [xs], dens=dens if dref == "dens3d" else None, den0=dref == "den0")
+            got = xs
+        else:
+            dq, dx, dd = _dev(cuda, q, xs0, dens)
+            a.advection_1d(w0, [dq], [dx], dens=dd if dref == "dens3d" else None, den0=dref == "den0")
+            a.stream_sync()
+            got = dx.cpu().numpy()
+    assert np.array_equal(got, ref)

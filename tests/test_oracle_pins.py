@@ -454,6 +454,54 @@ def test_line_scheme_dispatch():
         assert lib.orc_line_scheme(sc, 0, 2) == sc
 
 
+# ------------------------------------------- large-scale vertical advection -----
+
+def test_grad_1d_quadratic_in_height():
+    """grad_1d (gradients.f90:400-473) on a stretched grid: exact derivative
+    at the cell centres for quadratic-in-height data on levels 2..nz-1; at
+    level 1 the derivative of the parabola through (z_c(1) - dz*fdz0(1),
+    x(1)), (z_c(1), x(1)), (z_c(2), x(2)) (an independent polynomial fit);
+    gz(nz) = 0."""
+    g = gm.stretched(10, 0, 20)
+    z = g.zc()
+    q = lambda t: 3.0 - 0.002 * t + 1.5e-6 * t * t
+    dq = lambda t: -0.002 + 3.0e-6 * t
+    ora = Oracle(g)
+    gz = ora.grad_1d(q(z))
+    assert np.allclose(gz[1:g.nz - 1], dq(z[1:g.nz - 1]), rtol=1e-9, atol=0)
+    x1, x2 = q(z[0]), q(z[1])
+    c = np.polyfit([z[0] - g.dz * g.fdz[0], z[0], z[1]], [x1, x1, x2], 2)
+    assert gz[0] == pytest.approx(2 * c[0] * z[0] + c[1], rel=1e-9)
+    assert gz[g.nz - 1] == 0.0
+
+
+def test_advection_1d_closed_form_and_density_forms():
+    """advection_ac_1d (advection.f90:993-1042): xs -= dref w0 d(x/dref)/dz.
+    With dref = 1, a constant w0 and quadratic-in-height data the tendency
+    is -w0 q'(z) on levels 2..nz-1; a 3D density equal to a level profile
+    gives the same result as the profile bit for bit."""
+    g = gm.stretched(12, 10, 16)
+    z = g.zc()
+    q = lambda t: 2.0 + 0.001 * t - 2.0e-7 * t * t
+    dq = lambda t: 0.001 - 4.0e-7 * t
+    ora = Oracle(g)
+    w0 = np.full(g.nz, -0.01)
+    x = np.broadcast_to(q(z)[:, None, None], g.shape).copy()
+    xs = np.zeros(g.shape)
+    ora.advection_1d(w0, x, xs)
+    I = g.interior()
+    exp = (-w0 * dq(z))[1:g.nz - 1, None, None]
+    assert np.allclose(xs[I][1:g.nz - 1], np.broadcast_to(exp, xs[I][1:g.nz - 1].shape), rtol=1e-9, atol=0)
+    rng = np.random.default_rng(3)
+    prof = 1.2 * np.exp(-z / 8000.0)
+    xr = rng.uniform(0.5, 1.5, g.shape)
+    a, b = np.zeros(g.shape), np.zeros(g.shape)
+    ora.advection_1d(w0 * np.linspace(1, 2, g.nz), xr, a, dref_prof=prof)
+    ora.advection_1d(w0 * np.linspace(1, 2, g.nz), xr, b, dref3=np.broadcast_to(prof[:, None, None], g.shape).copy())
+    assert np.array_equal(a, b)
+    assert np.count_nonzero(a[I]) > 0 and np.count_nonzero(a) == np.count_nonzero(a[I])
+
+
 # ------------------------------------------------- operator invariants -----
 
 @pytest.mark.parametrize("scheme,lim,dirs", [("weno", "minmod", 7), ("muscl", "minmod", 7), ("muscl", "MC", 7),
