@@ -21,21 +21,6 @@
 
 namespace mimadv {
 
-__device__ __forceinline__ double weno5_left(double a, double b, double c, double d, double e)
-{
-    const double eps = 1e-6;
-    double q0 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
-    double q1 = (-b + 5.0 * c + 2.0 * d) / 6.0;
-    double q2 = (2.0 * c + 5.0 * d - e) / 6.0;
-    double b0 = 13.0 / 12.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) + 0.25 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
-    double b1 = 13.0 / 12.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 0.25 * (b - d) * (b - d);
-    double b2 = 13.0 / 12.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) + 0.25 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
-    double a0 = 0.1 / ((eps + b0) * (eps + b0));
-    double a1 = 0.6 / ((eps + b1) * (eps + b1));
-    double a2 = 0.3 / ((eps + b2) * (eps + b2));
-    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
-}
-
 template <class QF, class UF>
 __device__ __forceinline__ double weno5_hface(int f, QF Q, UF U)
 {
@@ -251,7 +236,7 @@ hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const Sc
     case 0: return launch_weno_march(g, sp, a, 0, s);
     case 1: return launch_scheme<1>(g, sp, a, s);
     case 2: return launch_scheme<2>(g, sp, a, s);
-    case 3: return launch_scheme<3>(g, sp, a, s);
+    case 3: return launch_weno_march(g, sp, a, 0, s);
     case 4: return launch_scheme<4>(g, sp, a, s);
     case 5: return launch_scheme<5>(g, sp, a, s);
     case 6: return launch_scheme<6>(g, sp, a, s);

@@ -132,6 +132,25 @@ __device__ __forceinline__ double weno3_zface(int f, int nz, int bottom, QF Q, W
 }
 
 // ---------------------------------------------------------------------------
+// WENO5-JS (extension, no reference counterpart): Jiang & Shu (1996) left
+// face value from a..e = x(f-2..f+2); eps = 1e-6, p = 2
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double weno5_left(double a, double b, double c, double d, double e)
+{
+    const double eps = 1e-6;
+    double q0 = (2.0 * a - 7.0 * b + 11.0 * c) / 6.0;
+    double q1 = (-b + 5.0 * c + 2.0 * d) / 6.0;
+    double q2 = (2.0 * c + 5.0 * d - e) / 6.0;
+    double b0 = 13.0 / 12.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) + 0.25 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
+    double b1 = 13.0 / 12.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 0.25 * (b - d) * (b - d);
+    double b2 = 13.0 / 12.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) + 0.25 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
+    double a0 = 0.1 / ((eps + b0) * (eps + b0));
+    double a1 = 0.6 / ((eps + b1) * (eps + b1));
+    double a2 = 0.3 / ((eps + b2) * (eps + b2));
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+// ---------------------------------------------------------------------------
 // MUSCL (advq_muscl.f90)
 // ---------------------------------------------------------------------------
 

@@ -1,5 +1,5 @@
 // weno_march.hip -- plain per-field FP64 kernel of the reference 'weno'
-// scheme, column-marching form.
+// scheme (and of the WENO5-JS extension in x/y), column-marching form.
 //
 // Same results as the generic per-plane kernel (scalar_fp64.hip) and the
 // CPU restatement, bit for bit: every face value is computed with the
@@ -16,10 +16,10 @@
 //     side alone gives the same value up to the sign of an exact zero.
 //
 // Work mapping: a workgroup owns a TX x TY tile of columns and a chunk of
-// levels.  Per level it stages the tile plus a 2-cell halo of q in LDS
-// (ghost columns read as the caller filled them), evaluates the TY*(TX+1)
-// x faces and (TY+1)*TX y faces of the tile, and each thread finishes its
-// cell with the z faces from a register window q(k-1..k+2).  A chunk
+// levels.  Per level it stages the tile plus a 2-cell (WENO5: 3-cell) halo
+// of q in LDS (ghost columns read as the caller filled them), evaluates the
+// TY*(TX+1) x faces and (TY+1)*TX y faces of the tile, and each thread
+// finishes its cell with the z faces from a register window q(k-1..k+2).  A chunk
 // starting above the floor first evaluates its bottom face.
 //
 // Compiled with -ffp-contract=off.
@@ -94,13 +94,35 @@ __device__ __forceinline__ double zface(int f, int nz, int bottom, double a, dou
     return weno_face_up(a, b, c, d, ww, f >= 3 && f <= nz - 3);
 }
 
-template <int TX, int TY>
+// x/y face value of the upwind side, from the LDS stencil starting at b
+// (step st): the reference 'weno' reads q(f-1..f+2) (4 values), WENO5 reads
+// q(f-2..f+3) (6 values).  As for 'weno', the reference-form sum
+// 0.5*fl*(u+|u|) + 0.5*fr*(u-|u|) has an exact zero off-side term.
+template <int XS>
+__device__ __forceinline__ double xy_face(const double *t, int b, int st, double uu)
+{
+    if constexpr (XS == 0) {
+        return weno_face_up(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], uu, true);
+    } else {
+        if (uu > 0.0) {
+            const double fl = weno5_left(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], t[b + 4 * st]);
+            return 0.5 * fl * (uu + fabs(uu));
+        }
+        const double fr = weno5_left(t[b + 5 * st], t[b + 4 * st], t[b + 3 * st], t[b + 2 * st], t[b + st]);
+        return 0.5 * fr * (uu - fabs(uu));
+    }
+}
+
+// XS: 0 the reference 'weno' in x/y, 3 WENO5-JS in x/y (both with the
+// reference 'weno' in z)
+template <int TX, int TY, int XS>
 __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemeParams sp, ScalarArgs a, int kchunk,
                                                             int ntx, int nwork)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
-    constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;
+    constexpr int H = XS == 3 ? 3 : 2;  // x/y halo of the LDS tile
+    constexpr int LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1;
     constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
     constexpr int FS = (NF + NT - 1) / NT, TS = (LH * LW + NT - 1) / NT;
     __shared__ double tile[LH * LW];
@@ -142,11 +164,11 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
             t_off[s] = -1;
             if (e < LH * LW) {
                 const int r = e / LW, c = e - r * LW;
-                int ii = i0 - 2 + c;
+                int ii = i0 - H + c;
                 if (ii > nx) ii = (ii - 4) % mx + 4;
                 int jj = 1;
                 if (THREE_D) {
-                    jj = j0 - 2 + r;
+                    jj = j0 - H + r;
                     if (jj > ny) jj = (jj - 4) % my + 4;
                 }
                 t_off[s] = (ii - 1) + nx * (jj - 1);
@@ -162,13 +184,13 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
                 // x face f + 1/2 (velocity u(f+1)), f = i0-1+fl, of tile row r
                 const int r = e / (TX + 1), fl = e - r * (TX + 1);
                 const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
-                f_base[s] = (THREE_D ? r + 2 : 0) * LW + fl;
+                f_base[s] = (THREE_D ? r + H : 0) * LW + fl;
                 f_step[s] = 1;
                 f_off[s] = (size_t)(min(i0 + fl, nx) - 1) + (size_t)nx * (jr - 1);
             } else if (e < NF) {
                 // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
                 const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
-                f_base[s] = fr * LW + c + 2;
+                f_base[s] = fr * LW + c + H;
                 f_step[s] = LW;
                 f_off[s] = (size_t)(min(i0 + c, nx - 2) - 1) + (size_t)nx * (min(j0 + fr, ny) - 1);
             }
@@ -230,7 +252,7 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
                 const int st = f_step[s];
                 if (st == 0) continue;
                 const int b = f_base[s];
-                face[tid + s * NT] = weno_face_up(tile[b], tile[b + st], tile[b + 2 * st], tile[b + 3 * st], vc[s], true);
+                face[tid + s * NT] = xy_face<XS>(tile, b, st, vc[s]);
             }
             __syncthreads();
             if (valid) {
@@ -313,7 +335,7 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
     }
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int XS>
 hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
     const int ntx = (g.nx - 5 + TX - 1) / TX;
@@ -322,7 +344,8 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
     if (kchunk > g.nz) kchunk = g.nz;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     const int nwork = ntx * nty * nchunk;
-    hipLaunchKernelGGL((weno_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx, nwork);
+    hipLaunchKernelGGL((weno_march_kernel<TX, TY, XS>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
+                       nwork);
     return hipGetLastError();
 }
 
@@ -330,8 +353,12 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
 
 hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
-    if (g.ny > 1) return launch_tile<64, 4>(g, sp, a, kchunk, s);
-    return launch_tile<64, 1>(g, sp, a, kchunk, s);
+    if (sp.scheme == 3) {
+        if (g.ny > 1) return launch_tile<64, 4, 3>(g, sp, a, kchunk, s);
+        return launch_tile<64, 1, 3>(g, sp, a, kchunk, s);
+    }
+    if (g.ny > 1) return launch_tile<64, 4, 0>(g, sp, a, kchunk, s);
+    return launch_tile<64, 1, 0>(g, sp, a, kchunk, s);
 }
 
 }  // namespace mimadv
