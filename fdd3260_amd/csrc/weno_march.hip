@@ -55,7 +55,11 @@ __device__ __forceinline__ double weno_face_up(double a, double b, double c, dou
     double ffip1 = 0.5 * b + 0.5 * c;
     double ffip2 = 1.5 * c - 0.5 * d;
     const double minp = dmind(dmind(dmind(ffim1, ffip1), ffip1), ffip2);
-    const double phi = fmin(fabs((0.0 - b) / (minp - b + eps)), 1.0);
+    // phi = min(|(0-b)/den|, 1): when |b| >= |den| the correctly rounded
+    // quotient is at least 1 in magnitude (or inf, or NaN for 0/0, which
+    // fmin maps to 1), so phi = 1 without the division
+    const double den = minp - b + eps;
+    const double phi = fabs(b) >= fabs(den) ? 1.0 : fmin(fabs((0.0 - b) / den), 1.0);
     const double fo = phi * ((left ? ffim1 : ffip2) - b) + b;
     const double fi = phi * (ffip1 - b) + b;
     const double f = wo * fo + wi * fi;
