@@ -567,8 +567,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("weno_march_kernel<64,4>" if args.scheme == "weno"
-                       else f"scalar_fp64_kernel<{SCHEME_ID[args.scheme]},64,4>"),
+            "kernel": ({"weno": "weno_march_kernel<64,4>", "weno5": "weno5_march_kernel<64,4>"}.get(args.scheme)
+                       or f"scalar_fp64_kernel<{SCHEME_ID[args.scheme]},64,4>"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

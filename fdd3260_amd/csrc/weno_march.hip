@@ -120,8 +120,8 @@ __device__ __forceinline__ double xy_face(const double *t, int b, int st, double
 // XS: 0 the reference 'weno' in x/y, 3 WENO5-JS in x/y (both with the
 // reference 'weno' in z)
 template <int TX, int TY, int XS>
-__global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemeParams sp, ScalarArgs a, int kchunk,
-                                                            int ntx, int nwork)
+__device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a,
+                                                int kchunk, int ntx, int nwork)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
@@ -339,6 +339,24 @@ __global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemePar
     }
 }
 
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemeParams sp, ScalarArgs a, int kchunk,
+                                                            int ntx, int nwork)
+{
+    weno_march_body<TX, TY, 0>(g, sp, a, kchunk, ntx, nwork);
+}
+
+// WENO5 left to itself takes 170 VGPRs, which leaves 2 waves per SIMD;
+// capped at 128 (4 waves) it spills a few per-work-item values and runs
+// 375 -> 324 us at 256x256x128.  The reference-weno instance gains
+// nothing from the same cap.
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno5_march_kernel(
+    GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
+{
+    weno_march_body<TX, TY, 3>(g, sp, a, kchunk, ntx, nwork);
+}
+
 template <int TX, int TY, int XS>
 hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
@@ -348,8 +366,12 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
     if (kchunk > g.nz) kchunk = g.nz;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     const int nwork = ntx * nty * nchunk;
-    hipLaunchKernelGGL((weno_march_kernel<TX, TY, XS>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
-                       nwork);
+    if constexpr (XS == 3)
+        hipLaunchKernelGGL((weno5_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
+                           nwork);
+    else
+        hipLaunchKernelGGL((weno_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
+                           nwork);
     return hipGetLastError();
 }
 
