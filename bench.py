@@ -567,7 +567,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ({"weno": "weno_march_kernel<64,4>", "weno5": "weno5_march_kernel<64,4>"}.get(args.scheme)
+            "kernel": ({"weno": "weno_march_kernel<32,8>", "weno5": "weno5_march_kernel<16,16>"}.get(args.scheme)
                        or f"scalar_fp64_kernel<{SCHEME_ID[args.scheme]},64,4>"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -17,7 +17,8 @@
 //
 // Work mapping: a workgroup owns a TX x TY tile of columns and a chunk of
 // levels.  Per level it stages the tile plus a 2-cell (WENO5: 3-cell) halo
-// of q in LDS (ghost columns read as the caller filled them), evaluates the
+// of q and the face velocities in LDS (ghost columns read as the caller
+// filled them; everything is requested one level ahead), evaluates the
 // TY*(TX+1) x faces and (TY+1)*TX y faces of the tile, and each thread
 // finishes its cell with the z faces from a register window q(k-1..k+2).  A chunk
 // starting above the floor first evaluates its bottom face.
@@ -163,6 +164,7 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     constexpr int FS = (NF + NT - 1) / NT, TS = (LH * LW + NT - 1) / NT;
     __shared__ double tile[LH * LW];
     __shared__ double face[FS * NT];
+    __shared__ double fvel[FS * NT];  // face velocities of each thread's slots
 
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t plane = (size_t)nx * ny;
@@ -186,18 +188,19 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
         const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
         // loads of threads past the edge use the last interior column
         const int il = min(i, nx - 2), jl = THREE_D ? min(j, ny - 2) : 1;
-        const size_t col = (size_t)(il - 1) + (size_t)nx * (jl - 1);
+        const unsigned col = (unsigned)((il - 1) + nx * (jl - 1));
         auto Qz = [&](int kk) { return a.q[col + (size_t)(clampi(kk, 1, nz) - 1) * plane]; };
         // fixed slots of this thread: tile values (ghost columns read as the
         // caller filled them; positions past the array take a periodic image
         // that feeds no interior cell) and faces of the flattened list
         // [x faces row-major | y faces] with their velocity offsets
-        int t_off[TS], f_base[FS], f_step[FS];
-        size_t f_off[FS];
+        // (plane offsets are 32-bit: a plane holds fewer than 2^31 points)
+        unsigned t_off[TS], f_off[FS];
+        int f_base[FS], f_step[FS];
 #pragma unroll
         for (int s = 0; s < TS; ++s) {
             const int e = tid + s * NT;
-            t_off[s] = -1;
+            t_off[s] = 0;
             if (e < LH * LW) {
                 const int r = e / LW, c = e - r * LW;
                 int ii = i0 - H + c;
@@ -207,7 +210,7 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
                     jj = j0 - H + r;
                     if (jj > ny) jj = (jj - 4) % my + 4;
                 }
-                t_off[s] = (ii - 1) + nx * (jj - 1);
+                t_off[s] = (unsigned)((ii - 1) + nx * (jj - 1));
             }
         }
 #pragma unroll
@@ -222,13 +225,13 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
                 const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
                 f_base[s] = (THREE_D ? r + H : 0) * LW + fl;
                 f_step[s] = 1;
-                f_off[s] = (size_t)(min(i0 + fl, nx) - 1) + (size_t)nx * (jr - 1);
+                f_off[s] = (unsigned)((min(i0 + fl, nx) - 1) + nx * (jr - 1));
             } else if (e < NF) {
                 // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
                 const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
                 f_base[s] = fr * LW + c + H;
                 f_step[s] = LW;
-                f_off[s] = (size_t)(min(i0 + c, nx - 2) - 1) + (size_t)nx * (min(j0 + fr, ny) - 1);
+                f_off[s] = (unsigned)((min(i0 + c, nx - 2) - 1) + nx * (min(j0 + fr, ny) - 1));
             }
         }
         // A chunk above the floor first evaluates its bottom face k0-1/2 in a
@@ -241,13 +244,13 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
         double tv[TS], vv[FS], wv = 0.0, zn = 0.0, q0v = 0.0, tdv = 0.0, xsv = 0.0;
         auto fetch = [&](int k) {
             const size_t kb = (size_t)(k - 1) * plane;
-            const double *qk = a.q + kb;
+            const double *qk = a.q + kb, *uk = a.u + kb, *vk = a.v + kb;
 #pragma unroll
             for (int s = 0; s < TS; ++s)
-                if (t_off[s] >= 0) tv[s] = qk[t_off[s]];
+                if (tid + s * NT < LH * LW) tv[s] = qk[t_off[s]];
 #pragma unroll
             for (int s = 0; s < FS; ++s)
-                vv[s] = f_step[s] == 1 ? a.u[kb + f_off[s]] : (f_step[s] != 0 ? a.v[kb + f_off[s]] : 0.0);
+                vv[s] = f_step[s] == 1 ? uk[f_off[s]] : (f_step[s] != 0 ? vk[f_off[s]] : 0.0);
             wv = k < nz ? a.w[col + kb + plane] : 0.0;
             zn = Qz(k + 2);
             if (valid) {
@@ -275,10 +278,9 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
             const size_t kb = (size_t)(k - 1) * plane;
 #pragma unroll
             for (int s = 0; s < TS; ++s)
-                if (t_off[s] >= 0) tile[tid + s * NT] = tv[s];
-            double vc[FS];
+                if (tid + s * NT < LH * LW) tile[tid + s * NT] = tv[s];
 #pragma unroll
-            for (int s = 0; s < FS; ++s) vc[s] = vv[s];
+            for (int s = 0; s < FS; ++s) fvel[tid + s * NT] = vv[s];
             const double q0c = q0v, tdc = tdv, xsc = xsv;
             __syncthreads();
             if (k < k1) fetch(k + 1);
@@ -288,7 +290,7 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
                 const int st = f_step[s];
                 if (st == 0) continue;
                 const int b = f_base[s];
-                face[tid + s * NT] = xy_face<XS>(tile, b, st, vc[s]);
+                face[tid + s * NT] = xy_face<XS>(tile, b, st, fvel[tid + s * NT]);
             }
             __syncthreads();
             if (valid) {
@@ -371,17 +373,17 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     }
 }
 
+// Both instances are capped at 128 VGPRs (4 waves per SIMD), at the price of
+// a few spilled per-work-item values.  Uncapped, the reference-weno 32x8
+// instance takes 151 VGPRs (3 waves) and runs 217 us per launch at
+// 256x256x128 against 208 capped; WENO5 takes 170 (2 waves).
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void weno_march_kernel(GridDev g, SchemeParams sp, ScalarArgs a, int kchunk,
-                                                            int ntx, int nwork)
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno_march_kernel(
+    GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
 {
     weno_march_body<TX, TY, 0>(g, sp, a, kchunk, ntx, nwork);
 }
 
-// WENO5 left to itself takes 170 VGPRs, which leaves 2 waves per SIMD;
-// capped at 128 (4 waves) it spills a few per-work-item values and runs
-// 375 -> 324 us at 256x256x128.  The reference-weno instance gains
-// nothing from the same cap.
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno5_march_kernel(
     GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
@@ -389,12 +391,12 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     weno_march_body<TX, TY, 3>(g, sp, a, kchunk, ntx, nwork);
 }
 
-template <int TX, int TY, int XS>
+template <int TX, int TY, int XS, int KC>
 hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
     const int ntx = (g.nx - 5 + TX - 1) / TX;
     const int nty = TY > 1 ? (g.ny - 5 + TY - 1) / TY : 1;
-    if (kchunk <= 0) kchunk = 16;
+    if (kchunk <= 0) kchunk = KC;
     if (kchunk > g.nz) kchunk = g.nz;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     const int nwork = ntx * nty * nchunk;
@@ -409,14 +411,19 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
 
 }  // namespace
 
+// Tiles, measured at 256x256x128 before the VGPR cap of the reference-weno
+// instance: the reference 'weno' on 32x8 columns with 8-level chunks (217 us
+// per launch; 64x4 took 228: 2.27 x/y faces per cell instead of 2.16 and a
+// wider halo), WENO5 on 16x16 with 16-level chunks (285 us; 32x8 takes 308,
+// 64x4 323).  Tiles of 512 threads are slower (fewer waves per SIMD).
 hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
     if (sp.scheme == 3) {
-        if (g.ny > 1) return launch_tile<64, 4, 3>(g, sp, a, kchunk, s);
-        return launch_tile<64, 1, 3>(g, sp, a, kchunk, s);
+        if (g.ny > 1) return launch_tile<16, 16, 3, 16>(g, sp, a, kchunk, s);
+        return launch_tile<64, 1, 3, 16>(g, sp, a, kchunk, s);
     }
-    if (g.ny > 1) return launch_tile<64, 4, 0>(g, sp, a, kchunk, s);
-    return launch_tile<64, 1, 0>(g, sp, a, kchunk, s);
+    if (g.ny > 1) return launch_tile<32, 8, 0, 8>(g, sp, a, kchunk, s);
+    return launch_tile<64, 1, 0, 16>(g, sp, a, kchunk, s);
 }
 
 }  // namespace mimadv
