@@ -165,6 +165,9 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     __shared__ double tile[LH * LW];
     __shared__ double face[FS * NT];
     __shared__ double fvel[FS * NT];  // face velocities of each thread's slots
+    // per-thread slot constants of the current work item, kept in LDS rather
+    // than registers: [t_off(TS) | f_off(FS) | f_slot(FS)] x NT
+    __shared__ unsigned slots[(TS + 2 * FS) * NT];
 
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const size_t plane = (size_t)nx * ny;
@@ -179,7 +182,9 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     const bool accum = a.epi == EPI_ACCUM, heun = a.epi == EPI_HEUN, so = a.epi == EPI_SO;
     const bool need_tend = heun && a.b_lm != 0.0, need_q0 = heun || so;
 
-    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    // one work item per workgroup (the grid is nwork groups): values derived
+    // from the thread index are not hoisted into a loop prologue and spilled
+    if (const int wi = blockIdx.x; wi < nwork) {
         const int tile_id = wi / nchunk, kc = wi - tile_id * nchunk;
         const int i0 = 4 + (tile_id % ntx) * TX;
         const int j0 = THREE_D ? 4 + (tile_id / ntx) * TY : 1;
@@ -194,13 +199,15 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
         // caller filled them; positions past the array take a periodic image
         // that feeds no interior cell) and faces of the flattened list
         // [x faces row-major | y faces] with their velocity offsets
-        // (plane offsets are 32-bit: a plane holds fewer than 2^31 points)
-        unsigned t_off[TS], f_off[FS];
-        int f_base[FS], f_step[FS];
+        // (plane offsets are 32-bit: a plane holds fewer than 2^31 points;
+        // f_slot = LDS stencil start | kind << 16, kind 0 empty, 1 x, 2 y face)
+        auto t_off = [&](int s) { return slots[s * NT + tid]; };
+        auto f_off = [&](int s) { return slots[(TS + s) * NT + tid]; };
+        auto f_slot = [&](int s) { return (int)slots[(TS + FS + s) * NT + tid]; };
 #pragma unroll
         for (int s = 0; s < TS; ++s) {
             const int e = tid + s * NT;
-            t_off[s] = 0;
+            unsigned off = 0;
             if (e < LH * LW) {
                 const int r = e / LW, c = e - r * LW;
                 int ii = i0 - H + c;
@@ -210,29 +217,28 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
                     jj = j0 - H + r;
                     if (jj > ny) jj = (jj - 4) % my + 4;
                 }
-                t_off[s] = (unsigned)((ii - 1) + nx * (jj - 1));
+                off = (unsigned)((ii - 1) + nx * (jj - 1));
             }
+            slots[s * NT + tid] = off;
         }
 #pragma unroll
         for (int s = 0; s < FS; ++s) {
             const int e = tid + s * NT;
-            f_base[s] = 0;
-            f_step[s] = 0;
-            f_off[s] = 0;
+            unsigned slot = 0, off = 0;
             if (e < NXF) {
                 // x face f + 1/2 (velocity u(f+1)), f = i0-1+fl, of tile row r
                 const int r = e / (TX + 1), fl = e - r * (TX + 1);
                 const int jr = THREE_D ? min(j0 + r, ny - 2) : 1;
-                f_base[s] = (THREE_D ? r + H : 0) * LW + fl;
-                f_step[s] = 1;
-                f_off[s] = (unsigned)((min(i0 + fl, nx) - 1) + nx * (jr - 1));
+                slot = ((THREE_D ? r + H : 0) * LW + fl) | 1 << 16;
+                off = (unsigned)((min(i0 + fl, nx) - 1) + nx * (jr - 1));
             } else if (e < NF) {
                 // y face g + 1/2 (velocity v(g+1)), g = j0-1+fr, of tile column c
                 const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX;
-                f_base[s] = fr * LW + c + H;
-                f_step[s] = LW;
-                f_off[s] = (unsigned)((min(i0 + c, nx - 2) - 1) + nx * (min(j0 + fr, ny) - 1));
+                slot = (fr * LW + c + H) | 2 << 16;
+                off = (unsigned)((min(i0 + c, nx - 2) - 1) + nx * (min(j0 + fr, ny) - 1));
             }
+            slots[(TS + s) * NT + tid] = off;
+            slots[(TS + FS + s) * NT + tid] = slot;
         }
         // A chunk above the floor first evaluates its bottom face k0-1/2 in a
         // z-only pass, so that every z face comes from the same code.
@@ -247,10 +253,13 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
             const double *qk = a.q + kb, *uk = a.u + kb, *vk = a.v + kb;
 #pragma unroll
             for (int s = 0; s < TS; ++s)
-                if (tid + s * NT < LH * LW) tv[s] = qk[t_off[s]];
+                if (tid + s * NT < LH * LW) tv[s] = qk[t_off(s)];
 #pragma unroll
             for (int s = 0; s < FS; ++s)
-                vv[s] = f_step[s] == 1 ? uk[f_off[s]] : (f_step[s] != 0 ? vk[f_off[s]] : 0.0);
+            {
+                const int kind = f_slot(s) >> 16;
+                vv[s] = kind == 1 ? uk[f_off(s)] : (kind == 2 ? vk[f_off(s)] : 0.0);
+            }
             wv = k < nz ? a.w[col + kb + plane] : 0.0;
             zn = Qz(k + 2);
             if (valid) {
@@ -287,9 +296,9 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
             // one face at a time: the exact FP64 face is register-hungry
 #pragma unroll 1
             for (int s = 0; s < FS; ++s) {
-                const int st = f_step[s];
-                if (st == 0) continue;
-                const int b = f_base[s];
+                const int fs = f_slot(s), kind = fs >> 16;
+                if (kind == 0) continue;
+                const int st = kind == 1 ? 1 : LW, b = fs & 0xffff;
                 face[tid + s * NT] = xy_face<XS>(tile, b, st, fvel[tid + s * NT]);
             }
             __syncthreads();
@@ -373,10 +382,10 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     }
 }
 
-// Both instances are capped at 128 VGPRs (4 waves per SIMD), at the price of
-// a few spilled per-work-item values.  Uncapped, the reference-weno 32x8
-// instance takes 151 VGPRs (3 waves) and runs 217 us per launch at
-// 256x256x128 against 208 capped; WENO5 takes 170 (2 waves).
+// Both instances are capped at 128 VGPRs (4 waves per SIMD); the per-thread
+// slot constants live in LDS to make room.  Uncapped, the reference-weno
+// 32x8 instance took 151 VGPRs (3 waves) and 217 us per launch at
+// 256x256x128; WENO5 took 170 (2 waves).
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno_march_kernel(
     GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
