@@ -159,24 +159,29 @@ __device__ __forceinline__ int perio(int i, int n, int m)
 }
 
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void adjoint_kernel(AdjArgs A)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
     constexpr int QW = TX + 6, QH = THREE_D ? TY + 6 : 1;   // q tile, halo 3
     constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;   // lam tile, halo 2
     constexpr int NFX = TX + 3, NFY = THREE_D ? TY + 3 : 0; // faces per row / column
-    constexpr int NXF = TY * NFX, NYF = NFY * TX;
+    constexpr int NXF = TY * NFX, NYF = NFY * TX, NF = NXF + NYF;
+    constexpr int QS = (QH * QW + NT - 1) / NT, LS = (LH * LW + NT - 1) / NT, FS = (NF + NT - 1) / NT;
     __shared__ double qt[QH][QW];
     __shared__ double lt[LH][LW];
-    __shared__ double cf[4][NXF + NYF];  // stencil contributions per face, slot-major
+    __shared__ double cf[4][NF];  // stencil contributions per face, slot-major
+    // plane offsets of this thread's tile values and face velocities, fixed
+    // for the work item: [q tile (QS) | lam tile (LS) | faces (FS)] x NT
+    __shared__ unsigned slots[(QS + LS + FS) * NT];
 
     const int nx = A.nx, ny = A.ny, nz = A.nz, mx = nx - 5, my = ny - 5;
     const size_t plane = (size_t)nx * ny;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
     const int nchunk = (nz + A.kchunk - 1) / A.kchunk;
 
-    for (int wi = blockIdx.x; wi < A.nwork; wi += gridDim.x) {
+    // one work item per workgroup (the grid is nwork groups)
+    if (const int wi = blockIdx.x; wi < A.nwork) {
         const int tile = wi / nchunk, kc = wi - tile * nchunk;
         const int bx = tile % A.ntx, by = tile / A.ntx;
         const int i0 = 4 + bx * TX, j0 = THREE_D ? 4 + by * TY : 1;
@@ -185,6 +190,50 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
         const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
         const int ip = perio(i, nx, mx), jp = THREE_D ? perio(j, ny, my) : 1;
         const size_t col = (size_t)(ip - 1) + (size_t)nx * (jp - 1);
+#pragma unroll
+        for (int s = 0; s < QS; ++s) {
+            const int t = tid + s * NT;
+            unsigned off = 0;
+            if (t < QH * QW) {
+                const int r = t / QW, c = t - r * QW;
+                const int ii = perio(i0 - 3 + c, nx, mx), jj = THREE_D ? perio(j0 - 3 + r, ny, my) : 1;
+                off = (unsigned)((ii - 1) + nx * (jj - 1));
+            }
+            slots[s * NT + tid] = off;
+        }
+#pragma unroll
+        for (int s = 0; s < LS; ++s) {
+            const int t = tid + s * NT;
+            unsigned off = 0;
+            if (t < LH * LW) {
+                const int r = t / LW, c = t - r * LW;
+                const int ii = perio(i0 - 2 + c, nx, mx), jj = THREE_D ? perio(j0 - 2 + r, ny, my) : 1;
+                off = (unsigned)((ii - 1) + nx * (jj - 1));
+            }
+            slots[(QS + s) * NT + tid] = off;
+        }
+#pragma unroll
+        for (int s = 0; s < FS; ++s) {
+            const int e = tid + s * NT;
+            unsigned off = 0;
+            if (!THREE_D || e < NXF) {
+                if (e < NXF) {
+                    // x face f + 1/2 (velocity u(f+1)), f = i0-2+fl, of tile row r
+                    const int r = e / NFX, fl = e - r * NFX, f = i0 - 2 + fl;
+                    const int jr = THREE_D ? perio(j0 + r, ny, my) : 1;
+                    off = (unsigned)((perio(f + 1, nx, mx) - 1) + nx * (jr - 1));
+                }
+            } else if (e < NF) {
+                // y face g + 1/2 (velocity v(g+1)), g = j0-2+fr, of tile column c
+                const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX, g = j0 - 2 + fr;
+                off = (unsigned)((perio(i0 + c, nx, mx) - 1) + nx * (perio(g + 1, ny, my) - 1));
+            }
+            slots[(QS + LS + s) * NT + tid] = off;
+        }
+        auto q_off = [&](int s) { return slots[s * NT + tid]; };
+        auto l_off = [&](int s) { return slots[(QS + s) * NT + tid]; };
+        auto f_off = [&](int s) { return slots[(QS + LS + s) * NT + tid]; };
+
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;  // levels k-1 .. k+2
         auto finalize = [&](int m, double val, double qb) {
             if (!valid || m < k0 || m > k1) return;
@@ -204,6 +253,25 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
         double z0 = Qz(ks - 1), z1 = Qz(ks), z2 = Qz(ks + 1), z3 = Qz(ks + 2);
         double qbm = QBz(ks - 1), qb0 = QBz(ks), qb1 = QBz(ks + 1);
         double wz = ks <= nz - 1 ? A.w[col + (size_t)ks * plane] : 0.0;
+        // tile values and face velocities of a chunk level, requested one
+        // level ahead
+        double qv[QS], lv[LS], fv[FS];
+        auto fetch = [&](int k) {
+            const size_t kb = (size_t)(k - 1) * plane;
+            const double *qk = A.q + kb, *lk = A.qbn + kb, *uk = A.u + kb, *vk = A.v + kb;
+#pragma unroll
+            for (int s = 0; s < QS; ++s)
+                if (tid + s * NT < QH * QW) qv[s] = qk[q_off(s)];
+#pragma unroll
+            for (int s = 0; s < LS; ++s)
+                if (tid + s * NT < LH * LW) lv[s] = lk[l_off(s)];
+#pragma unroll
+            for (int s = 0; s < FS; ++s) {
+                const int e = tid + s * NT;
+                fv[s] = (!THREE_D || e < NXF) ? (e < NXF ? uk[f_off(s)] : 0.0) : (e < NF ? vk[f_off(s)] : 0.0);
+            }
+        };
+        fetch(k0);
 #pragma unroll 1
         for (int k = ks; k <= ke; ++k) {
             const size_t kb = (size_t)(k - 1) * plane;
@@ -215,42 +283,40 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
                 wn = k + 1 <= nz - 1 ? A.w[col + (size_t)(k + 1) * plane] : 0.0;
             }
             if (k >= k0 && k <= k1) {
-                for (int t = tid; t < QH * QW; t += NT) {
-                    const int r = t / QW, c = t - r * QW;
-                    const int ii = perio(i0 - 3 + c, nx, mx);
-                    const int jj = THREE_D ? perio(j0 - 3 + r, ny, my) : 1;
-                    qt[r][c] = A.q[kb + (size_t)(ii - 1) + (size_t)nx * (jj - 1)];
-                }
-                for (int t = tid; t < LH * LW; t += NT) {
-                    const int r = t / LW, c = t - r * LW;
-                    const int ii = perio(i0 - 2 + c, nx, mx);
-                    const int jj = THREE_D ? perio(j0 - 2 + r, ny, my) : 1;
-                    lt[r][c] = A.ls * A.qbn[kb + (size_t)(ii - 1) + (size_t)nx * (jj - 1)];
-                }
+#pragma unroll
+                for (int s = 0; s < QS; ++s)
+                    if (tid + s * NT < QH * QW) (&qt[0][0])[tid + s * NT] = qv[s];
+#pragma unroll
+                for (int s = 0; s < LS; ++s)
+                    if (tid + s * NT < LH * LW) (&lt[0][0])[tid + s * NT] = A.ls * lv[s];
+                double vc[FS];
+#pragma unroll
+                for (int s = 0; s < FS; ++s) vc[s] = fv[s];
                 __syncthreads();
-                for (int e = tid; e < NXF + NYF; e += NT) {
+                if (k < k1) fetch(k + 1);
+#pragma unroll
+                for (int s = 0; s < FS; ++s) {
+                    const int e = tid + s * NT;
+                    if (e >= NF) continue;
                     double ga, gb, gc, gd;
                     if (!THREE_D || e < NXF) {
                         // x face f + 1/2 (velocity u(f+1)), f = i0-2+fl, of tile row r
                         const int r = e / NFX, fl = e - r * NFX, f = i0 - 2 + fl;
-                        const int jr = THREE_D ? perio(j0 + r, ny, my) : 1;
-                        const size_t ui = kb + (size_t)(perio(f + 1, nx, mx) - 1) + (size_t)nx * (jr - 1);
-                        const double uf = A.u[ui];
+                        const double uf = vc[s];
                         const int lr = THREE_D ? r + 2 : 0, qr = THREE_D ? r + 3 : 0;
                         const double Fb = (lt[lr][fl + 1] - lt[lr][fl]) * A.dx1;
                         const double R = face_vjp(qt[qr][fl], qt[qr][fl + 1], qt[qr][fl + 2], qt[qr][fl + 3],
                                                   !(uf < 0.0), 0, true, Fb * uf, ga, gb, gc, gd);
-                        if (f >= i0 && f < i0 + TX && f <= nx - 2 && (!THREE_D || j0 + r <= ny - 2)) A.ub[ui] += Fb * R;
+                        if (f >= i0 && f < i0 + TX && f <= nx - 2 && (!THREE_D || j0 + r <= ny - 2))
+                            A.ub[kb + f_off(s)] += Fb * R;
                     } else if constexpr (THREE_D) {
                         // y face g + 1/2 (velocity v(g+1)), g = j0-2+fr, of tile column c
                         const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX, g = j0 - 2 + fr;
-                        const int ic = perio(i0 + c, nx, mx);
-                        const size_t vi = kb + (size_t)(ic - 1) + (size_t)nx * (perio(g + 1, ny, my) - 1);
-                        const double vf = A.v[vi];
+                        const double vf = vc[s];
                         const double Fb = (lt[fr + 1][c + 2] - lt[fr][c + 2]) * A.dy1;
                         const double R = face_vjp(qt[fr][c + 3], qt[fr + 1][c + 3], qt[fr + 2][c + 3],
                                                   qt[fr + 3][c + 3], !(vf < 0.0), 0, true, Fb * vf, ga, gb, gc, gd);
-                        if (g >= j0 && g < j0 + TY && g <= ny - 2 && i0 + c <= nx - 2) A.vb[vi] += Fb * R;
+                        if (g >= j0 && g < j0 + TY && g <= ny - 2 && i0 + c <= nx - 2) A.vb[kb + f_off(s)] += Fb * R;
                     }
                     cf[0][e] = ga;
                     cf[1][e] = gb;
@@ -266,9 +332,10 @@ __global__ __launch_bounds__(TX *TY) void adjoint_kernel(AdjArgs A)
                     xy += cf[0][ey + 3 * TX] + cf[1][ey + 2 * TX] + cf[2][ey + TX] + cf[3][ey];
                 }
                 acc1 += xy;
-                // no barrier before the next level's tile writes: cf is
-                // rewritten only after the next barrier, which every thread
-                // reaches after its gather
+                // no barrier before the next level's tile writes: qt, lt and
+                // cf are read before the barrier above or by the gather,
+                // and cf is rewritten only after the next level's first
+                // barrier, which every thread reaches after its gather
             }
             // z face k+1/2 (velocity w(k+1)); no flux through floor and lid
             if (k <= nz - 1) {
