@@ -567,8 +567,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ({"weno": "weno_march_kernel<32,8>", "weno5": "weno5_march_kernel<16,16>"}.get(args.scheme)
-                       or f"scalar_fp64_kernel<{SCHEME_ID[args.scheme]},64,4>"),
+            "kernel": {"weno": "weno_march_kernel<32,8>", "weno5": "weno5_march_kernel<16,16>",
+                       "muscl": "line_march_kernel<2,32,8>", "ppm": "scalar_fp64_kernel<1,64,4>"}[args.scheme],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

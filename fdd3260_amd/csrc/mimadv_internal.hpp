@@ -65,8 +65,9 @@ hipError_t launch_adjoint_stage(const AdjArgs &a, hipStream_t s);
 // dst = src on interior cells, 0 on ghost cells
 hipError_t launch_interior_copy(const GridDev &g, const double *src, double *dst, hipStream_t s);
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s);
-// 'weno' scheme, column-marching form (weno_march.hip); kchunk 0 = default
-hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s);
+// column-marching kernels (weno_march.hip): 'weno', WENO5, MUSCL, QUICK,
+// Lax-Wendroff and the unlimited forms; kchunk 0 = default
+hipError_t launch_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s);
 hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int ntime, const double *mu,
                                 const double *mv, const double *mw, const double *ut, const double *vt,
                                 const double *wt, double *ua, double *va, double *wa, hipStream_t s);

@@ -1,4 +1,6 @@
-// scalar_fp64.hip -- plain per-field FP64 scalar advection on gfx950.
+// scalar_fp64.hip -- plain per-field FP64 scalar advection on gfx950: the
+// per-plane PPM kernel and the scheme dispatch (the other schemes run on the
+// column-marching kernels of weno_march.hip).
 //
 // One launch computes the fused x+y+z flux divergence of one field
 // (advection_x/y/z of src/TOOLS/advection.f90:556-798 for all three
@@ -21,88 +23,47 @@
 
 namespace mimadv {
 
-template <class QF, class UF>
-__device__ __forceinline__ double weno5_hface(int f, QF Q, UF U)
-{
-    double uu = U(f + 1);
-    double fl = weno5_left(Q(f - 2), Q(f - 1), Q(f), Q(f + 1), Q(f + 2));
-    double fr = weno5_left(Q(f + 3), Q(f + 2), Q(f + 1), Q(f), Q(f - 1));
-    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
-}
-
+// x/y contribution of cell i (PPM; the other schemes run on the column-
+// marching kernels of weno_march.hip)
 template <int SCHEME, class QF, class UF>
 __device__ __forceinline__ double h_contrib(int i, int n, double d, bool adv, int bc_hi, const SchemeParams &sp,
                                             QF Q, UF U)
 {
+    static_assert(SCHEME == 1, "per-plane kernel: PPM only");
     double d1 = 1.0 / d;
-    if constexpr (SCHEME == 0) {
-        return weno3_h(i, d1, adv, Q, U);
-    } else if constexpr (SCHEME == 1) {
-        return ppm_h(i, n, d1, adv, bc_hi, sp, Q, U);
-    } else if constexpr (SCHEME == 2) {
-        return muscl_h(i, d, d1, adv, sp, Q, U);
-    } else if constexpr (SCHEME >= 4) {
-        // QUICK (4), Lax-Wendroff (5) and their unlimited forms (6, 7)
-        auto face = [&](int f) -> double {
-            if constexpr (SCHEME == 4) return quick_hface<false>(f, sp, Q, U);
-            else if constexpr (SCHEME == 6) return quick_hface<true>(f, sp, Q, U);
-            else if constexpr (SCHEME == 5) return lw_hface<false>(f, d1, sp, Q, U);
-            else return lw_hface<true>(f, d1, sp, Q, U);
-        };
-        double t = (face(i) - face(i - 1)) * d1;
-        if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
-        return t;
-    } else {
-        double t = (weno5_hface(i, Q, U) - weno5_hface(i - 1, Q, U)) * d1;
-        if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
-        return t;
-    }
+    return ppm_h(i, n, d1, adv, bc_hi, sp, Q, U);
 }
 
+// z contribution of cell k (PPM)
 template <int SCHEME, class QF, class WF>
 __device__ __forceinline__ double z_contrib(int k, int nz, bool adv, const GridDev &g, const SchemeParams &sp,
                                             QF Q, WF W)
 {
+    static_assert(SCHEME == 1, "per-plane kernel: PPM only");
     const ZMetrics &zm = g.zm;
     int bc5 = g.bcl[4], bc6 = g.bcl[5];
     double bc51 = (bc5 == 0) ? 1.0 : 0.0, bc61 = (bc6 == 0) ? 1.0 : 0.0;
-    if constexpr (SCHEME == 1) {
-        // cells k-1, k, k+1 (level nz carries only its Flux_l)
-        PPMRec cm{0.0, 0.0}, c0{0.0, 0.0}, cp{0.0, 0.0};
-        auto cell = [&](int c) -> PPMRec {
-            if (c == nz) {
-                PPMRec r;
-                r.fr = 0.0;
-                r.fl = ppm_ztop_fl(nz, zm, sp, Q, W);
-                return r;
-            }
-            return ppm_zcell(c, nz, zm, sp, bc51, bc61, Q, W);
-        };
-        bool need = (k >= 2 && k <= nz - 1) || (k == 1 && bc5 == 1) || (k == nz && bc6 == 1);
-        if (!need) return 0.0;
-        if (k >= 2) cm = cell(k - 1);
-        c0 = cell(k);
-        if (k <= nz - 1) cp = cell(k + 1);
-        auto face = [&](int f) -> double {
-            if (f == k - 1) return ppm_faceflux(cm.fr, c0.fl, W(k));
-            return ppm_faceflux(c0.fr, cp.fl, W(k + 1));
-        };
-        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
-    } else if constexpr (SCHEME == 2) {
-        auto face = [&](int f) -> double { return muscl_zface(f, nz, zm, sp, bc51, bc61, Q, W); };
-        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
-    } else if constexpr (SCHEME >= 4) {
-        auto face = [&](int f) -> double {
-            if constexpr (SCHEME == 4) return quick_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
-            else if constexpr (SCHEME == 6) return quick_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
-            else if constexpr (SCHEME == 5) return lw_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
-            else return lw_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
-        };
-        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
-    } else {
-        auto face = [&](int f) -> double { return weno3_zface(f, nz, sp.weno_bottom, Q, W); };
-        return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
-    }
+    // cells k-1, k, k+1 (level nz carries only its Flux_l)
+    PPMRec cm{0.0, 0.0}, c0{0.0, 0.0}, cp{0.0, 0.0};
+    auto cell = [&](int c) -> PPMRec {
+        if (c == nz) {
+            PPMRec r;
+            r.fr = 0.0;
+            r.fl = ppm_ztop_fl(nz, zm, sp, Q, W);
+            return r;
+        }
+        return ppm_zcell(c, nz, zm, sp, bc51, bc61, Q, W);
+    };
+    bool need = (k >= 2 && k <= nz - 1) || (k == 1 && bc5 == 1) || (k == nz && bc6 == 1);
+    if (!need) return 0.0;
+    if (k >= 2) cm = cell(k - 1);
+    c0 = cell(k);
+    if (k <= nz - 1) cp = cell(k + 1);
+    auto face = [&](int f) -> double {
+        if (f == k - 1) return ppm_faceflux(cm.fr, c0.fl, W(k));
+        return ppm_faceflux(c0.fr, cp.fl, W(k + 1));
+    };
+    return z_assemble(k, nz, adv, bc5, bc6, zm, face, Q, W);
 }
 
 template <int SCHEME, int TX, int TY>
@@ -125,8 +86,7 @@ __global__ __launch_bounds__(TX *TY) void scalar_fp64_kernel(GridDev g, SchemePa
     for (int t = threadIdx.y * TX + threadIdx.x; t < LH * LW; t += TX * TY) {
         int r = t / LW, c = t - r * LW;
         // positions outside the array are replaced by their periodic
-        // interior image; only WENO5 reads them (x(n+1)), other schemes'
-        // closures never do
+        // interior image; the PPM closures never read them
         int ii = i0 - 3 + c;
         if (ii < 1 || ii > nx) {
             int rr = (ii - 4) % mx;
@@ -233,14 +193,8 @@ static hipError_t launch_scheme(const GridDev &g, const SchemeParams &sp, const 
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s)
 {
     switch (sp.scheme) {
-    case 0: return launch_weno_march(g, sp, a, 0, s);
     case 1: return launch_scheme<1>(g, sp, a, s);
-    case 2: return launch_scheme<2>(g, sp, a, s);
-    case 3: return launch_weno_march(g, sp, a, 0, s);
-    case 4: return launch_scheme<4>(g, sp, a, s);
-    case 5: return launch_scheme<5>(g, sp, a, s);
-    case 6: return launch_scheme<6>(g, sp, a, s);
-    case 7: return launch_scheme<7>(g, sp, a, s);
+    case 0: case 2: case 3: case 4: case 5: case 6: case 7: return launch_march(g, sp, a, 0, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -1,11 +1,11 @@
-// schemes_fp64.hpp -- per-cell device formulas of the MIMICA advection
-// schemes, FP64, in the reference's floating-point operation order.
+// schemes_fp64.hpp -- per-face and per-cell device formulas of the MIMICA
+// advection schemes, FP64, in the reference's floating-point operation order.
 //
 // The reference computes each direction as a sequential line sweep
 // (face fluxes into a line array, then differences).  Here every formula
-// is a pure function of a cell or face index and accessors Q(i), U(i)
-// (1-based line indices), so a thread can evaluate the contribution of its
-// own cell without the line sweep.  Position-dependent closures (tile
+// is a pure function of a face or cell index and accessors Q(i), U(i)
+// (1-based line indices), so a thread can evaluate one face (the marching
+// kernels) or one cell (the PPM kernel) without the line sweep.  Position-dependent closures (tile
 // edges, bottom/top levels) are selected by index exactly where the
 // reference's loops switch formulas.  Built without FMA contraction these
 // reproduce the reference arithmetic bit for bit.
@@ -32,104 +32,6 @@ struct SchemeParams {
 __device__ __forceinline__ double dmaxd(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmind(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double sgn1(double x) { return copysign(1.0, x); }
-
-// ---------------------------------------------------------------------------
-// Reference WENO (advq_weno.f90)
-// ---------------------------------------------------------------------------
-
-// weight{x,y,z} for one face (advq_weno.f90:329-350)
-__device__ __forceinline__ void w3_weights(double a, double b, double c, double d, double ww[4])
-{
-    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0, eps = 1.e-15, thres = 20.0;
-    double eta1 = (b - a) * (b - a);
-    double eta2 = (c - b) * (c - b);
-    double eta3 = (d - c) * (d - c);
-    ww[0] = g1; ww[1] = g2; ww[2] = g2; ww[3] = g1;
-    if (dmaxd(eta1, eta2) / (dmind(eta1, eta2) + eps) > thres) {
-        double s1 = (eps + eta1) * (eps + eta1);
-        double s2 = (eps + eta2) * (eps + eta2);
-        double w1 = g1 / s1, w2 = g2 / s2;
-        double sumw = w1 + w2;
-        ww[0] = w1 / sumw;
-        ww[1] = w2 / sumw;
-    }
-    if (dmaxd(eta2, eta3) / (dmind(eta2, eta3) + eps) > thres) {
-        double s2 = (eps + eta2) * (eps + eta2);
-        double s3 = (eps + eta3) * (eps + eta3);
-        double w3 = g2 / s2, w4 = g1 / s3;
-        double sumw = w3 + w4;
-        ww[2] = w3 / sumw;
-        ww[3] = w4 / sumw;
-    }
-}
-
-// flux at face i+1/2 from a..d = x(i-1..i+2), uu = u(i+1) (advq_weno.f90:92-110)
-__device__ __forceinline__ double w3_face(double a, double b, double c, double d, double uu,
-                                          const double ww[4], bool limit)
-{
-    double ffim1 = -(0.5 * a) + 1.5 * b;
-    double ffip1p = 0.5 * b + 0.5 * c;
-    double ffip1m = ffip1p;
-    double ffip2 = 1.5 * c - 0.5 * d;
-    if (limit) {  // limiter_weno (advq_weno.f90:462-491)
-        const double eps = 1.e-15;
-        double minp = dmind(dmind(dmind(ffim1, ffip1m), ffip1p), ffip2);
-        double phi = fmin(fabs((0.0 - b) / (minp - b + eps)), 1.0);
-        ffim1 = phi * (ffim1 - b) + b;
-        ffip1m = phi * (ffip1m - b) + b;
-        ffip1p = phi * (ffip1p - b) + b;
-        ffip2 = phi * (ffip2 - b) + b;
-    }
-    double fl = ww[0] * ffim1 + ww[1] * ffip1p;
-    double fr = ww[2] * ffip1m + ww[3] * ffip2;
-    return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
-}
-
-template <class QF, class UF>
-__device__ __forceinline__ double weno3_hface(int f, QF Q, UF U)
-{
-    double a = Q(f - 1), b = Q(f), c = Q(f + 1), d = Q(f + 2);
-    double ww[4];
-    w3_weights(a, b, c, d, ww);
-    return w3_face(a, b, c, d, U(f + 1), ww, true);
-}
-
-// x/y contribution of cell i (advsx_weno, advq_weno.f90:61-131)
-template <class QF, class UF>
-__device__ __forceinline__ double weno3_h(int i, double d1, bool adv, QF Q, UF U)
-{
-    double t = (weno3_hface(i, Q, U) - weno3_hface(i - 1, Q, U)) * d1;
-    if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
-    return t;
-}
-
-// z face flux k+1/2 (advsz_weno, advq_weno.f90:224-271)
-template <class QF, class WF>
-__device__ __forceinline__ double weno3_zface(int f, int nz, int bottom, QF Q, WF W)
-{
-    const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
-    double www = W(f + 1);
-    if (f == 1 && bottom == 1) {  // bottom closure kept
-        double ffip1p = 0.5 * Q(1) + 0.5 * Q(2);
-        double ffip2 = 1.5 * Q(2) - 0.5 * Q(3);
-        double fl = ffip1p;
-        double fr = g2 * ffip1p + g1 * ffip2;
-        return 0.5 * fl * (www + fabs(www)) + 0.5 * fr * (www - fabs(www));
-    }
-    if (f == nz - 1) {  // top closure, no limiter
-        double ffim1 = -(0.5 * Q(f - 1)) + 1.5 * Q(f);
-        double ffip1p = 0.5 * Q(f) + 0.5 * Q(f + 1);
-        double fl = g1 * ffim1 + g2 * ffip1p;
-        double fr = ffip1p;
-        return 0.5 * fl * (www + fabs(www)) + 0.5 * fr * (www - fabs(www));
-    }
-    // loop body; at f = 1 the reference reads x(0), taken as x(1)
-    double a = (f == 1) ? Q(1) : Q(f - 1);
-    double b = Q(f), c = Q(f + 1), d = Q(f + 2);
-    double ww[4] = {g1, g2, g2, g1};
-    if (f >= 3 && f <= nz - 3) w3_weights(a, b, c, d, ww);
-    return w3_face(a, b, c, d, www, ww, true);
-}
 
 // ---------------------------------------------------------------------------
 // WENO5-JS (extension, no reference counterpart): Jiang & Shu (1996) left
@@ -208,14 +110,6 @@ __device__ __forceinline__ double muscl_hface(int i, double d, double d1, const 
         fl = xx + 0.5 * slope_p * d;
     }
     return muscl_upwind(uu, fl, fr);
-}
-
-template <class QF, class UF>
-__device__ __forceinline__ double muscl_h(int i, double d, double d1, bool adv, const SchemeParams &sp, QF Q, UF U)
-{
-    double t = (muscl_hface(i, d, d1, sp, Q, U) - muscl_hface(i - 1, d, d1, sp, Q, U)) * d1;
-    if (adv) t = t - Q(i) * (U(i + 1) - U(i)) * d1;
-    return t;
 }
 
 // z face flux k+1/2 (advsz_muscl, advq_muscl.f90:245-302)

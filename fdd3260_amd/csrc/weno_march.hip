@@ -1,14 +1,16 @@
-// weno_march.hip -- plain per-field FP64 kernel of the reference 'weno'
-// scheme (and of the WENO5-JS extension in x/y), column-marching form.
+// weno_march.hip -- plain per-field FP64 kernels, column-marching form: the
+// reference 'weno' scheme, the WENO5-JS extension in x/y, and the line
+// schemes MUSCL, QUICK and Lax-Wendroff with their unlimited forms (face
+// formulas of schemes_fp64.hpp).
 //
-// Same results as the generic per-plane kernel (scalar_fp64.hip) and the
-// CPU restatement, bit for bit: every face value is computed with the
-// reference operations (advq_weno.f90:92-110, weight{x,y,z} :329-350,
-// limiter_weno :462-491) in the reference order, only
+// Same results as the CPU restatement, bit for bit: every face value is
+// computed with the reference operations (for 'weno' advq_weno.f90:92-110,
+// weight{x,y,z} :329-350, limiter_weno :462-491) in the reference order,
+// only
 //   * once per face instead of once per adjacent cell: the x and y faces of
 //     a level are evaluated by a flattened loop over the workgroup and
 //     exchanged through LDS, the z face k+1/2 is carried to level k+1;
-//   * for the upwind side only.  The reference forms
+//   * for 'weno' and WENO5, for the upwind side only.  The reference forms
 //     F = 0.5*fl*(u+|u|) + 0.5*fr*(u-|u|), where the off-side term is an
 //     exact zero, and the two sides' weights and candidates are the same
 //     expressions with outer and inner swapped (fl = w1*f1 + w2*f2,
@@ -131,15 +133,28 @@ __device__ __forceinline__ double zface(int f, int nz, int bottom, double a, dou
     return weno_face_up(a, b, c, d, ww, f >= 3 && f <= nz - 3);
 }
 
-// x/y face value of the upwind side, from the LDS stencil starting at b
-// (step st): the reference 'weno' reads q(f-1..f+2) (4 values), WENO5 reads
-// q(f-2..f+3) (6 values).  As for 'weno', the reference-form sum
-// 0.5*fl*(u+|u|) + 0.5*fr*(u-|u|) has an exact zero off-side term.
-template <int XS>
-__device__ __forceinline__ double xy_face(const double *t, int b, int st, double uu)
+// x/y face flux from the LDS stencil starting at b (step st): the reference
+// 'weno' and the line schemes read q(f-1..f+2) (4 values), WENO5 reads
+// q(f-2..f+3) (6 values).  'weno' and WENO5 evaluate the upwind side only:
+// in the reference-form sum 0.5*fl*(u+|u|) + 0.5*fr*(u-|u|) the off-side
+// term is an exact zero.  The line schemes (MUSCL, QUICK, Lax-Wendroff and
+// the unlimited forms) evaluate the reference face formula as it stands;
+// they have no position-dependent closures in x and y, so the stencil is
+// passed as cells 0..3 of face 1+1/2.  d, d1: grid spacing and its inverse.
+template <int S>
+__device__ __forceinline__ double xy_face(const double *t, int b, int st, double uu, double d, double d1,
+                                          const SchemeParams &sp)
 {
-    if constexpr (XS == 0) {
+    if constexpr (S == 0) {
         return weno_face_up(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], uu, true);
+    } else if constexpr (S != 3) {
+        auto Q = [&](int m) { return t[b + m * st]; };
+        auto U = [&](int) { return uu; };
+        if constexpr (S == 2) return muscl_hface(1, d, d1, sp, Q, U);
+        else if constexpr (S == 4) return quick_hface<false>(1, sp, Q, U);
+        else if constexpr (S == 6) return quick_hface<true>(1, sp, Q, U);
+        else if constexpr (S == 5) return lw_hface<false>(1, d1, sp, Q, U);
+        else return lw_hface<true>(1, d1, sp, Q, U);
     } else {
         if (uu > 0.0) {
             const double fl = weno5_left(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], t[b + 4 * st]);
@@ -150,15 +165,29 @@ __device__ __forceinline__ double xy_face(const double *t, int b, int st, double
     }
 }
 
-// XS: 0 the reference 'weno' in x/y, 3 WENO5-JS in x/y (both with the
-// reference 'weno' in z)
-template <int TX, int TY, int XS>
-__device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a,
+// z face f + 1/2 of a line scheme; Q(kk) any level of the column, ww = w(f+1)
+template <int S, class QF>
+__device__ __forceinline__ double line_zface(int f, int nz, const ZMetrics &zm, const SchemeParams &sp, double bc51,
+                                             double bc61, QF Q, double ww)
+{
+    auto W = [&](int) { return ww; };
+    if constexpr (S == 2) return muscl_zface(f, nz, zm, sp, bc51, bc61, Q, W);
+    else if constexpr (S == 4) return quick_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
+    else if constexpr (S == 6) return quick_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
+    else if constexpr (S == 5) return lw_zface<false>(f, nz, zm, sp, bc51, bc61, Q, W);
+    else return lw_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
+}
+
+// S (scheme code): 0 the reference 'weno', 3 WENO5-JS in x/y with the
+// reference 'weno' in z, 2 MUSCL, 4 QUICK, 5 Lax-Wendroff, 6 and 7 their
+// unlimited forms
+template <int TX, int TY, int S>
+__device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a,
                                                 int kchunk, int ntx, int nwork)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
-    constexpr int H = XS == 3 ? 3 : 2;  // x/y halo of the LDS tile
+    constexpr int H = S == 3 ? 3 : 2;  // x/y halo of the LDS tile
     constexpr int LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1;
     constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
     constexpr int FS = (NF + NT - 1) / NT, TS = (LH * LW + NT - 1) / NT;
@@ -177,6 +206,7 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     const double dx1 = 1.0 / g.dx, dy1 = 1.0 / g.dy;
     const bool adv = a.adv != 0;
     const int bc5 = g.bcl[4], bc6 = g.bcl[5];
+    const double bc51 = bc5 == 0 ? 1.0 : 0.0, bc61 = bc6 == 0 ? 1.0 : 0.0;  // periodic floor, lid
     const double *__restrict__ dz1w = g.zm.dz1w;
     const bool zdir = (a.dirmask & 4) != 0;
     const bool accum = a.epi == EPI_ACCUM, heun = a.epi == EPI_HEUN, so = a.epi == EPI_SO;
@@ -279,7 +309,19 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
             }
             const double wc = k < k0 ? wz : wv;
             double fzt = 0.0;
-            if (zdir && k <= nz - 1) fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
+            if (zdir && k <= nz - 1) {
+                if constexpr (S == 0 || S == 3) {
+                    fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
+                } else {
+                    // q(k-1..k+2) from the window, other levels (the periodic
+                    // floor and lid closures) from memory
+                    auto Qw = [&](int kk) {
+                        const int m = kk - (k - 1);
+                        return m == 0 ? z0 : m == 1 ? z1 : m == 2 ? z2 : m == 3 ? z3 : Qz(kk);
+                    };
+                    fzt = line_zface<S>(k, nz, g.zm, sp, bc51, bc61, Qw, wc);
+                }
+            }
             if (k < k0) {
                 fzb = fzt;
                 continue;
@@ -299,7 +341,8 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
                 const int fs = f_slot(s), kind = fs >> 16;
                 if (kind == 0) continue;
                 const int st = kind == 1 ? 1 : LW, b = fs & 0xffff;
-                face[tid + s * NT] = xy_face<XS>(tile, b, st, fvel[tid + s * NT]);
+                face[tid + s * NT] = xy_face<S>(tile, b, st, fvel[tid + s * NT], kind == 1 ? g.dx : g.dy,
+                                                kind == 1 ? dx1 : dy1, sp);
             }
             __syncthreads();
             if (valid) {
@@ -382,7 +425,7 @@ __device__ __forceinline__ void weno_march_body(const GridDev &g, const SchemePa
     }
 }
 
-// Both instances are capped at 128 VGPRs (4 waves per SIMD); the per-thread
+// All instances are capped at 128 VGPRs (4 waves per SIMD); the per-thread
 // slot constants live in LDS to make room.  Uncapped, the reference-weno
 // 32x8 instance took 151 VGPRs (3 waves) and 217 us per launch at
 // 256x256x128; WENO5 took 170 (2 waves).
@@ -390,17 +433,24 @@ template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno_march_kernel(
     GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
 {
-    weno_march_body<TX, TY, 0>(g, sp, a, kchunk, ntx, nwork);
+    march_body<TX, TY, 0>(g, sp, a, kchunk, ntx, nwork);
 }
 
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void weno5_march_kernel(
     GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
 {
-    weno_march_body<TX, TY, 3>(g, sp, a, kchunk, ntx, nwork);
+    march_body<TX, TY, 3>(g, sp, a, kchunk, ntx, nwork);
 }
 
-template <int TX, int TY, int XS, int KC>
+template <int S, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void line_march_kernel(
+    GridDev g, SchemeParams sp, ScalarArgs a, int kchunk, int ntx, int nwork)
+{
+    march_body<TX, TY, S>(g, sp, a, kchunk, ntx, nwork);
+}
+
+template <int TX, int TY, int S, int KC>
 hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
     const int ntx = (g.nx - 5 + TX - 1) / TX;
@@ -409,12 +459,15 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
     if (kchunk > g.nz) kchunk = g.nz;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     const int nwork = ntx * nty * nchunk;
-    if constexpr (XS == 3)
+    if constexpr (S == 3)
         hipLaunchKernelGGL((weno5_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
                            nwork);
-    else
+    else if constexpr (S == 0)
         hipLaunchKernelGGL((weno_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk, ntx,
                            nwork);
+    else
+        hipLaunchKernelGGL((line_march_kernel<S, TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, sp, a, kchunk,
+                           ntx, nwork);
     return hipGetLastError();
 }
 
@@ -425,14 +478,21 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
 // per launch; 64x4 took 228: 2.27 x/y faces per cell instead of 2.16 and a
 // wider halo), WENO5 on 16x16 with 16-level chunks (285 us; 32x8 takes 308,
 // 64x4 323).  Tiles of 512 threads are slower (fewer waves per SIMD).
-hipError_t launch_weno_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
+// The line schemes use the reference-weno tiles.
+hipError_t launch_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
-    if (sp.scheme == 3) {
-        if (g.ny > 1) return launch_tile<16, 16, 3, 16>(g, sp, a, kchunk, s);
-        return launch_tile<64, 1, 3, 16>(g, sp, a, kchunk, s);
+    const bool three_d = g.ny > 1;
+    switch (sp.scheme) {
+    case 0: return three_d ? launch_tile<32, 8, 0, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 0, 16>(g, sp, a, kchunk, s);
+    case 3:
+        return three_d ? launch_tile<16, 16, 3, 16>(g, sp, a, kchunk, s) : launch_tile<64, 1, 3, 16>(g, sp, a, kchunk, s);
+    case 2: return three_d ? launch_tile<32, 8, 2, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 2, 16>(g, sp, a, kchunk, s);
+    case 4: return three_d ? launch_tile<32, 8, 4, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 4, 16>(g, sp, a, kchunk, s);
+    case 5: return three_d ? launch_tile<32, 8, 5, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 5, 16>(g, sp, a, kchunk, s);
+    case 6: return three_d ? launch_tile<32, 8, 6, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 6, 16>(g, sp, a, kchunk, s);
+    case 7: return three_d ? launch_tile<32, 8, 7, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 7, 16>(g, sp, a, kchunk, s);
+    default: return hipErrorInvalidValue;
     }
-    if (g.ny > 1) return launch_tile<32, 8, 0, 8>(g, sp, a, kchunk, s);
-    return launch_tile<64, 1, 0, 16>(g, sp, a, kchunk, s);
 }
 
 }  // namespace mimadv
