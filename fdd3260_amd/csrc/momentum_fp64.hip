@@ -301,10 +301,14 @@ __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs 
 // within two cells of the tile) sits in a five-level LDS ring of FP32
 // tiles with a 2-cell halo; each level is loaded once, coalesced,
 // and serves five output levels.  The per-cell formulas are those of the
-// FP64 kernel (mom_cell), so results do not depend on the mapping.
+// FP64 kernel (mom_cell), so results do not depend on the mapping.  One
+// work item per workgroup.  Tiles of 32x8 columns keep the ring at 51.8 KB,
+// so three groups share a CU; the 168-VGPR cap that gives each SIMD three
+// waves spills a few values and still wins (2.9 -> 2.5 ms per stage at
+// 512x512x160 against 64x4 tiles at two waves).
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void momentum_march_kernel(GridDev g, MomCoefs c, int lavisc, MomBatchArgs a,
-                                                                int kchunk, int ntx, int nwork)
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) void momentum_march_kernel(
+    GridDev g, MomCoefs c, int lavisc, MomBatchArgs a, int kchunk, int ntx, int nwork)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(TX *TY) void momentum_march_kernel(GridDev g, MomCo
     const int nchunk = (nz + kchunk - 1) / kchunk;
     const float *src[6] = {a.m[0], a.m[1], a.m[2], a.ut, a.vt, a.wt};
 
-    for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    if (const int wi = blockIdx.x; wi < nwork) {
         const int tile = wi / nchunk, kc = wi - tile * nchunk;
         const int i0 = 4 + (tile % ntx) * TX;
         const int j0 = THREE_D ? 4 + (tile / ntx) * TY : 1;
@@ -467,7 +471,7 @@ hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, cons
     const int kchunk = g.nz < 16 ? g.nz : 16;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     if (g.ny > 1) {
-        constexpr int TX = 64, TY = 4;
+        constexpr int TX = 32, TY = 8;
         const int ntx = (g.nx - 5 + TX - 1) / TX, nty = (g.ny - 5 + TY - 1) / TY;
         const int nwork = ntx * nty * nchunk;
         hipLaunchKernelGGL((momentum_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, c, lavisc, a, kchunk,
