@@ -305,7 +305,8 @@ __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs 
 // work item per workgroup.  Tiles of 32x8 columns keep the ring at 51.8 KB,
 // so three groups share a CU; the 168-VGPR cap that gives each SIMD three
 // waves spills a few values and still wins (2.9 -> 2.5 ms per stage at
-// 512x512x160 against 64x4 tiles at two waves).
+// 512x512x160 against 64x4 tiles at two waves).  The ring offsets are
+// recomputed per load rather than held, which halves the spills.
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) void momentum_march_kernel(
     GridDev g, MomCoefs c, int lavisc, MomBatchArgs a, int kchunk, int ntx, int nwork)
@@ -329,28 +330,25 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
         const int i = i0 + tx, j = j0 + ty;
         const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
-        // plane offsets of this thread's ring entries (periodic images for
-        // positions outside the interior; -1: no entry)
-        int t_off[TS];
-#pragma unroll
-        for (int s = 0; s < TS; ++s) {
+        // plane offset of ring entry s of this thread (periodic images for
+        // positions outside the interior; -1: no entry), recomputed where it
+        // is used rather than held in registers through the level loop
+        auto t_off = [&](int s) -> int {
             const int e = tid + s * NT;
-            t_off[s] = -1;
-            if (e < PL) {
-                const int r = e / LW, cc = e - r * LW;
-                // ghost columns are read as the caller filled them, like the
-                // reference; positions past the array (padding of a partial
-                // tile) take a periodic image that feeds no interior cell
-                int ii = i0 - H + cc;
-                if (ii > nx) ii = (ii - 4) % mx + 4;
-                int jj = 1;
-                if (THREE_D) {
-                    jj = j0 - H + r;
-                    if (jj > ny) jj = (jj - 4) % my + 4;
-                }
-                t_off[s] = (ii - 1) + nx * (jj - 1);
+            if (e >= PL) return -1;
+            const int r = e / LW, cc = e - r * LW;
+            // ghost columns are read as the caller filled them, like the
+            // reference; positions past the array (padding of a partial
+            // tile) take a periodic image that feeds no interior cell
+            int ii = i0 - H + cc;
+            if (ii > nx) ii = (ii - 4) % mx + 4;
+            int jj = 1;
+            if (THREE_D) {
+                jj = j0 - H + r;
+                if (jj > ny) jj = (jj - 4) % my + 4;
             }
-        }
+            return (ii - 1) + nx * (jj - 1);
+        };
         // level kk of the six operands into registers (zeros outside 1..nz),
         // then into its ring slot
         float pre[6][TS];
@@ -358,15 +356,17 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             const bool in = kk >= 1 && kk <= nz;
             const size_t kb = in ? sz * (kk - 1) : 0;
 #pragma unroll
-            for (int s = 0; s < TS; ++s)
+            for (int s = 0; s < TS; ++s) {
+                const int off = t_off(s);
 #pragma unroll
-                for (int n = 0; n < 6; ++n) pre[n][s] = (in && t_off[s] >= 0) ? src[n][kb + t_off[s]] : 0.0f;
+                for (int n = 0; n < 6; ++n) pre[n][s] = (in && off >= 0) ? src[n][kb + off] : 0.0f;
+            }
         };
         auto store = [&](int kk) {
             const int slot = (kk + 2 * NR) % NR;
 #pragma unroll
             for (int s = 0; s < TS; ++s) {
-                if (t_off[s] < 0) continue;
+                if (tid + s * NT >= PL) continue;
 #pragma unroll
                 for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = pre[n][s];
             }
