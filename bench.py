@@ -41,7 +41,7 @@ HBM_TRAFFIC = {
     ("fused", "512,512,160", "weno", "ssprk3", 20): 18.84e9,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1023.6e6,
 }
-SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3}
+SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
 
 
 def log(*a):
@@ -428,7 +428,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--scheme", default="weno", choices=["weno", "ppm", "muscl", "weno5"])
+    ap.add_argument("--scheme", default="weno", choices=["weno", "ppm", "muscl", "weno5", "quick", "lw"])
     ap.add_argument("--limiter", default="MC")
     ap.add_argument("--integrator", default="heun", choices=["heun", "ssprk3"])
     ap.add_argument("--size", default="256,256,128")
@@ -508,7 +508,8 @@ def main():
         vsteps = max(1, min(args.steps, 20))
         for name, scheme, lim, integ in (("weno/ssprk3", "weno", "MC", "ssprk3"), ("ppm/heun", "ppm", "MC", "heun"),
                                          ("muscl-MC/heun", "muscl", "MC", "heun"),
-                                         ("weno5/ssprk3", "weno5", "MC", "ssprk3")):
+                                         ("weno5/ssprk3", "weno5", "MC", "ssprk3"), ("quick/heun", "quick", "MC", "heun"),
+                                         ("lw/heun", "lw", "MC", "heun")):
             va = Advection(g, scal_adv=scheme, muscllimiter=lim)
             vq = [torch.from_numpy(f).to(dev) for f in fields]
             vcode = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
@@ -568,7 +569,8 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": {"weno": "weno_march_kernel<32,8>", "weno5": "weno5_march_kernel<16,16>",
-                       "muscl": "line_march_kernel<2,32,8>", "ppm": "scalar_fp64_kernel<1,64,4>"}[args.scheme],
+                       "muscl": "line_march_kernel<2,32,8>", "quick": "line_march_kernel<4,32,8>",
+                       "lw": "line_march_kernel<5,32,8>", "ppm": "scalar_fp64_kernel<1,64,4>"}[args.scheme],
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

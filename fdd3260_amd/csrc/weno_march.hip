@@ -471,26 +471,31 @@ hipError_t launch_tile(const GridDev &g, const SchemeParams &sp, const ScalarArg
     return hipGetLastError();
 }
 
-}  // namespace
-
 // Tiles, measured at 256x256x128 before the VGPR cap of the reference-weno
 // instance: the reference 'weno' on 32x8 columns with 8-level chunks (217 us
 // per launch; 64x4 took 228: 2.27 x/y faces per cell instead of 2.16 and a
 // wider halo), WENO5 on 16x16 with 16-level chunks (285 us; 32x8 takes 308,
 // 64x4 323).  Tiles of 512 threads are slower (fewer waves per SIMD).
 // The line schemes use the reference-weno tiles.
+template <int S, int TX3, int TY3, int KC3>
+hipError_t launch_dims(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
+{
+    if (g.ny > 1) return launch_tile<TX3, TY3, S, KC3>(g, sp, a, kchunk, s);
+    return launch_tile<64, 1, S, 16>(g, sp, a, kchunk, s);
+}
+
+}  // namespace
+
 hipError_t launch_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {
-    const bool three_d = g.ny > 1;
     switch (sp.scheme) {
-    case 0: return three_d ? launch_tile<32, 8, 0, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 0, 16>(g, sp, a, kchunk, s);
-    case 3:
-        return three_d ? launch_tile<16, 16, 3, 16>(g, sp, a, kchunk, s) : launch_tile<64, 1, 3, 16>(g, sp, a, kchunk, s);
-    case 2: return three_d ? launch_tile<32, 8, 2, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 2, 16>(g, sp, a, kchunk, s);
-    case 4: return three_d ? launch_tile<32, 8, 4, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 4, 16>(g, sp, a, kchunk, s);
-    case 5: return three_d ? launch_tile<32, 8, 5, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 5, 16>(g, sp, a, kchunk, s);
-    case 6: return three_d ? launch_tile<32, 8, 6, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 6, 16>(g, sp, a, kchunk, s);
-    case 7: return three_d ? launch_tile<32, 8, 7, 8>(g, sp, a, kchunk, s) : launch_tile<64, 1, 7, 16>(g, sp, a, kchunk, s);
+    case 0: return launch_dims<0, 32, 8, 8>(g, sp, a, kchunk, s);
+    case 3: return launch_dims<3, 16, 16, 16>(g, sp, a, kchunk, s);
+    case 2: return launch_dims<2, 32, 8, 8>(g, sp, a, kchunk, s);
+    case 4: return launch_dims<4, 32, 8, 8>(g, sp, a, kchunk, s);
+    case 5: return launch_dims<5, 32, 8, 8>(g, sp, a, kchunk, s);
+    case 6: return launch_dims<6, 32, 8, 8>(g, sp, a, kchunk, s);
+    case 7: return launch_dims<7, 32, 8, 8>(g, sp, a, kchunk, s);
     default: return hipErrorInvalidValue;
     }
 }
