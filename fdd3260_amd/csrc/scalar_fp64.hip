@@ -11,9 +11,11 @@
 //
 // Work mapping: a workgroup of TX x TY threads owns a TX x TY tile of one
 // k-plane.  The plane tile plus a 3-cell halo in x and y is staged in LDS
-// with coalesced x-fastest loads; the x and y stencils read LDS, the z
-// stencil reads the thread's own column (coalesced across the wavefront,
-// which spans 64 consecutive x).  Tiles for neighbouring k of the same
+// with coalesced x-fastest loads.  The x and y cell records (Flux_r,
+// Flux_l of limiter_ppm and the CW84 averages) of the tile's cells and one
+// neighbour on each side are computed once each into LDS, and every cell
+// takes its x and y fluxes from them; the z records of levels k-1..k+1 come
+// from the thread's own column (coalesced across the wavefront).  Tiles for neighbouring k of the same
 // (x,y) tile have linear block ids 256-apart-multiples, so they land on the
 // same XCD and share its L2 for the column reads.
 //
@@ -22,17 +24,6 @@
 #include "mimadv_internal.hpp"
 
 namespace mimadv {
-
-// x/y contribution of cell i (PPM; the other schemes run on the column-
-// marching kernels of weno_march.hip)
-template <int SCHEME, class QF, class UF>
-__device__ __forceinline__ double h_contrib(int i, int n, double d, bool adv, int bc_hi, const SchemeParams &sp,
-                                            QF Q, UF U)
-{
-    static_assert(SCHEME == 1, "per-plane kernel: PPM only");
-    double d1 = 1.0 / d;
-    return ppm_h(i, n, d1, adv, bc_hi, sp, Q, U);
-}
 
 // z contribution of cell k (PPM)
 template <int SCHEME, class QF, class WF>
@@ -71,9 +62,14 @@ __global__ __launch_bounds__(TX *TY) void scalar_fp64_kernel(GridDev g, SchemePa
 {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     constexpr bool THREE_D = TY > 1;
+    constexpr int NT = TX * TY;
     constexpr int LW = TX + 6;
     constexpr int LH = THREE_D ? TY + 6 : 1;
+    // cell records (Flux_r, Flux_l) of the x cells i0-1 .. i0+TX of each row
+    // and the y cells j0-1 .. j0+TY of each column, each computed once
+    constexpr int NXR = TY * (TX + 2), NYR = THREE_D ? (TY + 2) * TX : 0, NR = NXR + NYR;
     __shared__ double tile[LH][LW];
+    __shared__ double rfr[NR], rfl[NR];
 
     const int k = blockIdx.z + 1;
     const int i0 = 4 + blockIdx.x * TX;
@@ -104,6 +100,33 @@ __global__ __launch_bounds__(TX *TY) void scalar_fp64_kernel(GridDev g, SchemePa
     }
     __syncthreads();
 
+    // positions past the array (padding of a partial tile) read a periodic
+    // image; their records feed no interior cell
+    auto wrap_hi = [](int x, int n, int m) { return x > n ? (x - 4) % m + 4 : x; };
+    for (int e = threadIdx.y * TX + threadIdx.x; e < NR; e += NT) {
+        PPMRec rec;
+        if (e < NXR) {
+            if (!(a.dirmask & 1)) continue;
+            const int rr = e / (TX + 2), c = i0 - 1 + (e - rr * (TX + 2));
+            const int jr = THREE_D ? wrap_hi(j0 + rr, ny, my) : 1;
+            const double *__restrict__ urow = a.u + kbase + (size_t)nx * (jr - 1);
+            const int lr = THREE_D ? rr + 3 : 0;
+            auto Q = [&](int ii) { return tile[lr][ii - (i0 - 3)]; };
+            auto U = [&](int ii) { return urow[wrap_hi(ii, nx, mx) - 1]; };
+            rec = ppm_hcell(c, nx, 1.0 / g.dx, sp, Q, U);
+        } else {
+            if (!(THREE_D && (a.dirmask & 2))) continue;
+            const int e2 = e - NXR, rr = e2 / TX, cc = e2 - rr * TX, c = j0 - 1 + rr;
+            const double *__restrict__ vcol = a.v + kbase + (size_t)(wrap_hi(i0 + cc, nx, mx) - 1);
+            auto Q = [&](int jj) { return tile[jj - (j0 - 3)][cc + 3]; };
+            auto V = [&](int jj) { return vcol[(size_t)nx * (wrap_hi(jj, ny, my) - 1)]; };
+            rec = ppm_hcell(c, ny, 1.0 / g.dy, sp, Q, V);
+        }
+        rfr[e] = rec.fr;
+        rfl[e] = rec.fl;
+    }
+    __syncthreads();
+
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i = i0 + tx, j = j0 + ty;
     if (i > nx - 2 || (THREE_D && j > ny - 2)) return;
@@ -113,17 +136,35 @@ __global__ __launch_bounds__(TX *TY) void scalar_fp64_kernel(GridDev g, SchemePa
     const int lrow = THREE_D ? ty + 3 : 0;
 
     double X = 0.0, Y = 0.0, Z = 0.0;
+    const double qc = tile[lrow][tx + 3];
     if (a.dirmask & 1) {
+        // ppm_h (advq_ppm.f90:136-163) on the records of cells i-1, i, i+1
         const double *__restrict__ urow = a.u + kbase + (size_t)nx * (j - 1);
-        auto Q = [&](int ii) { return tile[lrow][ii - (i0 - 3)]; };
-        auto U = [&](int ii) { return urow[ii - 1]; };
-        X = h_contrib<SCHEME>(i, nx, g.dx, adv, g.bcl[1], sp, Q, U);
+        const double d1 = 1.0 / g.dx, ui = urow[i - 1], uip = urow[i];
+        const int e = ty * (TX + 2) + tx;
+        double fr_i = rfr[e + 1], fl_ip = rfl[e + 2];
+        if (g.bcl[1] != 0 && i == nx - 2) {
+            fr_i = rfl[e + 1];  // Flux_r(ii2) = Flux_l(ii2)
+            fl_ip = rfr[e];     // Flux_l(ii2+1) = Flux_r(ii2-1)
+        }
+        const double flux_old = ppm_faceflux(rfr[e], rfl[e + 1], ui);
+        const double flux = ppm_faceflux(fr_i, fl_ip, uip);
+        X = (flux - flux_old) * d1;
+        if (adv) X = X - qc * (uip - ui) * d1;
     }
     if (THREE_D && (a.dirmask & 2)) {
         const double *__restrict__ vcol = a.v + kbase + (size_t)(i - 1);
-        auto Q = [&](int jj) { return tile[jj - (j0 - 3)][tx + 3]; };
-        auto V = [&](int jj) { return vcol[(size_t)nx * (jj - 1)]; };
-        Y = h_contrib<SCHEME>(j, ny, g.dy, adv, g.bcl[3], sp, Q, V);
+        const double d1 = 1.0 / g.dy, vj = vcol[(size_t)nx * (j - 1)], vjp = vcol[(size_t)nx * j];
+        const int e = NXR + ty * TX + tx;
+        double fr_j = rfr[e + TX], fl_jp = rfl[e + 2 * TX];
+        if (g.bcl[3] != 0 && j == ny - 2) {
+            fr_j = rfl[e + TX];
+            fl_jp = rfr[e];
+        }
+        const double flux_old = ppm_faceflux(rfr[e], rfl[e + TX], vj);
+        const double flux = ppm_faceflux(fr_j, fl_jp, vjp);
+        Y = (flux - flux_old) * d1;
+        if (adv) Y = Y - qc * (vjp - vj) * d1;
     }
     if (a.dirmask & 4) {
         const double *__restrict__ qc = a.q + col;
