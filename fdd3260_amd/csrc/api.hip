@@ -108,7 +108,7 @@ int ensure_stage(mimadv_ctx *c, size_t count)
 {
     size_t need = c->n3() * count;
     if (need <= c->stage_elems) return MIMADV_OK;
-    if (c->stage) hipFree(c->stage);
+    if (c->stage) (void)hipFree(c->stage);
     c->stage = nullptr;
     c->stage_elems = 0;
     hipError_t e = hipMalloc(&c->stage, need * sizeof(double));
@@ -121,18 +121,12 @@ int ensure_work(mimadv_ctx *c, size_t count)
 {
     size_t need = c->n3() * count;
     if (need <= c->work_elems) return MIMADV_OK;
-    if (c->work) hipFree(c->work);
+    if (c->work) (void)hipFree(c->work);
     c->work = nullptr;
     c->work_elems = 0;
     hipError_t e = hipMalloc(&c->work, need * sizeof(double));
     if (e != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "work allocation failed");
     c->work_elems = need;
-    return MIMADV_OK;
-}
-
-int check_ptr_dev(mimadv_ctx *c, const void *p, const char *name)
-{
-    if (!p) return fail(c, MIMADV_ERR_ARG, std::string(name) + " is NULL");
     return MIMADV_OK;
 }
 
@@ -293,20 +287,20 @@ int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx *
 void mimadv_destroy(mimadv_ctx *c)
 {
     if (!c) return;
-    if (c->own_stream) hipStreamSynchronize(c->own_stream);
-    if (c->stream && c->stream != c->own_stream) hipStreamSynchronize(c->stream);
-    if (c->d_metrics) hipFree(c->d_metrics);
-    if (c->d_red) hipFree(c->d_red);
-    if (c->stage) hipFree(c->stage);
-    if (c->work) hipFree(c->work);
-    if (c->bwork) hipFree(c->bwork);
-    if (c->adj) hipFree(c->adj);
-    if (c->d_den0) hipFree(c->d_den0);
-    if (c->d_w0) hipFree(c->d_w0);
-    if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    if (c->stream && c->stream != c->own_stream) (void)hipStreamSynchronize(c->stream);
+    if (c->d_metrics) (void)hipFree(c->d_metrics);
+    if (c->d_red) (void)hipFree(c->d_red);
+    if (c->stage) (void)hipFree(c->stage);
+    if (c->work) (void)hipFree(c->work);
+    if (c->bwork) (void)hipFree(c->bwork);
+    if (c->adj) (void)hipFree(c->adj);
+    if (c->d_den0) (void)hipFree(c->d_den0);
+    if (c->d_w0) (void)hipFree(c->d_w0);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
 
@@ -400,7 +394,7 @@ int mimadv_scalar_tendency(mimadv_ctx *c, double dt0, const double *u, const dou
             HIPCHK(c, hipMemcpyAsync(xs[f], sx, bytes, hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
             float ms = 0.0f;
-            hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
             kms_total += ms;
         } else {
             a.q = q[f];
@@ -444,7 +438,7 @@ int mimadv_momentum_tendency(mimadv_ctx *c, int ntime, const double *mu, const d
         HIPCHK(c, hipMemcpyAsync(wa, s + 8 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         c->last_ms = ms;
         return MIMADV_OK;
     }
@@ -761,11 +755,11 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         bytes += 2 * n3 * es;
     }
     if (bytes > c->bwork_bytes) {
-        if (c->bwork) hipFree(c->bwork);
+        if (c->bwork) (void)hipFree(c->bwork);
         c->bwork = nullptr;
         c->bwork_bytes = 0;
         if (c->graph_exec) {
-            hipGraphExecDestroy(c->graph_exec);
+            (void)hipGraphExecDestroy(c->graph_exec);
             c->graph_exec = nullptr;
         }
         if (hipMalloc(&c->bwork, bytes) != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "batch scratch allocation failed");
@@ -844,7 +838,7 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         key.push_back((const void *)dbits);
         if (!c->graph_exec || key != c->graph_key) {
             if (c->graph_exec) {
-                hipGraphExecDestroy(c->graph_exec);
+                (void)hipGraphExecDestroy(c->graph_exec);
                 c->graph_exec = nullptr;
             }
             hipGraph_t graph = nullptr;
@@ -854,7 +848,7 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
             if (le != hipSuccess) return hipfail(c, le, "launch during capture");
             if (ce != hipSuccess) return hipfail(c, ce, "hipStreamEndCapture");
             hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             if (ie != hipSuccess) return hipfail(c, ie, "hipGraphInstantiate");
             c->graph_key = key;
         }
