@@ -87,7 +87,7 @@ def test_momentum(cuda, ntime, mom_ord, lavisc, bc, dims):
             assert relmax(outs[2].cpu().numpy(), ref[2]) <= TOL
 
 
-@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "ppm", "muscl", "quick", "lw", "weno5"])
 @pytest.mark.parametrize("integrator", ["heun", "ssprk3"])
 def test_advance_matches_oracle_steps(cuda, scheme, integrator):
     print(f"case advance {scheme} {integrator}", flush=True)
