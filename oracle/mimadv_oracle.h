@@ -23,7 +23,6 @@
  *
  * Pins: every routine is checked against closed forms, invariants or
  * hand-derived known answers in tests/test_oracle_pins.py, except
- *   orc_fill_winds               -- parity unpinned (not on the measured path);
  *   advsz_ppm on a stretched grid -- parity partly unpinned (its face
  *                                   formula is pinned only in the uniform
  *                                   limit and by conservation).

@@ -670,6 +670,58 @@ def test_fill_scalar_is_periodic():
     assert np.array_equal(x[:, :g.ny - my, :], x[:, my:, :])
 
 
+def _periodic_extension(g, base):
+    """base (nz, my, mx) on the interior, extended to every index with the
+    periodic image map x(1:3) = x(nx-4:nx-2), x(nx-1:nx) = x(4:5)
+    (bc.f90:407-415), i.e. index i -> (i - 4) mod mx, same in y."""
+    ii = (np.arange(1, g.nx + 1) - 4) % g.maxx
+    jj = (np.arange(1, g.ny + 1) - 4) % g.maxy if g.ny > 1 else np.zeros(1, dtype=int)
+    return base[:, jj][:, :, ii]
+
+
+def test_fill_winds_doubly_periodic():
+    """orc_fill_winds (bc.f90:81-144) for periodic x and y sides: ghosts of
+    u, v, w start as NaN, and afterwards every element, corners included,
+    equals the periodic extension of the interior."""
+    g = gm.uniform(17, 13, 9)
+    rng = np.random.default_rng(11)
+    expect = [_periodic_extension(g, rng.standard_normal((g.nz, g.maxy, g.maxx))) for _ in range(3)]
+    arrs = [np.full(g.shape, np.nan) for _ in range(3)]
+    for a, e in zip(arrs, expect):
+        a[:, 3:g.ny - 2, 3:g.nx - 2] = e[:, 3:g.ny - 2, 3:g.nx - 2]
+    Oracle(g).fill_winds(*arrs)
+    for a, e in zip(arrs, expect):
+        assert np.array_equal(a, e)
+
+
+def test_fill_winds_2d_and_open_sides():
+    """2D (ny = 1): u and w take their x images and v is not touched; with
+    open x sides and periodic y sides only the y ghosts are filled."""
+    g = gm.uniform(17, 0, 9)
+    rng = np.random.default_rng(12)
+    eu, ew = (_periodic_extension(g, rng.standard_normal((g.nz, 1, g.maxx))) for _ in range(2))
+    u, w = np.full(g.shape, np.nan), np.full(g.shape, np.nan)
+    u[:, :, 3:g.nx - 2], w[:, :, 3:g.nx - 2] = eu[:, :, 3:g.nx - 2], ew[:, :, 3:g.nx - 2]
+    v = rng.standard_normal(g.shape)
+    v0 = v.copy()
+    Oracle(g).fill_winds(u, v, w)
+    assert np.array_equal(u, eu) and np.array_equal(w, ew) and np.array_equal(v, v0)
+
+    g = gm.uniform(17, 13, 9, bcl=("ope", "ope", "per", "per", "nnf", "nnf"))
+    x = rng.standard_normal(g.shape)
+    x[:, :, :3] = np.nan
+    x[:, :, g.nx - 2:] = np.nan
+    x[:, :3, :] = np.nan
+    x[:, g.ny - 2:, :] = np.nan
+    ref = x.copy()
+    ref[:, :3, :] = ref[:, g.ny - 5:g.ny - 2, :]
+    ref[:, g.ny - 2:, :] = ref[:, 3:5, :]
+    arrs = [x.copy() for _ in range(3)]
+    Oracle(g).fill_winds(*arrs)
+    for a in arrs:
+        assert np.array_equal(a, ref, equal_nan=True)
+
+
 def test_totalav_is_interior_mean():
     g = gm.uniform(17, 13, 9)
     rng = np.random.default_rng(10)
