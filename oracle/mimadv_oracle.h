@@ -22,10 +22,9 @@
  * SSP-RK3 stage coefficients), which are pinned only by closed forms.
  *
  * Pins: every routine is checked against closed forms, invariants or
- * hand-derived known answers in tests/test_oracle_pins.py, except
- *   advsz_ppm on a stretched grid -- parity partly unpinned (its face
- *                                   formula is pinned only in the uniform
- *                                   limit and by conservation).
+ * hand-derived known answers in tests/test_oracle_pins.py (advsz_ppm on a
+ * stretched grid through the Colella-Woodward interface formula on the grid
+ * whose widths are dz1w, the metric the reference passes in).
  *
  * Arrays are Fortran column-major with x fastest: element (i,j,k), 1-based,
  * lives at [(i-1) + nx*((j-1) + ny*(k-1))].  nx = MAXX+5, ny = MAXY+5 (or 1

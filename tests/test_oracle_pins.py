@@ -776,6 +776,30 @@ def test_ppm_vertical_uniform_grid_quadratic_exact():
         assert xa[c - 1] == pytest.approx(F(c) - F(c - 1), rel=1e-10, abs=1e-12)
 
 
+@pytest.mark.parametrize("degree", [2, 3])
+def test_ppm_vertical_stretched_face_formula(degree):
+    """On a stretched grid advsz_ppm's core face value (advq_ppm.f90:379-383)
+    is the Colella-Woodward (1984, eq. 1.6-1.7) interface formula evaluated
+    with delta(k) = dz1w(k) in place of the zone width.  That formula is the
+    derivative of the quartic through the cumulative integral, exact for cell
+    averages of a cubic.  So on the grid whose widths are dz1w(k), with cell
+    averages of a polynomial p, the limiter off and dt0 = 0 (Flux_r = Flux_l
+    = the face value), every core face carries p at that face, and the core
+    tendency is (w(k+1) p(xi_{k+1/2}) - w(k) p(xi_{k-1/2})) dz1w(k)."""
+    g = gm.stretched(10, 0, 40)
+    d = np.asarray(g.dz1w, dtype=float)
+    xi = np.concatenate([[0.0], np.cumsum(d)])  # xi[k] = top face of cell k (1-based)
+    cs = [0.7, -1.3, 0.4, 0.05][:degree + 1]
+    p = lambda t: sum(c * t ** n for n, c in enumerate(cs))
+    P = lambda t: sum(c * t ** (n + 1) / (n + 1) for n, c in enumerate(cs))
+    x = (P(xi[1:]) - P(xi[:-1])) / d
+    w = np.random.default_rng(13).uniform(-1, 1, g.nz)
+    xa = Oracle(g, scal_adv="ppm", ppmlimiter="none").line_z(0.0, x, w)
+    for k in range(4, g.nz - 1):
+        expect = (w[k] * p(xi[k]) - w[k - 1] * p(xi[k - 1])) * d[k - 1]
+        assert xa[k - 1] == pytest.approx(expect, rel=1e-9, abs=1e-12)
+
+
 def test_ppm_nonperiodic_right_edge_fixup():
     """bcl(2) /= 'per' (advq_ppm.f90:143-146): Flux_r(n-2) := Flux_l(n-2), so
     for linear data (limiter off, dt0 = 0) the last interior face carries
