@@ -294,6 +294,26 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                 for (int s = 0; s < FS; ++s) vc[s] = fv[s];
                 __syncthreads();
                 if (k < k1) fetch(k + 1);
+                // the velocity adjoints of the faces this thread owns, requested
+                // before the face arithmetic that updates them
+                double vbo[FS];
+                bool own[FS];
+#pragma unroll
+                for (int s = 0; s < FS; ++s) {
+                    const int e = tid + s * NT;
+                    own[s] = false;
+                    vbo[s] = 0.0;
+                    if (e >= NF) continue;
+                    if (!THREE_D || e < NXF) {
+                        const int r = e / NFX, f = i0 - 2 + (e - r * NFX);
+                        own[s] = f >= i0 && f < i0 + TX && f <= nx - 2 && (!THREE_D || j0 + r <= ny - 2);
+                        if (own[s]) vbo[s] = A.ub[kb + f_off(s)];
+                    } else {
+                        const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX, g = j0 - 2 + fr;
+                        own[s] = g >= j0 && g < j0 + TY && g <= ny - 2 && i0 + c <= nx - 2;
+                        if (own[s]) vbo[s] = A.vb[kb + f_off(s)];
+                    }
+                }
 #pragma unroll
                 for (int s = 0; s < FS; ++s) {
                     const int e = tid + s * NT;
@@ -307,8 +327,8 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                         const double Fb = (lt[lr][fl + 1] - lt[lr][fl]) * A.dx1;
                         const double R = face_vjp(qt[qr][fl], qt[qr][fl + 1], qt[qr][fl + 2], qt[qr][fl + 3],
                                                   !(uf < 0.0), 0, true, Fb * uf, ga, gb, gc, gd);
-                        if (f >= i0 && f < i0 + TX && f <= nx - 2 && (!THREE_D || j0 + r <= ny - 2))
-                            A.ub[kb + f_off(s)] += Fb * R;
+                        (void)f;
+                        if (own[s]) A.ub[kb + f_off(s)] = vbo[s] + Fb * R;
                     } else if constexpr (THREE_D) {
                         // y face g + 1/2 (velocity v(g+1)), g = j0-2+fr, of tile column c
                         const int e2 = e - NXF, fr = e2 / TX, c = e2 - fr * TX, g = j0 - 2 + fr;
@@ -316,7 +336,8 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                         const double Fb = (lt[fr + 1][c + 2] - lt[fr][c + 2]) * A.dy1;
                         const double R = face_vjp(qt[fr][c + 3], qt[fr + 1][c + 3], qt[fr + 2][c + 3],
                                                   qt[fr + 3][c + 3], !(vf < 0.0), 0, true, Fb * vf, ga, gb, gc, gd);
-                        if (g >= j0 && g < j0 + TY && g <= ny - 2 && i0 + c <= nx - 2) A.vb[kb + f_off(s)] += Fb * R;
+                        (void)g;
+                        if (own[s]) A.vb[kb + f_off(s)] = vbo[s] + Fb * R;
                     }
                     cf[0][e] = ga;
                     cf[1][e] = gb;
