@@ -386,6 +386,15 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             __syncthreads();
             if (k < k1) fetch(k + 3);
             if (valid) {
+                const size_t kb = sz * (k - 1);
+                const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
+                // the step-start momenta, requested before the cell arithmetic
+                float m0v[3] = {0.0f, 0.0f, 0.0f};
+                if (a.epi != 0 && a.c0 != 0.0) {
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+                        if (n != 1 || THREE_D) m0v[n] = a.m0[n][id];
+                }
                 // ring offset of level kk (k-2 <= kk <= k+2)
                 auto lev = [&](int kk) { return ((kk + 2 * NR) % NR) * PL; };
                 auto acc = [&](int n) {
@@ -397,8 +406,6 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                 double d[3];
                 mom_cell(g, c, lavisc, true, i, j, k, acc(0), acc(1), acc(2), acc(3), acc(4), acc(5), d[0], d[1],
                          d[2]);
-                const size_t kb = sz * (k - 1);
-                const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
                 const int ce = lev(k) + (THREE_D ? (ty + H) * LW : 0) + tx + H;
 #pragma unroll
                 for (int n = 0; n < 3; ++n) {
@@ -409,7 +416,7 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                         qv = (float)L;
                     } else {
                         const double m = (double)ring[n][0][ce];
-                        const double qn = a.c0 != 0.0 ? a.c0 * (double)a.m0[n][id] + a.c1 * (m + a.dt * L)
+                        const double qn = a.c0 != 0.0 ? a.c0 * (double)m0v[n] + a.c1 * (m + a.dt * L)
                                                       : a.c1 * (m + a.dt * L);
                         qv = (float)qn;
                     }
