@@ -47,14 +47,16 @@ def test_scalar_tendency(cuda, scheme, lim, adv, dims):
     assert err <= TOL
 
 
+@pytest.mark.parametrize("nz", [21, 9, 17])
 @pytest.mark.parametrize("scheme,lim", [("weno", "minmod"), ("weno5", "minmod"), ("ppm", "minmod"),
                                         ("muscl", "MC"), ("quick", "minmod"), ("lw", "minmod")])
-def test_multi_tile_grid(cuda, scheme, lim):
-    """70x19x21 interior: several x and y tiles of every kernel shape with a
+def test_multi_tile_grid(cuda, scheme, lim, nz):
+    """70x19xnz interior: several x and y tiles of every kernel shape with a
     partial last tile (positions past the array read periodic images), and
-    level chunks that do not divide nz.  Bit-identical to the oracle."""
-    print(f"case multi-tile {scheme} {lim}", flush=True)
-    g, u, v, w, q = case3d(maxx=70, maxy=19, nz=21, seed=12)
+    level chunks that do not divide nz (nz = 9 and 17 leave a last chunk of
+    one level for the 8- and 16-level chunks).  Bit-identical to the oracle."""
+    print(f"case multi-tile {scheme} {lim} nz={nz}", flush=True)
+    g, u, v, w, q = case3d(maxx=70, maxy=19, nz=nz, seed=12)
     dt = cases.cfl_dt(g, u, v, w, 0.5)
     ora = Oracle(g, scal_adv=scheme, muscllimiter=lim)
     ref = np.zeros(g.shape)
