@@ -204,6 +204,40 @@ def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
                 assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
 
 
+@pytest.mark.parametrize("nz", [24, 17, 9])
+@pytest.mark.parametrize("mom_ord,lavisc", [(4, True), (2, False)])
+def test_batch_momentum_bitwise_fp64_kernel(cuda, nz, mom_ord, lavisc):
+    """The batched momentum kernel carries each z-face flux up the column from
+    the level that owns it and reads FP32 operands; the per-cell FP64 kernel
+    of mimadv_momentum_tendency evaluates every flux it needs in place.  On
+    FP32-representable inputs both run the same FP64 operations, so the batch
+    tendency is the FP64 one negated and rounded once to FP32, bit for bit.
+    Columns march in chunks of 16 levels: nz = 17 leaves a last chunk of one
+    level, nz = 9 a single chunk.  The batched path takes the no-flux floor
+    and lid only."""
+    print(f"case batch_momentum_bitwise nz={nz} ord={mom_ord} lavisc={lavisc}", flush=True)
+    g = gm.stretched(40, 36, nz)
+    u, v, w = cases.turbulent_wind(g, seed=5, rms=3.0)
+    u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
+    mom = _momenta(g, u, v, w, seed=43)
+    q = cases.lognormal(g, seed=44, mu=-9.0, sigma=1.0).astype(np.float32).astype(np.float64)
+    with Advection(g, scal_adv="weno", mom_ord=mom_ord, lavisc=lavisc) as a:
+        d64 = [torch.from_numpy(x).to(cuda) for x in mom + [u, v, w]]
+        ref = [torch.zeros(g.shape, dtype=torch.float64, device=cuda) for _ in range(3)]
+        a.qnadv_uvw(0, d64[:3], d64[3:], ref)
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = [torch.from_numpy(x.astype(np.float32)).to(cuda) for x in mom + [q]]
+        outs = [torch.zeros_like(f) for f in fields]
+        a.batch_tendency(du, dv, dw, fields, outs, momentum=True)
+        a.stream_sync()
+        I = g.interior()
+        for c in range(3):
+            want = (-ref[c]).float().cpu().numpy()[I]
+            got = outs[c].cpu().numpy()[I]
+            nbit = int(np.count_nonzero(want.view(np.uint32) != got.view(np.uint32)))
+            assert nbit == 0, f"momentum component {c}: {nbit} cells differ"
+
+
 @pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
 def test_batch_momentum_advance(cuda, integrator):
     """Three steps of the momenta (advecting velocity held fixed) with θ, qv
