@@ -37,8 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # bytes written).  Reported only for the exact configuration measured.
 HBM_TRAFFIC = {
     ("plain", "256,256,128", "weno", "heun", 2): 551.2e6,
-    ("fused", "512,512,160", "weno", "ssprk3", 17): 13.79e9,
-    ("fused", "512,512,160", "weno", "ssprk3", 20): 19.19e9,
+    ("fused", "512,512,160", "weno", "ssprk3", 20): 18.59e9,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1078.2e6,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
@@ -160,7 +159,7 @@ def fused(args):
         "config": {"workload": f"fused multi-field batch {maxx}x{maxy}x{nz}, {nf} fields (3 FP32 momenta + 2 FP64 "
                                f"+ {nf - 5} FP32 species), {adv.scal_adv}, {args.integrator}, hipGraph={graph}",
                    "cells": ncell, "fields": nf, "stages_per_step": nst, "dt_s": dt},
-        "roofline": {"bound": "hbm", "kernel": "fused_kernel + momentum_batch_kernel (all launches of one stage)",
+        "roofline": {"bound": "hbm", "kernel": "fused_kernel + momentum_march_kernel (all launches of one stage)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": HBM_TRAFFIC.get(("fused", size, adv.scal_adv, args.integrator, nf)),
                      "alg_bytes_per_stage": sum(per_stage) / nst, "avg_stage_us": stage_s * 1e6},

@@ -243,8 +243,19 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 // T: the precision of every field of the launch (FP64 and FP32 members of
 // a batch go to two launches, so that the FP32 kernel keeps the register
 // budget of FP32 arithmetic).
+// Waves per SIMD the register allocation must allow.  The FP32 64x4 tile
+// takes 32 KB of LDS, so five groups fit a CU; at 97 VGPRs it would be held
+// to four waves, and 96 reach five without spills.  The other instances are
+// bound by LDS or need their registers.
+template <int TY, class T>
+constexpr int fused_min_waves()
+{
+    return (TY == 4 && sizeof(T) == 4) ? 5 : 1;
+}
+
 template <int SCHEME, int TX, int TY, class T>
-__global__ __launch_bounds__(TX *TY) void fused_kernel(FusedArgs A)
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(fused_min_waves<TY, T>(), 8))) void
+fused_kernel(FusedArgs A)
 {
     using I = Item<TX, TY>;
     __shared__ T tile_lds[I::LH * I::LW];
@@ -343,7 +354,9 @@ template <int SCHEME, class T>
 hipError_t launch_prec(const FusedArgs &A, hipStream_t s, int bpc)
 {
     if (!A.three_d) return launch_tiles<SCHEME, 64, 1, T>(A, s, bpc);
-    switch (A.tile_y) {
+    // default rows: 4 for FP32 (five waves per SIMD), 8 for FP64
+    const int ty = A.tile_y > 0 ? A.tile_y : (sizeof(T) == 4 ? 4 : 8);
+    switch (ty) {
     case 2: return launch_tiles<SCHEME, 64, 2, T>(A, s, bpc);
     case 4: return launch_tiles<SCHEME, 64, 4, T>(A, s, bpc);
     default: return launch_tiles<SCHEME, 64, 8, T>(A, s, bpc);

@@ -26,7 +26,7 @@ struct FusedArgs {
     double lim_tol, lim_tol_inv;
     int weno_bottom;
     int ntx, nty, kchunk, nwork, xcd_remap;
-    int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 8)
+    int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 4 for FP32, 8 for FP64)
     // multi-domain launches only
     int scheme;        // 0 WENO, 2 MUSCL
     int tile_x;        // tile width; tile_x * tile_y = 512
