@@ -1333,7 +1333,7 @@ int mfail(mimadv_multi *m, int code, const std::string &msg)
 // cost model: face evaluations of the padded tile cover per interior cell
 int model_tile_x(int mx, int my)
 {
-    static const int cand[4] = {64, 32, 128, 16};
+    static const int cand[3] = {64, 32, 16};
     int best = 64;
     double bestc = 1e300;
     for (int tx : cand) {
@@ -1477,7 +1477,7 @@ int mimadv_multi_set_tiles(mimadv_multi *m, const int *tile_x, const int *kchunk
     if (!m) return MIMADV_ERR_ARG;
     for (int d = 0; d < m->ndom; ++d) {
         if (tile_x && !multi_tile_ok(tile_x[d], MULTI_NT / (tile_x[d] > 0 ? tile_x[d] : 1)))
-            return mfail(m, MIMADV_ERR_ARG, "tile_x must be 16, 32, 64 or 128");
+            return mfail(m, MIMADV_ERR_ARG, "tile_x must be 16, 32 or 64");
         if (kchunk && kchunk[d] < 0) return mfail(m, MIMADV_ERR_ARG, "kchunk < 0");
     }
     for (int d = 0; d < m->ndom; ++d) {
@@ -1502,7 +1502,7 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
 {
     if (!m) return MIMADV_ERR_ARG;
     if (reps < 1) reps = 1;
-    static const int cand_x[4] = {64, 32, 128, 16};
+    static const int cand_x[3] = {64, 32, 16};
     static const int cand_k[2] = {4, 8};
     // A candidate is timed at saturation: the subdomain's work list is
     // repeated until the launch holds at least `target` work items (the

@@ -29,7 +29,7 @@ struct FusedArgs {
     int tile_y;  // rows of the 64-wide tile in 3D (2, 4, 8; 0 = 4 for FP32, 8 for FP64)
     // multi-domain launches only
     int scheme;        // 0 WENO, 2 MUSCL
-    int tile_x;        // tile width; tile_x * tile_y = 512
+    int tile_x;        // tile width; tile_x * tile_y = 256
     int work_begin;    // first work id of this subdomain in the launch
     int dirmask;  // directions included in the divergence (bit0 x, bit1 y, bit2 z)
 };
@@ -39,9 +39,9 @@ int fused_padded_count(int n32);
 hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStream_t s, int blocks_per_cu);
 
 // Multi-domain launch: table[0..ndom-1] on the device, one entry per
-// subdomain (FP32 fields only, 3D, tile_x * tile_y = 512, ntx/nty/kchunk/
+// subdomain (FP32 fields only, 3D, tile_x * tile_y = 256, ntx/nty/kchunk/
 // work_begin filled in by multi_prepare), nwork the total work count.
-constexpr int MULTI_NT = 512;
+constexpr int MULTI_NT = 256;
 bool multi_tile_ok(int tile_x, int tile_y);
 void multi_prepare(FusedArgs &A, int work_begin);  // sets ntx, nty, nwork; kchunk clamped
 hipError_t launch_multi(const FusedArgs *d_table, int ndom, int nwork, hipStream_t s);

@@ -11,9 +11,10 @@
 // face list, z face carried up the column, FP64 divergence and stage
 // update rounded once to FP32.
 //
-// The tile shape is chosen per subdomain (TX x TY with TX*TY = 512:
-// 16x32, 32x16, 64x8, 128x4) so that small or narrow subdomains waste
-// fewer lanes; faces are evaluated with the same formulas as the
+// The tile shape is chosen per subdomain (TX x TY with TX*TY = 256:
+// 16x16, 32x8, 64x4) so that small or narrow subdomains waste fewer lanes.
+// 256-thread groups keep the LDS of a work item at 32 KB, so five groups
+// share a CU (five waves per SIMD at 94 VGPRs); faces are evaluated with the same formulas as the
 // single-grid kernel, so the results do not depend on the shape.
 #include "fused.hpp"
 #include "fused_faces.hpp"
@@ -27,9 +28,9 @@ using namespace faces;
 constexpr int NT = MULTI_NT;
 constexpr int FS = 3;                  // 2*NT + TX + TY faces per tile
 constexpr int TS = 3;                  // (TX+4)*(TY+4) tile values
-constexpr int TILE_MAX = 132 * 8;      // largest (TX+4)*(TY+4) of the shapes
+constexpr int TILE_MAX = 68 * 8;       // largest (TX+4)*(TY+4) of the shapes
 constexpr int KC_MAX = 8;              // levels of a work item (velocities staged in LDS)
-constexpr int NV_MAX = 2 * NT + 132 + NT;  // face velocities and column w of one level
+constexpr int NV_MAX = 2 * NT + 68 + NT;  // face velocities and column w of one level
 
 struct Geom {
     int TX, TY, LW, LH, NXF, NF, NV;
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
 
 bool multi_tile_ok(int tile_x, int tile_y)
 {
-    return tile_x * tile_y == MULTI_NT && (tile_x == 16 || tile_x == 32 || tile_x == 64 || tile_x == 128);
+    return tile_x * tile_y == MULTI_NT && (tile_x == 16 || tile_x == 32 || tile_x == 64);
 }
 
 void multi_prepare(FusedArgs &A, int work_begin)

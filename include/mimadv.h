@@ -267,7 +267,7 @@ int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const voi
  * its FP32 velocity (u[d], v[d], w[d]) and nf[d] FP32 fields q[d][0..nf-1]
  * (device arrays, at most 18 per subdomain).  Every stage of every
  * subdomain is one work list of one persistent launch; the tile shape
- * (tile_x * tile_y = 512 with tile_x in 16, 32, 64, 128) and the level
+ * (tile_x * tile_y = 256 with tile_x in 16, 32, 64) and the level
  * chunk are chosen per subdomain.  The plan keeps two stage arrays per
  * field and runs on the stream of ctx[0]. */
 typedef struct mimadv_multi mimadv_multi;
