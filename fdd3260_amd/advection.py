@@ -264,21 +264,22 @@ class Advection:
             out.append(a.data_ptr())
         return out
 
-    def batch_tendency(self, u, v, w, fields, outs, dirs=_lib.DIR_ALL, momentum=False):
-        """momentum: fields[0:3] are the FP32 momenta rho0*(u, v, w), advanced with the momentum operator."""
+    def batch_tendency(self, u, v, w, fields, outs, dirs=_lib.DIR_ALL, momentum=False, flags=0):
+        """momentum: fields[0:3] are the FP32 momenta rho0*(u, v, w), advanced with the momentum operator.
+        flags: further MIMADV_* call flags, passed through."""
         pr, pq = self._batch_fields(fields)
         pr2, px = self._batch_fields(outs)
         if list(pr) != list(pr2):
             raise MimadvError("tendencies must have the precision of their fields")
         pu, pv, pw = self._vel32(u, v, w)
-        flags = int(dirs) | (_lib.MOMENTUM if momentum else 0)
+        flags = int(dirs) | (_lib.MOMENTUM if momentum else 0) | int(flags)
         rc = self.lib.mimadv_batch_tendency(self._h, pu, pv, pw, len(fields), pr, pq, px, flags)
         self._check(rc, "mimadv_batch_tendency")
 
-    def batch_advance(self, integrator, nsteps, dt, u, v, w, fields, graph=False, momentum=False):
+    def batch_advance(self, integrator, nsteps, dt, u, v, w, fields, graph=False, momentum=False, flags=0):
         pr, pq = self._batch_fields(fields)
         pu, pv, pw = self._vel32(u, v, w)
-        flags = (_lib.GRAPH if graph else 0) | (_lib.MOMENTUM if momentum else 0)
+        flags = (_lib.GRAPH if graph else 0) | (_lib.MOMENTUM if momentum else 0) | int(flags)
         rc = self.lib.mimadv_batch_advance(self._h, int(integrator), int(nsteps), float(dt), pu, pv, pw,
                                            len(fields), pr, pq, flags)
         self._check(rc, "mimadv_batch_advance")

@@ -126,3 +126,43 @@ def cfl_dt(g: Grid, u, v, w, cfl=0.5):
     if g.three_d:
         s = s + np.abs(v) / g.dy
     return float(cfl / s.max())
+
+
+def config3_fields(g: Grid, u, v, w, seed=5, nspec=15):
+    """The 20 prognostic fields of the fused multi-field batch, in batch
+    order: the momenta rho0*(u, v, w) (FP32), theta and qv (FP64), then
+    nspec positive-definite species (FP32; log-normal with mu from -20 to
+    -6, sigma 1.5, 30 % of cells exactly zero).  rho0 = 1.2 exp(-z/8000) at
+    cell centres.  Returns (fields as float64 arrays holding the values the
+    batch stores, precision codes 'f32'/'f64')."""
+    den0 = 1.2 * np.exp(-g.zc() / 8000.0)[:, None, None]
+    r32 = lambda a: a.astype(np.float32).astype(np.float64)
+    out = [r32(r32(a) * r32(den0)) for a in (u, v, w)]
+    prec = ["f32"] * 3
+    out += [theta_profile(g, seed=seed), lognormal(g, seed=seed + 1, mu=-5.0, sigma=0.3)]
+    prec += ["f64", "f64"]
+    for s in range(nspec):
+        q = lognormal(g, seed=seed + 5 + s, mu=-20 + 14 * s / max(1, nspec - 1), sigma=1.5, zero_frac=0.3)
+        out.append(r32(q))
+        prec.append("f32")
+    return out, prec
+
+
+def config4_draw(seed=4, ndom=48):
+    """Seeded draw of the uneven-subdomain batch: per subdomain MAXX, MAXY in
+    [48, 192], nz in [48, 96], scheme 'weno' or 'muscl' (even odds), 4-16
+    species, wind RMS in [1, 4] m/s, and per species the log-normal mu in
+    [-20, -6] (sigma 1.5).  Returns a list of dicts; field seeds are
+    1000 + d (wind) and 2000 + 20 d + s (species)."""
+    rng = np.random.default_rng(seed)
+    doms = []
+    for d in range(ndom):
+        mx, my = (int(x) for x in rng.integers(48, 193, 2))
+        nz = int(rng.integers(48, 97))
+        scheme = "weno" if rng.random() < 0.5 else "muscl"
+        ns = int(rng.integers(4, 17))
+        rms = float(rng.uniform(1.0, 4.0))
+        mus = [float(rng.uniform(-20, -6)) for _ in range(ns)]
+        doms.append({"maxx": mx, "maxy": my, "nz": nz, "scheme": scheme, "nspec": ns, "rms": rms, "mu": mus,
+                     "wind_seed": 1000 + d, "species_seeds": [2000 + 20 * d + s for s in range(ns)]})
+    return doms

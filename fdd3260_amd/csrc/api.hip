@@ -92,8 +92,19 @@ int line_scheme(int scheme, unsigned flags)
 
 bool uses_dt0(int line) { return line == MIMADV_PPM || line == MIMADV_LW || line == 7; }
 
+// The captured batch step bakes in the scheme, its options and the tile
+// shapes: any change of those drops it, and the next MIMADV_GRAPH call
+// captures again.
+void drop_graph(mimadv_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+    c->graph_key.clear();
+}
+
 void apply_opts(mimadv_ctx *c, const mimadv_opts *o)
 {
+    drop_graph(c);
     c->opts = *o;
     c->sp.scheme = o->scheme;
     c->sp.muscl_limiter = o->muscl_limiter;
@@ -555,6 +566,7 @@ int mimadv_set_tuning(mimadv_ctx *c, const mimadv_tuning *t)
         return fail(c, MIMADV_ERR_ARG, "tuning values out of range");
     if (t->tile_y != 0 && t->tile_y != 2 && t->tile_y != 4 && t->tile_y != 8)
         return fail(c, MIMADV_ERR_ARG, "tile_y must be 2, 4 or 8");
+    drop_graph(c);
     c->tuning = *t;
     return MIMADV_OK;
 }
@@ -587,6 +599,17 @@ int plan_groups(mimadv_ctx *c, int nf, const int *prec, std::vector<BatchGroup> 
         for (int n = 0; n < g32 && p < i32.size(); ++n) b.f32.push_back(i32[p++]);
         groups.push_back(b);
     } while (p < i32.size());
+    return MIMADV_OK;
+}
+
+// The batched kernels run the limited scheme of the context on device
+// arrays; the line-kernel selection flags of mimadv_scalar_tendency have no
+// batched counterpart and are refused rather than ignored.
+int check_batch_flags(mimadv_ctx *c, unsigned flags)
+{
+    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path works on device arrays");
+    if (flags & (MIMADV_UNLIMITED | MIMADV_SORDER1))
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "MIMADV_UNLIMITED and MIMADV_SORDER1 are not implemented on the batched path");
     return MIMADV_OK;
 }
 
@@ -679,8 +702,9 @@ int mimadv_batch_tendency(mimadv_ctx *c, const float *u, const float *v, const f
                           const int *precision, const void *const *q, void *const *xs, unsigned flags)
 {
     if (!c) return MIMADV_ERR_ARG;
-    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path works on device arrays");
-    int rc = check_batch(c, u, v, w);
+    int rc = check_batch_flags(c, flags);
+    if (rc) return rc;
+    rc = check_batch(c, u, v, w);
     if (rc) return rc;
     if (nf <= 0) return nf == 0 ? MIMADV_OK : fail(c, MIMADV_ERR_ARG, "nf < 0");
     if (!precision || !q || !xs) return fail(c, MIMADV_ERR_ARG, "NULL field list");
@@ -729,8 +753,9 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
                          const float *w, int nf, const int *precision, void *const *q, unsigned flags)
 {
     if (!c) return MIMADV_ERR_ARG;
-    if (flags & MIMADV_HOST_PTRS) return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path works on device arrays");
-    int rc = check_batch(c, u, v, w);
+    int rc = check_batch_flags(c, flags);
+    if (rc) return rc;
+    rc = check_batch(c, u, v, w);
     if (rc) return rc;
     if (integrator != MIMADV_INT_HEUN && integrator != MIMADV_INT_SSPRK3)
         return fail(c, MIMADV_ERR_UNSUPPORTED, "batched integrators: MIMADV_INT_HEUN, MIMADV_INT_SSPRK3");
@@ -758,10 +783,7 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         if (c->bwork) (void)hipFree(c->bwork);
         c->bwork = nullptr;
         c->bwork_bytes = 0;
-        if (c->graph_exec) {
-            (void)hipGraphExecDestroy(c->graph_exec);
-            c->graph_exec = nullptr;
-        }
+        drop_graph(c);
         if (hipMalloc(&c->bwork, bytes) != hipSuccess) return fail(c, MIMADV_ERR_NOMEM, "batch scratch allocation failed");
         c->bwork_bytes = bytes;
     }
@@ -837,10 +859,7 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         std::memcpy(&dbits, &dtk, sizeof dbits);
         key.push_back((const void *)dbits);
         if (!c->graph_exec || key != c->graph_key) {
-            if (c->graph_exec) {
-                (void)hipGraphExecDestroy(c->graph_exec);
-                c->graph_exec = nullptr;
-            }
+            drop_graph(c);
             hipGraph_t graph = nullptr;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             hipError_t le = one_step();

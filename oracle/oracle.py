@@ -42,20 +42,23 @@ class OOpts(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(path=None):
+    """The oracle library (default: liboracle.so, built if missing).  A
+    path selects another build of the same source, e.g. one with a planted
+    mistake in a test that checks a pin catches it."""
+    path = path or LIB_PATH
+    if path in _libs:
+        return _libs[path]
+    if path == LIB_PATH and not os.path.exists(LIB_PATH):
         build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     vp, i, d, dp = C.c_void_p, C.c_int, C.c_double, C.POINTER(C.c_double)
     G, O = C.POINTER(OGrid), C.POINTER(OOpts)
     lib.orc_advs_h.argtypes = [O, i, d, i, i, dp, dp, dp, i]
@@ -79,7 +82,7 @@ def load():
     lib.orc_advection_1d.argtypes = [G, dp, dp, dp, dp, dp]
     lib.orc_line_scheme.restype = i
     lib.orc_set_threads.restype = i
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -95,8 +98,8 @@ class Oracle:
     fdd3260_amd.advection.Advection)."""
 
     def __init__(self, grid, scal_adv="weno", muscllimiter="minmod", ppmlimiter="PPM", lim_tol=1.0e-9,
-                 mom_ord=4, lavisc=True, weno_bottom=0, dt0=0.0):
-        self.lib = load()
+                 mom_ord=4, lavisc=True, weno_bottom=0, dt0=0.0, lib_path=None):
+        self.lib = load(lib_path)
         self.grid = grid
         self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (grid.fdz, grid.dzw, grid.dz1w, grid.dz1)]
         g = OGrid()

@@ -137,6 +137,14 @@ def test_batch_rejects_unsupported(cuda):
         t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
         with pytest.raises(Exception, match="WENO and MUSCL"):
             a.batch_tendency(t, t, t, [t], [t.clone()])
+    # the line-kernel selection flags of the plain path have no batched form
+    with Advection(g, scal_adv="weno") as a:
+        t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
+        for fl in (_lib.UNLIMITED, _lib.SORDER1):
+            with pytest.raises(Exception, match="not implemented on the batched path"):
+                a.batch_tendency(t, t, t, [t], [t.clone()], flags=fl)
+            with pytest.raises(Exception, match="not implemented on the batched path"):
+                a.batch_advance(_lib.INT_SSPRK3, 1, 1.0, t, t, t, [t.clone()], flags=fl)
 
 
 @pytest.mark.parametrize("dirs", [1, 2, 4, 7])
@@ -342,3 +350,120 @@ def test_batch_config3_field_errors(cuda):
             assert mx <= 1e-9, names[n]
         else:
             assert rms <= 1e-2, names[n]
+
+
+@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+def test_batch_graph_follows_option_changes(cuda, scheme):
+    """A captured step bakes in the scheme and the tile shape.  After
+    set_options or set_tuning on the same arrays, a graph advance must run
+    the new configuration: it is compared bit for bit with eager launches."""
+    print(f"case batch_graph_follows_option_changes {scheme}", flush=True)
+    g, u, v, w, f64, f32 = _case(nspec=3, zero_frac=0.0)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    other = "muscl" if scheme == "weno" else "weno"
+    with Advection(g, scal_adv=scheme, muscllimiter="MC") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        init = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
+                                                             for q in f32]
+        fields = [f.clone() for f in init]
+
+        def run(graph):
+            for f, i0 in zip(fields, init):
+                f.copy_(i0)
+            torch.cuda.synchronize()
+            a.batch_advance(_lib.INT_SSPRK3, 2, dt, du, dv, dw, fields, graph=graph)
+            a.stream_sync()
+            return [f.clone() for f in fields]
+
+        first = run(True)
+        a.set_options(scal_adv=other)
+        after_opts = run(True)
+        eager_opts = run(False)
+        for x, y, z in zip(after_opts, eager_opts, first):
+            assert torch.equal(x, y), "graph replay kept the old scheme"
+        assert any(not torch.equal(x, z) for x, z in zip(after_opts, first)), "scheme change had no effect"
+        a.set_tuning(tile_y=2, kchunk=5)
+        after_tune = run(True)
+        eager_tune = run(False)
+        for x, y in zip(after_tune, eager_tune):
+            assert torch.equal(x, y), "graph replay kept the old tiles"
+
+
+@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
+    """Every public tile instance (64x2, 64x4, 64x8 for both precisions) and
+    level chunks of 8 and 5 on nz = 17, where the last chunk is ragged
+    (17 = 2*8 + 1 = 3*5 + 2): each z face is evaluated by one code site
+    whatever the chunking, so all shapes give the same bits; the default
+    shape is held to the oracle."""
+    print(f"case batch_tile_shapes {scheme}", flush=True)
+    g, u, v, w, f64, f32 = _case(maxx=44, maxy=37, nz=17, nspec=3)
+    ora = Oracle(g, scal_adv=scheme, muscllimiter="MC")
+    refs = []
+    for q in f64 + f32:
+        xs = np.zeros(g.shape)
+        ora.scalar_tendency(0.0, u, v, w, q, xs)
+        refs.append(xs)
+    results = {}
+    with Advection(g, scal_adv=scheme, muscllimiter="MC") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
+                                                               for q in f32]
+        for ty in (0, 2, 4, 8):
+            for kc in (0, 5):
+                a.set_tuning(tile_y=ty, kchunk=kc)
+                outs = [torch.zeros_like(f) for f in fields]
+                a.batch_tendency(du, dv, dw, fields, outs)
+                a.stream_sync()
+                results[(ty, kc)] = [o.cpu() for o in outs]
+    base = results[(0, 0)]
+    for n, (o, r) in enumerate(zip(base, refs)):
+        got = o.double().numpy()
+        if n < len(f64):
+            assert _relmax(got, r, g) <= 1e-10, f"fp64 field {n}"
+        else:
+            assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
+    for key, outs in results.items():
+        for n, (x, y) in enumerate(zip(outs, base)):
+            assert torch.equal(x, y), f"tile_y/kchunk {key}: field {n} differs from the default shape"
+
+
+@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+def test_batch_2d_with_momentum(cuda, scheme):
+    """A 2D batch (ny = 1, the 64x1 instances of both kernels): two FP32
+    momenta (rho0 u, rho0 w; q[1] is left alone), θ in FP64 and FP32
+    species, tendency against the oracle."""
+    print(f"case batch_2d {scheme}", flush=True)
+    g = gm.stretched(40, 0, 17)
+    u, v, w = cases.turbulent_wind(g, seed=7, rms=3.0)
+    u, w = (a.astype(np.float32).astype(np.float64) for a in (u, w))
+    v = np.zeros(g.shape)
+    mom = _momenta(g, u, v, w, seed=8)
+    th = cases.theta_profile(g, seed=9)
+    specs = [cases.lognormal(g, seed=20 + s, mu=-9.0, sigma=1.0, zero_frac=0.3).astype(np.float32)
+             .astype(np.float64) for s in range(3)]
+    ora = Oracle(g, scal_adv=scheme, muscllimiter="MC")
+    ua, va, wa = ora.qnadv_uvw(0, *mom, u, v, w)
+    srefs = []
+    for q in [th] + specs:
+        xs = np.zeros(g.shape)
+        ora.scalar_tendency(0.0, u, v, w, q, xs)
+        srefs.append(xs)
+    with Advection(g, scal_adv=scheme, muscllimiter="MC") as a:
+        du, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, w))
+        fields = ([torch.from_numpy(m.astype(np.float32)).to(cuda) for m in mom] +
+                  [torch.from_numpy(th).to(cuda)] +
+                  [torch.from_numpy(q.astype(np.float32)).to(cuda) for q in specs])
+        outs = [torch.zeros_like(f) for f in fields]
+        a.batch_tendency(du, None, dw, fields, outs, momentum=True)
+        a.stream_sync()
+        for c, r in ((0, ua), (2, wa)):
+            got = outs[c].double().cpu().numpy()
+            assert _relmax(got, -r, g) <= 1e-6, f"momentum component {c}"
+        assert not bool(outs[1].any()), "q[1] is not touched in 2D"
+        for n, (o, r) in enumerate(zip(outs[3:], srefs)):
+            got = o.double().cpu().numpy()
+            if n == 0:
+                assert _relmax(got, r, g) <= 1e-10, "fp64 theta"
+            else:
+                assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
