@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X advection operator (one JSON line on stdout).
 
-Workload (default): the 3D plain-method forward run on one GPU --
-256x256x128 interior cells, dx = dy = 50 m, dz stretched linearly from 20 m
-to 80 m, periodic laterally and no-flux at floor and lid, divergence-free
-turbulent wind (curl of a seeded vector potential, k^-5/3, 2 m/s RMS), two
-advected fields (a log-normal positive-definite scalar, mu = -9, sigma = 1,
-and theta = 300 K + 3 K/km + 0.1 K noise), reference 'weno' scheme, Heun
-stages (sorder = 2), FP64, one plain per-field launch per field and stage,
-all state resident in HBM.  A "step" is one full time step (two stages of
-both fields); K steps are timed.
+Workload (default, --workload fused): the fused multi-field batch on one
+GPU -- 512x512x160 interior cells, dx = dy = 50 m, dz stretched linearly
+from 20 m to 80 m, periodic laterally and no-flux at floor and lid,
+divergence-free turbulent FP32 wind (curl of a seeded vector potential,
+k^-5/3, 3 m/s RMS) and 20 prognostic fields: the momenta rho0*(u, v, w) in
+FP32 (momentum operator of qnadv_uvw), theta and qv in FP64, and 15
+positive-definite FP32 species (log-normal, mu in [-20, -6], sigma 1.5,
+30 % exact zeros); reference 'weno' scheme, SSP-RK3, every step replayed
+from a hipGraph, all state resident in HBM.  A "step" is one full time
+step (three stages of all 20 fields); K steps are timed.
 
 value: cell.species-updates/s = interior cells x fields x stages x K / time.
-roofline: the dominant kernel (the fused scalar tendency + stage kernel)
-against the 8 TB/s HBM peak, with its algorithmic bytes per launch and its
-average duration from HIP events recorded by the library on its stream.
-cpu_baseline: the C restatement of the reference (the oracle) on the host
-cores, on a bounded number of Heun steps of the same workload.
+roofline: the dominant kernel (the FP32 species launch of a stage) against
+the 8 TB/s HBM peak: algorithmic bytes per launch over its average
+duration, from HIP events the library records around every launch of a
+second, eager run of K steps on its stream (MIMADV_PROFILE).
+cpu_baseline: the C restatement of the reference (the oracle, FP64) on the
+host cores, on a bounded sample of the same 20-field workload.
+sub_records: the plain per-field FP64 forward at 256x256x128 (with its
+scheme variants), the trajectory adjoint of 200 SSP-RK3 steps (cost ratio)
+and the 48 uneven subdomains, each measured the same way.
+
+Other workloads alone: --workload plain | adjoint | multi.
 """
 from __future__ import annotations
 
@@ -25,6 +32,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -37,7 +46,6 @@ HBM_PEAK_GBS = 8000.0
 # bytes written).  Reported only for the exact configuration measured.
 HBM_TRAFFIC = {
     ("plain", "256,256,128", "weno", "heun", 2): 551.2e6,
-    ("fused", "512,512,160", "weno", "ssprk3", 20): 18.59e9,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1078.2e6,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
@@ -90,82 +98,179 @@ def cpu_baseline(g, u, v, w, fields, dt, scheme, limiter, integrator, steps, thr
     return ncell * len(fields) * nst * steps / el, el
 
 
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def fused_cpu_baseline(g, u, v, w, fields, dt, steps, threads):
+    """The FP64 oracle on the same 20-field workload: `steps` SSP-RK3 steps
+    of every field (momenta with qnadv_uvw, the rest with the scalar
+    'weno'), on `threads` host threads; plus one stage of one species on a
+    single thread.  Returns (rate, seconds, rate on one thread, its seconds)."""
+    from oracle.oracle import Oracle
+
+    ora = Oracle(g, scal_adv="weno")
+    ora.set_threads(threads)
+    ncell = g.maxx * max(g.maxy, 1) * g.nz
+    state = [f.copy() for f in fields]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m0 = state[:3]
+        cur = [m.copy() for m in m0]
+        for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
+            L = ora.qnadv_uvw(0, *cur, u, v, w)
+            cur = [c0 * a0 + c1 * (x - dt * l) for a0, x, l in zip(m0, cur, L)]
+            for x in cur:
+                ora.fill_scalar(x)
+            del L
+        state[:3] = cur
+        for q in state[3:]:
+            ora.ssprk3_step(dt, u, v, w, q)
+    el = time.perf_counter() - t0
+    rate = ncell * len(fields) * 3 * steps / el
+    ora.set_threads(1)
+    xs = np.zeros(g.shape)
+    t0 = time.perf_counter()
+    ora.scalar_tendency(dt, u, v, w, fields[5], xs)
+    el1 = time.perf_counter() - t0
+    return rate, el, ncell / el1, el1
+
+
 def fused(args):
     """Batched mixed-precision run (512x512x160 by default): the three
     momenta rho0*(u, v, w) in FP32 (advanced with the momentum operator),
     theta and qv in FP64, 15 positive-definite species in FP32 (mass/number
     log-normal, mu in [-20, -6], sigma 1.5, 30 % exact zeros), FP32 velocity
     at 3 m/s RMS."""
-    import numpy as np
     import torch
 
     from fdd3260_amd import Advection, _lib, cases, grid as gm
 
-    size = args.size if args.size != "256,256,128" else "512,512,160"
+    size = args.size or "512,512,160"
+    integ = args.integrator or "ssprk3"
+    steps = args.steps if args.steps is not None else 50
     maxx, maxy, nz = (int(s) for s in size.split(","))
     t0 = time.time()
     g = gm.stretched(maxx, maxy, nz, dx=50.0, dy=50.0, dz_bottom=20.0, dz_top=80.0)
     u, v, w = cases.turbulent_wind(g, seed=2, rms=3.0)
+    u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
     dt = cases.cfl_dt(g, u, v, w, 0.5)
+    host, prec = cases.config3_fields(g, u, v, w)
     dev = torch.device("cuda:0")
-    u, v, w = (a.astype(np.float32) for a in (u, v, w))
-    du, dv, dw = (torch.from_numpy(a).to(dev) for a in (u, v, w))
-    # anelastic reference density 1.2 exp(-z/8000) at cell centres
-    den0 = (1.2 * np.exp(-g.zc() / 8000.0)).astype(np.float32)[:, None, None]
-    host = [u * den0, v * den0, w * den0,
-            cases.theta_profile(g, seed=5), cases.lognormal(g, seed=6, mu=-5.0, sigma=0.3)]
-    nspec = 15
-    for s in range(nspec):
-        q = cases.lognormal(g, seed=10 + s, mu=-20 + 14 * s / (nspec - 1), sigma=1.5, zero_frac=0.3)
-        host.append(q.astype(np.float32))
-    fields = [torch.from_numpy(np.ascontiguousarray(h)).to(dev) for h in host]
-    del host
+    du, dv, dw = (torch.from_numpy(a.astype(np.float32)).to(dev) for a in (u, v, w))
+    init = [torch.from_numpy(np.ascontiguousarray(h.astype(np.float32 if p == "f32" else np.float64))).to(dev)
+            for h, p in zip(host, prec)]
+    fields = [f.clone() for f in init]
     log(f"fused case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
-    adv = Advection(g, scal_adv=args.scheme if args.scheme in ("weno", "muscl") else "weno", muscllimiter=args.limiter)
+    scheme = args.scheme if args.scheme in ("weno", "muscl") else "weno"
+    adv = Advection(g, scal_adv=scheme, muscllimiter=args.limiter)
     adv.set_tuning(group32=args.group32, kchunk=args.kchunk, tile_y=args.tile_y)
-    code = _lib.INT_HEUN if args.integrator == "heun" else _lib.INT_SSPRK3
-    nst = 2 if args.integrator == "heun" else 3
+    code = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
+    nst = 2 if integ == "heun" else 3
     graph = not args.no_graph
     if args.warmup > 0:
         adv.batch_advance(code, args.warmup, dt, du, dv, dw, fields, graph=graph, momentum=True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    adv.batch_advance(code, args.steps, dt, du, dv, dw, fields, graph=graph, momentum=True)
+    adv.batch_advance(code, steps, dt, du, dv, dw, fields, graph=graph, momentum=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
-    kern_ms = adv.last_kernel_ms()
+    log(f"fused: {steps} steps in {wall:.3f}s")
+    # the same steps again, eager, with an event pair around every launch
+    for f, i0 in zip(fields, init):
+        f.copy_(i0)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    adv.batch_advance(code, steps, dt, du, dv, dw, fields, momentum=True, profile=True)
+    torch.cuda.synchronize()
+    wall_eager = time.perf_counter() - t_start
+    kinds = adv.batch_kernel_ms()
+    for f in fields:
+        if not bool(torch.isfinite(f).all()):
+            raise RuntimeError("non-finite state after the timed steps")
     ncell = maxx * maxy * nz
     nf = len(fields)
-    updates = ncell * nf * nst * args.steps
-    # algorithmic bytes per stage over all fields: stage 1 reads q, writes
-    # q1; later stages read the stage state and q0 and write; 3 FP32 velocities
-    fb = sum(f.element_size() for f in fields)
-    per_stage = [(2 * fb + 12) * ncell] + [(3 * fb + 12) * ncell] * (nst - 1)
-    stage_s = kern_ms * 1e-3 / (nst * args.steps)
-    achieved = (sum(per_stage) / nst) / stage_s / 1e9
-    out = {
+    updates = ncell * nf * nst * steps
+    # algorithmic bytes per stage and launch kind: stage 1 reads q and writes
+    # q1; later stages read the stage state and q0 and write; the launch
+    # reads the 3 FP32 velocities once
+    groups = {"momentum": [0, 1, 2], "f64": [3, 4], "f32": list(range(5, nf))}
+
+    def kind_bytes(idx):
+        fb = sum(fields[n].element_size() for n in idx)
+        per = [(2 * fb + 12) * ncell] + [(3 * fb + 12) * ncell] * (nst - 1)
+        return sum(per) / nst
+
+    per_kind = {}
+    for k, idx in groups.items():
+        ms, cnt = kinds[k]
+        if cnt == 0:
+            continue
+        avg_s = ms * 1e-3 / cnt
+        ab = kind_bytes(idx)
+        per_kind[k] = {"launches": cnt, "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": ab,
+                       "achieved_GBps": ab / avg_s / 1e9, "frac": ab / avg_s / 1e9 / HBM_PEAK_GBS}
+    dom = max(per_kind, key=lambda k: per_kind[k]["avg_launch_us"])
+    stage_us = sum(v["avg_launch_us"] for v in per_kind.values())
+    stage_bytes_all = sum(kind_bytes(idx) for idx in groups.values())
+    kernel_names = {"f32": "fused_kernel<0,64,4,float> (15 FP32 species)",
+                    "f64": "fused_kernel<0,64,8,double> (theta, qv)",
+                    "momentum": "momentum_batch_kernel<32,8> (3 FP32 momenta)"}
+    cpu = None
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else 1
+    if cpu_steps > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {cpu_steps} SSP-RK3 step(s) of the 20 fields with {threads} threads")
+        rate, el, rate1, el1 = fused_cpu_baseline(g, u, v, w, host, dt, cpu_steps, threads)
+        cpu = {
+            "value": rate,
+            "unit": "cell*species-updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{cpu_steps} SSP-RK3 step(s) of the same {maxx}x{maxy}x{nz} 20-field workload in FP64 "
+                      f"(momenta with qnadv_uvw, 17 scalars with weno), {el:.1f}s on {threads} threads; one "
+                      f"stage of one species on 1 thread: {rate1:.3e} updates/s ({el1:.1f}s); C restatement "
+                      f"of the reference, gcc 11.4 -O2 -ffp-contract=off, OpenMP over lines; host CPU "
+                      f"{cpu_info()}",
+            "value_1core": rate1,
+        }
+    return {
         "metric": "cell*species-updates/s and % HBM roofline at 1 GPU",
         "value": updates / wall,
         "unit": "cell*species-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic (seeded turbulent wind, rho0 u/v/w momenta, log-normal species with 30% zeros, theta, qv)",
         "config": {"workload": f"fused multi-field batch {maxx}x{maxy}x{nz}, {nf} fields (3 FP32 momenta + 2 FP64 "
-                               f"+ {nf - 5} FP32 species), {adv.scal_adv}, {args.integrator}, hipGraph={graph}",
-                   "cells": ncell, "fields": nf, "stages_per_step": nst, "dt_s": dt},
-        "roofline": {"bound": "hbm", "kernel": "fused_kernel + momentum_march_kernel (all launches of one stage)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": HBM_TRAFFIC.get(("fused", size, adv.scal_adv, args.integrator, nf)),
-                     "alg_bytes_per_stage": sum(per_stage) / nst, "avg_stage_us": stage_s * 1e6},
-        "cpu_baseline": None,
+                               f"+ {nf - 5} FP32 species), {adv.scal_adv}, {integ}, hipGraph={graph}",
+                   "cells": ncell, "fields": nf, "stages_per_step": nst, "dt_s": dt,
+                   "eager_profiled_value": updates / wall_eager,
+                   "stage_us": stage_us, "stage_alg_bytes": stage_bytes_all,
+                   "stage_frac": stage_bytes_all / (stage_us * 1e-6) / 1e9 / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "kernel": kernel_names[dom],
+                     "achieved": per_kind[dom]["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": per_kind[dom]["frac"],
+                     "traffic": HBM_TRAFFIC.get(("fused", dom, size, scheme, integ, nf)),
+                     "alg_bytes_per_launch": per_kind[dom]["alg_bytes_per_launch"],
+                     "avg_launch_us": per_kind[dom]["avg_launch_us"],
+                     "kernels": {kernel_names[k]: v for k, v in per_kind.items()}},
+        "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
 
 
 def adjoint(args):
@@ -182,7 +287,8 @@ def adjoint(args):
 
     from fdd3260_amd import Advection, _lib, cases
 
-    maxx, maxy, nz = (int(s) for s in args.size.split(","))
+    size = args.size or "256,256,128"
+    maxx, maxy, nz = (int(s) for s in size.split(","))
     t0 = time.time()
     g, u, v, w, fields, dt = build_case(maxx, maxy, nz, seed=2, rms=2.0)
     q0 = fields[0]
@@ -198,7 +304,8 @@ def adjoint(args):
     log(f"adjoint case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
     adv = Advection(g, scal_adv="weno")
     adv.set_stream(torch.cuda.current_stream(dev).cuda_stream)
-    code, nsteps, ck = _lib.INT_SSPRK3, args.steps, args.checkpoint
+    code, ck = _lib.INT_SSPRK3, args.checkpoint
+    nsteps = args.steps if args.steps is not None else 200
 
     def forward(q0_, u_, v_, w_):
         q = q0_.clone()
@@ -306,12 +413,12 @@ def adjoint(args):
                    "device_arrays": 3 + nseg + 3 * kk},
         "roofline": {"bound": "hbm", "kernel": "adjoint_kernel<64,8> (one stage adjoint)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": HBM_TRAFFIC.get(("adjoint", args.size, "weno", "ssprk3", 1)),
+                     "traffic": HBM_TRAFFIC.get(("adjoint", size, "weno", "ssprk3", 1)),
                      "alg_bytes_per_launch": adj_bytes, "avg_launch_us": adj_kernel_s * 1e6},
         "gradient_norms": {"q0_lognormal": q_grad_norm, "q0_theta": theta_grad_norm},
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def multi(args):
@@ -326,29 +433,24 @@ def multi(args):
 
     from fdd3260_amd import Advection, MultiDomain, _lib, cases, grid as gm
 
-    rng = np.random.default_rng(4)
+    steps = args.steps if args.steps is not None else 20
     ndom = args.ndom
     t0 = time.time()
     dev = torch.device("cuda:0")
     advs, vels, fields, dts, cells, nspec = [], [], [], [], [], []
-    for d in range(ndom):
-        mx, my = (int(x) for x in rng.integers(48, 193, 2))
-        nz = int(rng.integers(48, 97))
-        scheme = "weno" if rng.random() < 0.5 else "muscl"
-        ns = int(rng.integers(4, 17))
-        rms = float(rng.uniform(1.0, 4.0))
-        g = gm.stretched(mx, my, nz)
-        u, v, w = cases.turbulent_wind(g, seed=1000 + d, rms=rms)
+    for d in cases.config4_draw(seed=4, ndom=ndom):
+        g = gm.stretched(d["maxx"], d["maxy"], d["nz"])
+        u, v, w = cases.turbulent_wind(g, seed=d["wind_seed"], rms=d["rms"])
         dts.append(cases.cfl_dt(g, u, v, w, 0.5))
         vels.append(tuple(torch.from_numpy(a.astype(np.float32)).to(dev) for a in (u, v, w)))
         fl = []
-        for s in range(ns):
-            q = cases.lognormal(g, seed=2000 + 20 * d + s, mu=float(rng.uniform(-20, -6)), sigma=1.5)
+        for sd, mu in zip(d["species_seeds"], d["mu"]):
+            q = cases.lognormal(g, seed=sd, mu=mu, sigma=1.5)
             fl.append(torch.from_numpy(q.astype(np.float32)).to(dev))
         fields.append(fl)
-        advs.append(Advection(g, scal_adv=scheme, muscllimiter="MC"))
-        cells.append(mx * my * nz)
-        nspec.append(ns)
+        advs.append(Advection(g, scal_adv=d["scheme"], muscllimiter="MC"))
+        cells.append(d["maxx"] * d["maxy"] * d["nz"])
+        nspec.append(d["nspec"])
     torch.cuda.synchronize()
     log(f"multi case built in {time.time() - t0:.1f}s")
     code, nst = _lib.INT_SSPRK3, 3
@@ -362,7 +464,7 @@ def multi(args):
         m.advance(code, args.warmup, dts)
     torch.cuda.synchronize()
     ts = time.perf_counter()
-    m.advance(code, args.steps, dts)
+    m.advance(code, steps, dts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - ts
     kern_ms = m.last_ms()
@@ -371,12 +473,12 @@ def multi(args):
     ts = time.perf_counter()
     for a, (u, v, w), fl, dt in zip(advs, vels, fields, dts):
         a.set_stream(torch.cuda.current_stream(dev).cuda_stream)
-        a.batch_advance(code, args.steps, dt, u, v, w, fl)
+        a.batch_advance(code, steps, dt, u, v, w, fl)
     torch.cuda.synchronize()
     wall_seq = time.perf_counter() - ts
-    upd = sum(c * n for c, n in zip(cells, nspec)) * nst * args.steps
+    upd = sum(c * n for c, n in zip(cells, nspec)) * nst * steps
     stage_bytes = sum(c * (n * 12 + 12) for c, n in zip(cells, nspec))  # q, q0, out per species + 3 velocities
-    stage_s = kern_ms * 1e-3 / (nst * args.steps)
+    stage_s = kern_ms * 1e-3 / (nst * steps)
     achieved = stage_bytes / stage_s / 1e9
     shapes = {}
     for tx, ty, kc in tiles:
@@ -386,9 +488,9 @@ def multi(args):
         "value": upd / wall,
         "unit": "cell*species-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -406,7 +508,9 @@ def multi(args):
         "cpu_baseline": None,
     }
     m.close()
-    print(json.dumps(out), flush=True)
+    for a in advs:
+        a.close()
+    return out
 
 
 def copy_probe(mib):
@@ -425,15 +529,21 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: 50 fused, 100 plain, "
+                                                               "200 adjoint, 20 multi)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scheme", default="weno", choices=["weno", "ppm", "muscl", "weno5", "quick", "lw"])
     ap.add_argument("--limiter", default="MC")
-    ap.add_argument("--integrator", default="heun", choices=["heun", "ssprk3"])
-    ap.add_argument("--size", default="256,256,128")
-    ap.add_argument("--cpu-steps", type=int, default=6, help="Heun steps of the CPU-baseline sample (0: skip)")
+    ap.add_argument("--integrator", default=None, choices=["heun", "ssprk3"],
+                    help="default: ssprk3 for fused, heun (the reference's) for plain")
+    ap.add_argument("--size", default=None, help="MAXX,MAXY,NZ (default: 512,512,160 fused, 256,256,128 plain "
+                                                 "and adjoint)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the CPU-baseline sample (0: skip; default 1 fused, 6 plain)")
+    ap.add_argument("--no-subrecords", action="store_true",
+                    help="fused workload: skip the plain, adjoint and multi-domain sub-records")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (at most 16)")
-    ap.add_argument("--workload", default="plain", choices=["plain", "fused", "adjoint", "multi"])
+    ap.add_argument("--workload", default="fused", choices=["plain", "fused", "adjoint", "multi"])
     ap.add_argument("--ndom", type=int, default=48, help="multi workload: number of subdomains")
     ap.add_argument("--no-autotune", action="store_true", help="multi workload: cost-model tiles only")
     ap.add_argument("--checkpoint", type=int, default=20, help="adjoint workload: steps between checkpoints")
@@ -456,18 +566,46 @@ def main():
               "MI355X; runs over several GPUs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
+    if args.workload == "adjoint":
+        out = adjoint(args)
+    elif args.workload == "multi":
+        out = multi(args)
+    elif args.workload == "plain":
+        out = plain(args)
+    else:
+        out = fused(args)
+        if not args.no_subrecords:
+            # the other benchmark workloads, each measured the same way, with
+            # their own default sizes and steps
+            sub = {}
+            for name, fn in (("plain_fp64_256x256x128", plain), ("adjoint_200_steps", adjoint),
+                             ("multi_48_subdomains", multi)):
+                sa = argparse.Namespace(**vars(args))
+                sa.size = None
+                sa.steps = None
+                sa.integrator = None
+                sa.cpu_steps = 0 if name != "plain_fp64_256x256x128" else None
+                log(f"sub-record {name}")
+                r = fn(sa)
+                sub[name] = {k: r[k] for k in ("metric", "value", "unit", "steps", "ms_per_step", "dtype", "config",
+                                                "roofline", "cpu_baseline") if k in r}
+            out["sub_records"] = sub
+    print(json.dumps(out), flush=True)
+
+
+def plain(args):
+    """The 3D plain per-field FP64 forward (256x256x128 by default): the
+    log-normal scalar and theta, the reference 'weno' with Heun stages (the
+    reference's integrator), one launch per field and stage, and the other
+    schemes as variants."""
     import torch
 
     from fdd3260_amd import Advection, _lib
 
-    if args.workload == "fused":
-        return fused(args)
-    if args.workload == "adjoint":
-        return adjoint(args)
-    if args.workload == "multi":
-        return multi(args)
-    maxx, maxy, nz = (int(s) for s in args.size.split(","))
+    size = args.size or "256,256,128"
+    integ = args.integrator or "heun"
+    steps = args.steps if args.steps is not None else 100
+    maxx, maxy, nz = (int(s) for s in size.split(","))
     t0 = time.time()
     g, u, v, w, fields, dt = build_case(maxx, maxy, nz, seed=2, rms=2.0)
     log(f"case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
@@ -475,14 +613,14 @@ def main():
     du, dv, dw = (torch.from_numpy(a).to(dev) for a in (u, v, w))
     dq = [torch.from_numpy(f).to(dev) for f in fields]
     adv = Advection(g, scal_adv=args.scheme, muscllimiter=args.limiter)
-    code = _lib.INT_HEUN if args.integrator == "heun" else _lib.INT_SSPRK3
-    nst = 2 if args.integrator == "heun" else 3
+    code = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
+    nst = 2 if integ == "heun" else 3
 
     if args.warmup > 0:
         adv.advance(code, args.warmup, dt, du, dv, dw, dq)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    adv.advance(code, args.steps, dt, du, dv, dw, dq)
+    adv.advance(code, steps, dt, du, dv, dw, dq)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     kern_ms = adv.last_kernel_ms()
@@ -492,19 +630,19 @@ def main():
 
     ncell = maxx * max(maxy, 1) * nz
     nfield = len(fields)
-    updates = ncell * nfield * nst * args.steps
+    updates = ncell * nfield * nst * steps
     value = updates / wall
-    launches = nfield * nst * args.steps
+    launches = nfield * nst * steps
     avg_launch_s = kern_ms * 1e-3 / launches
-    sb = stage_bytes(args.integrator, ncell)
+    sb = stage_bytes(integ, ncell)
     alg_bytes = sum(sb) / len(sb)
     achieved = alg_bytes / avg_launch_s / 1e9
 
     # the other scheme variants of the same workload (and the reference scheme
     # with SSP-RK3), timed the same way over a short run
     variants = {}
-    if not args.no_variants and args.scheme == "weno" and args.integrator == "heun":
-        vsteps = max(1, min(args.steps, 20))
+    if not args.no_variants and args.scheme == "weno" and integ == "heun":
+        vsteps = max(1, min(steps, 20))
         for name, scheme, lim, integ in (("weno/ssprk3", "weno", "MC", "ssprk3"), ("ppm/heun", "ppm", "MC", "heun"),
                                          ("muscl-MC/heun", "muscl", "MC", "heun"),
                                          ("weno5/ssprk3", "weno5", "MC", "ssprk3"), ("quick/heun", "quick", "MC", "heun"),
@@ -527,17 +665,18 @@ def main():
             del vq
 
     cpu = None
-    if args.cpu_steps > 0:
+    cpu_steps = cpu_steps if cpu_steps is not None else 6
+    if cpu_steps > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_steps} steps with {threads} threads")
-        rate, el = cpu_baseline(g, u, v, w, fields, dt, args.scheme, args.limiter, args.integrator,
-                                args.cpu_steps, threads)
+        log(f"cpu baseline: {cpu_steps} steps with {threads} threads")
+        rate, el = cpu_baseline(g, u, v, w, fields, dt, args.scheme, args.limiter, integ,
+                                cpu_steps, threads)
         cpu = {
             "value": rate,
             "unit": "cell*species-updates/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_steps} {args.integrator} steps of the same {maxx}x{max(maxy, 1)}x{nz} "
+            "sample": f"{cpu_steps} {integ} steps of the same {maxx}x{max(maxy, 1)}x{nz} "
                       f"{nfield}-field workload, C restatement (gcc -O2, no FMA contraction, OpenMP over "
                       f"lines), {el:.1f}s",
         }
@@ -547,9 +686,9 @@ def main():
         "value": value,
         "unit": "cell*species-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -558,7 +697,7 @@ def main():
         "config": {
             "workload": f"3D plain per-field forward {maxx}x{max(maxy, 1)}x{nz}, stretched dz 20-80 m, "
                         f"{args.scheme}{'/' + args.limiter if args.scheme == 'muscl' else ''}, "
-                        f"{args.integrator}, {nfield} fields, FP64",
+                        f"{integ}, {nfield} fields, FP64",
             "cells": ncell,
             "fields": nfield,
             "stages_per_step": nst,
@@ -574,13 +713,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": HBM_TRAFFIC.get(("plain", args.size, args.scheme, args.integrator, nfield)),
+            "traffic": HBM_TRAFFIC.get(("plain", size, args.scheme, integ, nfield)),
             "alg_bytes_per_launch": alg_bytes,
             "avg_launch_us": avg_launch_s * 1e6,
         },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    adv.close()
+    return out
+
+
 
 
 if __name__ == "__main__":

@@ -33,6 +33,7 @@ SORDER1 = 256
 DREF_DEN0 = 512
 GRAPH = 32
 MOMENTUM = 64
+PROFILE = 1024
 F64, F32 = 0, 1
 OWN_STREAM = C.c_void_p(-1 & ((1 << 64) - 1)).value
 
@@ -46,6 +47,8 @@ EXPORTED = (
     "mimadv_scalar_tendency",
     "mimadv_momentum_tendency",
     "mimadv_fill_halo",
+    "mimadv_fill_winds",
+    "mimadv_integ_mom",
     "mimadv_integ_scal",
     "mimadv_totalav",
     "mimadv_advance",
@@ -54,6 +57,7 @@ EXPORTED = (
     "mimadv_batch_tendency",
     "mimadv_batch_advance",
     "mimadv_batch_budget",
+    "mimadv_batch_kernel_ms",
     "mimadv_adjoint_tendency",
     "mimadv_adjoint_advance",
     "mimadv_advance_record",
@@ -145,6 +149,10 @@ def load() -> C.CDLL:
     lib.mimadv_momentum_tendency.restype = i
     lib.mimadv_fill_halo.argtypes = [vp, i, pp, u]
     lib.mimadv_fill_halo.restype = i
+    lib.mimadv_fill_winds.argtypes = [vp, vp, vp, vp, u]
+    lib.mimadv_fill_winds.restype = i
+    lib.mimadv_integ_mom.argtypes = [vp, i, i, d, C.POINTER(d), d, d, i, pp, pp, pp, pp, u]
+    lib.mimadv_integ_mom.restype = i
     lib.mimadv_integ_scal.argtypes = [vp, i, i, d, i, pp, pp, pp, pp, u]
     lib.mimadv_integ_scal.restype = i
     lib.mimadv_totalav.argtypes = [vp, vp, C.POINTER(d), C.POINTER(d), u]
@@ -160,6 +168,8 @@ def load() -> C.CDLL:
     lib.mimadv_batch_advance.argtypes = [vp, i, i, d, vp, vp, vp, i, ip, pp, u]
     lib.mimadv_batch_advance.restype = i
     lib.mimadv_batch_budget.argtypes = [vp, i, ip, pp, C.POINTER(d)]
+    lib.mimadv_batch_kernel_ms.argtypes = [vp, C.POINTER(d), ip]
+    lib.mimadv_batch_kernel_ms.restype = i
     lib.mimadv_adjoint_tendency.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     lib.mimadv_adjoint_tendency.restype = i
     lib.mimadv_adjoint_advance.argtypes = [vp, i, i, d, vp, vp, vp, vp, vp, vp, vp, vp, i]

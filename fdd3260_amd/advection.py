@@ -202,6 +202,21 @@ class Advection:
         rc = self.lib.mimadv_fill_halo(self._h, len(fields), _lib.ptr_array(ptrs), hflag)
         self._check(rc, "mimadv_fill_halo")
 
+    def fill_winds(self, u, v, w):
+        """lbcu_p4: ghost fill of the winds (in place)."""
+        ptrs, hflag = self._ptrs([u, v, w], writable=[u, v, w])
+        self._check(self.lib.mimadv_fill_winds(self._h, *ptrs, hflag), "mimadv_fill_winds")
+
+    def integ_mom(self, sorder, im, dt, z0, zmax, zdamp, sponge, old, a, tend, x):
+        """integ_mom on (u, v, w) triples; tend and x updated in place."""
+        ptrs, hflag = self._ptrs(list(old) + list(a) + list(tend) + list(x), writable=list(tend) + list(x))
+        z = np.ascontiguousarray(z0, dtype=np.float64)
+        zp = z.ctypes.data_as(C.POINTER(C.c_double))
+        arr = [_lib.ptr_array(ptrs[3 * n:3 * n + 3]) for n in range(4)]
+        rc = self.lib.mimadv_integ_mom(self._h, int(sorder), int(im), float(dt), zp, float(zmax), float(zdamp),
+                                       int(sponge), *arr, hflag)
+        self._check(rc, "mimadv_integ_mom")
+
     def integ_scal(self, sorder, im, dt, xold, a, tend, x):
         ptrs, hflag = self._ptrs([xold, a, tend, x], writable=[tend, x])
         arr = [_lib.ptr_array([p]) for p in ptrs]
@@ -276,13 +291,24 @@ class Advection:
         rc = self.lib.mimadv_batch_tendency(self._h, pu, pv, pw, len(fields), pr, pq, px, flags)
         self._check(rc, "mimadv_batch_tendency")
 
-    def batch_advance(self, integrator, nsteps, dt, u, v, w, fields, graph=False, momentum=False, flags=0):
+    def batch_advance(self, integrator, nsteps, dt, u, v, w, fields, graph=False, momentum=False, flags=0,
+                      profile=False):
+        """profile: time every launch with events (eager launches); read with batch_kernel_ms()."""
         pr, pq = self._batch_fields(fields)
         pu, pv, pw = self._vel32(u, v, w)
-        flags = (_lib.GRAPH if graph else 0) | (_lib.MOMENTUM if momentum else 0) | int(flags)
+        flags = ((_lib.GRAPH if graph else 0) | (_lib.MOMENTUM if momentum else 0) | (_lib.PROFILE if profile else 0)
+                 | int(flags))
         rc = self.lib.mimadv_batch_advance(self._h, int(integrator), int(nsteps), float(dt), pu, pv, pw,
                                            len(fields), pr, pq, flags)
         self._check(rc, "mimadv_batch_advance")
+
+    def batch_kernel_ms(self):
+        """Per-kind kernel time of the last profiled batched call: {kind: (ms, launches)}
+        for kinds 'momentum', 'f64', 'f32'."""
+        ms = (C.c_double * 3)()
+        cnt = (C.c_int * 3)()
+        self._check(self.lib.mimadv_batch_kernel_ms(self._h, ms, cnt), "mimadv_batch_kernel_ms")
+        return {k: (ms[n], cnt[n]) for n, k in enumerate(("momentum", "f64", "f32"))}
 
     def batch_budget(self, fields):
         pr, pq = self._batch_fields(fields)

@@ -46,6 +46,12 @@ struct mimadv_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     bool timing_pending = false;
+    // MIMADV_PROFILE: an event pair around every batched launch of the last
+    // profiled call, with its kind (PROF_MOMENTUM, PROF_F64, PROF_F32)
+    std::vector<hipEvent_t> prof_ev;
+    std::vector<int> prof_kind;
+    size_t prof_used = 0;
+    bool profiling = false;
     size_t n3() const { return (size_t)g.nx * g.ny * g.nz; }
 };
 
@@ -266,7 +272,9 @@ int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx *
     for (int b = 0; b < 6; ++b) c->g.bcl[b] = grid->bcl[b];
     int nz = grid->nz;
     std::vector<double> ppmc = ppm_coefficients(nz, grid->dz1w);
-    size_t nmet = (size_t)4 * nz + ppmc.size();
+    std::vector<float> momz = momentum_z_table(nz, grid->dzw, grid->dz1w, grid->dz1);
+    size_t nmomz = (momz.size() + 1) / 2;  // in doubles
+    size_t nmet = (size_t)4 * nz + ppmc.size() + nmomz;
     e = hipMalloc(&c->d_metrics, nmet * sizeof(double));
     if (e != hipSuccess) return bad(fail(c, MIMADV_ERR_NOMEM, "metric allocation failed"));
     std::vector<double> h(nmet);
@@ -275,6 +283,7 @@ int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx *
     std::memcpy(h.data() + 2 * nz, grid->dz1w, nz * sizeof(double));
     std::memcpy(h.data() + 3 * nz, grid->dz1, nz * sizeof(double));
     std::memcpy(h.data() + 4 * nz, ppmc.data(), ppmc.size() * sizeof(double));
+    std::memcpy(h.data() + 4 * nz + ppmc.size(), momz.data(), momz.size() * sizeof(float));
     e = hipMemcpy(c->d_metrics, h.data(), nmet * sizeof(double), hipMemcpyHostToDevice);
     if (e != hipSuccess) return bad(hipfail(c, e, "metric upload"));
     c->g.zm.fdz0 = c->d_metrics;
@@ -282,6 +291,7 @@ int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx *
     c->g.zm.dz1w = c->d_metrics + 2 * nz;
     c->g.zm.dz1 = c->d_metrics + 3 * nz;
     c->g.zm.ppmc = c->d_metrics + 4 * nz;
+    c->g.zm.momz = (const float *)(c->d_metrics + 4 * nz + ppmc.size());
     c->g.zm.dz = grid->dz;
     e = hipMalloc(&c->d_red, (size_t)(2 + nz) * sizeof(double));
     if (e != hipSuccess) return bad(fail(c, MIMADV_ERR_NOMEM, "reduction allocation failed"));
@@ -309,6 +319,7 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->d_den0) (void)hipFree(c->d_den0);
     if (c->d_w0) (void)hipFree(c->d_w0);
     if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -478,6 +489,67 @@ int mimadv_fill_halo(mimadv_ctx *c, int nf, double *const *x, unsigned flags)
         } else {
             HIPCHK(c, launch_fill_scalar(c->g, x[f], c->stream));
         }
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_fill_winds(mimadv_ctx *c, double *u, double *v, double *w, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    const bool three_d = c->g.ny > 1;
+    if (!u || !w || (three_d && !v)) return fail(c, MIMADV_ERR_ARG, "NULL wind component");
+    if (flags & MIMADV_HOST_PTRS) {
+        size_t n3 = c->n3(), bytes = n3 * sizeof(double);
+        int rc = ensure_stage(c, 3);
+        if (rc) return rc;
+        double *host[3] = {u, three_d ? v : nullptr, w};
+        for (int n = 0; n < 3; ++n)
+            if (host[n]) HIPCHK(c, hipMemcpyAsync(c->stage + n * n3, host[n], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_fill_winds(c->g, c->stage, c->stage + n3, c->stage + 2 * n3, c->stream));
+        for (int n = 0; n < 3; ++n)
+            if (host[n]) HIPCHK(c, hipMemcpyAsync(host[n], c->stage + n * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return MIMADV_OK;
+    }
+    HIPCHK(c, launch_fill_winds(c->g, u, v, w, c->stream));
+    return MIMADV_OK;
+}
+
+int mimadv_integ_mom(mimadv_ctx *c, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                     int sponge, const double *const *old, const double *const *a, double *const *tend,
+                     double *const *x, unsigned flags)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (!old || !a || !x || (sorder > 1 && !tend)) return fail(c, MIMADV_ERR_ARG, "bad component list");
+    if (im < 1) return fail(c, MIMADV_ERR_ARG, "stage index im is 1-based");
+    if (sponge == 2 && !z0) return fail(c, MIMADV_ERR_ARG, "the implicit sponge needs the level heights z0");
+    if (sponge == 2 && !(zmax > zdamp)) return fail(c, MIMADV_ERR_ARG, "the implicit sponge needs zmax > zdamp");
+    const bool three_d = c->g.ny > 1;
+    for (int n = 0; n < 3; ++n) {
+        if (n == 1 && !three_d) continue;
+        if (!old[n] || !a[n] || !x[n] || (sorder > 1 && !tend[n]))
+            return fail(c, MIMADV_ERR_ARG, "NULL wind component");
+    }
+    const double tw = sponge_factor(c->g.nz, sorder, im, dt, z0, zmax, zdamp, sponge);
+    if (!(flags & MIMADV_HOST_PTRS)) {
+        HIPCHK(c, launch_integ_mom(c->g, sorder, im, dt, tw, old, a, sorder > 1 ? tend : nullptr, x, c->stream));
+        return MIMADV_OK;
+    }
+    // host arrays: one component at a time through four staging arrays
+    size_t n3 = c->n3(), bytes = n3 * sizeof(double);
+    int rc = ensure_stage(c, 4);
+    if (rc) return rc;
+    double *st = c->stage;
+    for (int n = 0; n < 3; ++n) {
+        if (n == 1 && !three_d) continue;
+        HIPCHK(c, hipMemcpyAsync(st, old[n], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(st + n3, a[n], bytes, hipMemcpyHostToDevice, c->stream));
+        if (sorder > 1) HIPCHK(c, hipMemcpyAsync(st + 2 * n3, tend[n], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_integ_mom_one(c->g, n, sorder, im, dt, n == 2 ? tw : 1.0, st, st + n3,
+                                        sorder > 1 ? st + 2 * n3 : nullptr, st + 3 * n3, c->stream));
+        if (sorder > 1) HIPCHK(c, hipMemcpyAsync(tend[n], st + 2 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(x[n], st + 3 * n3, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return MIMADV_OK;
 }
@@ -667,6 +739,35 @@ FusedArgs base_args(mimadv_ctx *c, const float *u, const float *v, const float *
 
 // One launch per group.  in/in0/out give, per field index, the stencil
 // input, the step-start state and the output.
+enum { PROF_MOMENTUM = 0, PROF_F64 = 1, PROF_F32 = 2, PROF_KINDS = 3 };
+
+// Brackets one launch with an event pair when the call profiles.
+struct ProfScope {
+    mimadv_ctx *c;
+    size_t slot = 0;
+    hipError_t err = hipSuccess;
+    ProfScope(mimadv_ctx *ctx, int kind) : c(ctx)
+    {
+        if (!c->profiling) return;
+        slot = c->prof_used;
+        while (c->prof_ev.size() < 2 * (slot + 1)) {
+            hipEvent_t e = nullptr;
+            err = hipEventCreate(&e);
+            if (err != hipSuccess) return;
+            c->prof_ev.push_back(e);
+        }
+        c->prof_kind.resize(slot + 1);
+        c->prof_kind[slot] = kind;
+        c->prof_used = slot + 1;
+        err = hipEventRecord(c->prof_ev[2 * slot], c->stream);
+    }
+    hipError_t end()
+    {
+        if (!c->profiling || err != hipSuccess) return err;
+        return hipEventRecord(c->prof_ev[2 * slot + 1], c->stream);
+    }
+};
+
 hipError_t launch_groups(mimadv_ctx *c, const FusedArgs &base, const std::vector<BatchGroup> &groups,
                          const std::vector<const void *> &in, const std::vector<const void *> &in0,
                          const std::vector<void *> &out)
@@ -687,11 +788,50 @@ hipError_t launch_groups(mimadv_ctx *c, const FusedArgs &base, const std::vector
             A.q0_32[n] = (const float *)in0[f];
             A.out32[n] = (float *)out[f];
         }
-        int n32 = fused_padded_count(A.n32_active);
-        hipError_t e = launch_fused(c->sp.scheme, A.n64_active, n32, A, c->stream, c->tuning.blocks_per_cu);
-        if (e != hipSuccess) return e;
+        // the FP64 and the FP32 members are separate launches
+        if (A.n64_active > 0) {
+            FusedArgs B = A;
+            B.n32_active = 0;
+            ProfScope p(c, PROF_F64);
+            if (p.err != hipSuccess) return p.err;
+            hipError_t e = launch_fused(c->sp.scheme, B.n64_active, 0, B, c->stream, c->tuning.blocks_per_cu);
+            if (e == hipSuccess) e = p.end();
+            if (e != hipSuccess) return e;
+        }
+        if (A.n32_active > 0) {
+            FusedArgs B = A;
+            B.n64_active = 0;
+            ProfScope p(c, PROF_F32);
+            if (p.err != hipSuccess) return p.err;
+            hipError_t e = launch_fused(c->sp.scheme, 0, fused_padded_count(B.n32_active), B, c->stream,
+                                        c->tuning.blocks_per_cu);
+            if (e == hipSuccess) e = p.end();
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
+}
+
+// Momentum launch of a batched stage, bracketed when profiling.
+hipError_t launch_momentum_prof(mimadv_ctx *c, const MomBatchArgs &M)
+{
+    ProfScope p(c, PROF_MOMENTUM);
+    if (p.err != hipSuccess) return p.err;
+    hipError_t e = launch_momentum_batch(c->g, c->opts.mom_ord, c->opts.lavisc, M, c->stream);
+    return e == hipSuccess ? p.end() : e;
+}
+
+// Start of a batched call: MIMADV_PROFILE turns the per-launch events on
+// (eager launches only).
+int begin_profile(mimadv_ctx *c, unsigned flags)
+{
+    c->profiling = false;
+    if (!(flags & MIMADV_PROFILE)) return MIMADV_OK;
+    if (flags & MIMADV_GRAPH) return fail(c, MIMADV_ERR_ARG, "MIMADV_PROFILE times eager launches: drop MIMADV_GRAPH");
+    c->profiling = true;
+    c->prof_used = 0;
+    c->prof_kind.clear();
+    return MIMADV_OK;
 }
 
 }  // namespace
@@ -710,6 +850,8 @@ int mimadv_batch_tendency(mimadv_ctx *c, const float *u, const float *v, const f
     if (!precision || !q || !xs) return fail(c, MIMADV_ERR_ARG, "NULL field list");
     const int nm = (flags & MIMADV_MOMENTUM) ? 3 : 0;
     rc = check_momentum(c, flags, nf, precision);
+    if (rc) return rc;
+    rc = begin_profile(c, flags);
     if (rc) return rc;
     if (nm && (flags & MIMADV_DIR_ALL) && (flags & MIMADV_DIR_ALL) != MIMADV_DIR_ALL)
         return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched momentum tendency covers all directions");
@@ -742,7 +884,7 @@ int mimadv_batch_tendency(mimadv_ctx *c, const float *u, const float *v, const f
         M.vt = v ? v : u;
         M.wt = w;
         M.epi = 0;
-        HIPCHK(c, launch_momentum_batch(c->g, c->opts.mom_ord, c->opts.lavisc, M, c->stream));
+        HIPCHK(c, launch_momentum_prof(c, M));
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
@@ -764,6 +906,8 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
     if (!precision || !q) return fail(c, MIMADV_ERR_ARG, "NULL field list");
     const int nm = (flags & MIMADV_MOMENTUM) ? 3 : 0;
     rc = check_momentum(c, flags, nf, precision);
+    if (rc) return rc;
+    rc = begin_profile(c, flags);
     if (rc) return rc;
     std::vector<BatchGroup> groups;
     rc = plan_groups(c, nf - nm, precision + nm, groups);
@@ -839,7 +983,7 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
                 M.dt = dt;
                 M.epi = 1;
                 M.fill = 1;
-                hipError_t e = launch_momentum_batch(c->g, c->opts.mom_ord, c->opts.lavisc, M, c->stream);
+                hipError_t e = launch_momentum_prof(c, M);
                 if (e != hipSuccess) return e;
             }
         }
@@ -891,6 +1035,24 @@ int mimadv_batch_budget(mimadv_ctx *c, int nf, const int *precision, const void 
         HIPCHK(c, launch_wsum(c->g, q[f], precision[f] == MIMADV_F32, c->d_red + 2, c->d_red, c->stream));
         HIPCHK(c, hipMemcpyAsync(out + f, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return MIMADV_OK;
+}
+
+int mimadv_batch_kernel_ms(mimadv_ctx *c, double *ms, int *count)
+{
+    if (!c || !ms || !count) return MIMADV_ERR_ARG;
+    for (int k = 0; k < PROF_KINDS; ++k) {
+        ms[k] = 0.0;
+        count[k] = 0;
+    }
+    if (c->prof_used == 0) return MIMADV_OK;
+    HIPCHK(c, hipEventSynchronize(c->prof_ev[2 * c->prof_used - 1]));
+    for (size_t s = 0; s < c->prof_used; ++s) {
+        float t = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->prof_ev[2 * s], c->prof_ev[2 * s + 1]));
+        ms[c->prof_kind[s]] += t;
+        count[c->prof_kind[s]] += 1;
     }
     return MIMADV_OK;
 }

@@ -26,44 +26,49 @@ __device__ __forceinline__ double rcp<double>(double x)
     return fma(y, e, y);
 }
 
-template <class T>
-__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
-template <class T>
-__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+// NaN-free min/max: single v_min/v_max instructions (the ternary forms
+// compile to a compare and a select).
+__device__ __forceinline__ float tmin(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double tmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
 
 // Reference WENO face flux, operation-reduced.  a..d = q(f-1..f+2), uu the
 // face velocity, nonlin: whether the smoothness switch is active at this face
-// (always in x/y, k = 3..nz-3 in z).
+// (always in x/y, k = 3..nz-3 in z).  The candidates are formed from the
+// cell differences: ffim1 = b + (b-a)/2, ffip1 = b + (c-b)/2, ffip2 =
+// c - (d-c)/2; only the upwind outer candidate and its smoothness enter
+// the weights.
 template <class T>
 __device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
 {
     const T eps = T(1e-15), thr = T(20);
-    T ffim1 = T(1.5) * b - T(0.5) * a;
-    T ffip1 = T(0.5) * (b + c);
-    T ffip2 = T(1.5) * c - T(0.5) * d;
+    const T dm = b - a, dc = c - b, dp = d - c;
+    const T ffim1 = b + T(0.5) * dm;
+    const T ffip1 = b + T(0.5) * dc;
+    const T ffip2 = c - T(0.5) * dp;
     // limiter_weno: rescale about b (the left/lower cell)
-    T minp = tmin(tmin(ffim1, ffip1), ffip2);
-    T phi = tmin(fabs(b * rcp((minp - b) + eps)), T(1));
-    bool pos = uu > T(0);
-    T e2 = (c - b) * (c - b);
-    T oth = pos ? (b - a) : (d - c);
-    T eo = oth * oth;
-    T fo = pos ? ffim1 : ffip2;
+    const T minp = tmin(tmin(ffim1, ffip1), ffip2);
+    const T phi = tmin(fabs(b * rcp((minp - b) + eps)), T(1));
+    const bool pos = uu > T(0);
+    const T dout = pos ? dm : dp;
+    const T fo = pos ? ffim1 : ffip2;
+    // smoothness eta + eps of the inner and the upwind outer pair; the
+    // switch max(eta) > 20 (min(eta) + eps) in these terms is
+    // max(e) > 20 min(e) + eps
+    const T ei = dc * dc + eps, eo = dout * dout + eps;
     T wo = T(1.0 / 3.0);
     if (nonlin) {
-        T mx = tmax(eo, e2), mn = tmin(eo, e2);
-        if (mx > thr * (mn + eps)) {
-            T so = eps + eo;
-            so = so * so;
-            T si = eps + e2;
-            si = si * si;
+        const T mx = tmax(eo, ei), mn = tmin(eo, ei);
+        if (mx > thr * mn + eps) {
+            const T so = eo * eo, si = ei * ei;
             // (g1/so) / (g1/so + g2/si) with g2 = 2 g1
             wo = si * rcp(si + T(2) * so);
         }
     }
-    T F = ffip1 + wo * (fo - ffip1);
-    F = b + phi * (F - b);
-    return uu * F;
+    // b + phi (F - b) with F - b = (ffip1 - b) + wo (fo - ffip1)
+    const T Fb = T(0.5) * dc + wo * (fo - ffip1);
+    return uu * (b + phi * Fb);
 }
 
 // top closure k = nz-1 of advsz_weno (no limiter, linear weights)

@@ -85,6 +85,16 @@ struct MomBatchArgs {
 };
 hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s);
 hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s);
+// lbcu_p4 of a single-tile run (u, v, w; v unused in 2D)
+hipError_t launch_fill_winds(const GridDev &g, double *u, double *v, double *w, hipStream_t s);
+// integ_mom of one component (tw = the sponge factor for w, 1 otherwise)
+hipError_t launch_integ_mom_one(const GridDev &g, int comp, int sorder, int im, double dt, double tw,
+                                const double *old, const double *a, double *tend, double *x, hipStream_t s);
+// integ_mom on (u, v, w) with the sponge factor tw applied to w
+hipError_t launch_integ_mom(const GridDev &g, int sorder, int im, double dt, double tw, const double *const *old,
+                            const double *const *a, double *const *tend, double *const *x, hipStream_t s);
+// the reference's implicit sponge factor of integ_mom (host, FP64)
+double sponge_factor(int nz, int sorder, int im, double dt, const double *z0, double zmax, double zdamp, int sponge);
 // xs -= dref*w0*grad_1d(q/dref) on interior columns; dref = d3 (3D) if not
 // NULL, else dprof (levels) if not NULL, else 1 (util.hip)
 hipError_t launch_advection_1d(const GridDev &g, const double *w0, const double *dprof, const double *d3,
@@ -101,5 +111,17 @@ hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_par
 
 // Host-side PPM per-level coefficient table (12 doubles per level).
 std::vector<double> ppm_coefficients(int nz, const double *dz1w);
+
+// Level coefficients of the FP32 batched momentum operator, MOMZ_N floats
+// per level k (1-based, entry k-1), from the reference's metrics
+// (windadv.f90:656-688, 1077-1121):
+//   [0] wa = dzw(k+1)/(dzw(k+1)+dzw(k)), [1] wb = dzw(k)/(dzw(k+1)+dzw(k))
+//       (the dzw-weighted face average between levels k and k+1);
+//   [2] dz1w(k); [3] dz1(k);
+//   [4] 1/(dzw(k-1)+dzw(k)+dzw(k+1)); [5] 1/(dzw(k)+dzw(k-1));
+//   [6] (level nz only) dzw(nz-1)^2/(dzw(nz)+dzw(nz-1))^2, the 'nnf' lid
+//       factor of advf(nz-1).
+constexpr int MOMZ_N = 8;
+std::vector<float> momentum_z_table(int nz, const double *dzw, const double *dz1w, const double *dz1);
 
 }  // namespace mimadv

@@ -1,0 +1,364 @@
+// momentum_batch.hip -- the momenta of the batched FP32 path on gfx950.
+//
+// The three momenta (rho0 u, rho0 v, rho0 w) in FP32, advected by the FP32
+// velocities of the batch with the flux-form centred operator of
+// qnadv_uvw (src/INTEG/windadv.f90:82-492, 494-1127): mom_ord 2 or 4,
+// the 4th-difference artificial viscosity (lavisc, avisc = 1/16), 'nnf'
+// floor and lid, directions in the even-ntime order x, y, z.  Arithmetic
+// is FP32 with the level metrics the reference divides by taken as
+// precomputed factors (momentum_z_table); the Shu-Osher stage update
+// c0*m0 + c1*(m + dt*L) is evaluated in FP64 and rounded once.  The FP64
+// parity operator (momentum_fp64.hip) keeps the reference operation order;
+// this one is checked against it with tolerances.
+//
+// Column-marching form: a workgroup owns a TX x TY tile of columns and a
+// chunk of levels.  Every operand the stencils of level k read (the three
+// momenta and the three advecting velocities at levels k-2..k+2, columns
+// within two cells of the tile) sits in a five-level LDS ring of FP32
+// tiles with a 2-cell halo; each level is loaded once, coalesced, and
+// serves five output levels.  One work item per workgroup.
+#include "mimadv_internal.hpp"
+
+namespace mimadv {
+
+std::vector<float> momentum_z_table(int nz, const double *dzw, const double *dz1w, const double *dz1)
+{
+    std::vector<float> t((size_t)MOMZ_N * nz, 0.0f);
+    auto Z = [&](int k) { return dzw[k - 1]; };  // 1-based
+    for (int k = 1; k <= nz; ++k) {
+        float *e = t.data() + (size_t)MOMZ_N * (k - 1);
+        if (k < nz) {
+            const double s = Z(k + 1) + Z(k);
+            e[0] = (float)(Z(k + 1) / s);
+            e[1] = (float)(Z(k) / s);
+        }
+        e[2] = (float)dz1w[k - 1];
+        e[3] = (float)dz1[k - 1];
+        if (k >= 2 && k < nz) e[4] = (float)(1.0 / (Z(k - 1) + Z(k) + Z(k + 1)));
+        if (k >= 2) e[5] = (float)(1.0 / (Z(k) + Z(k - 1)));
+        if (k == nz) {
+            const double s = Z(nz) + Z(nz - 1);
+            e[6] = (float)(Z(nz - 1) * Z(nz - 1) / (s * s));
+        }
+    }
+    return t;
+}
+
+namespace {
+
+constexpr float AVISC = 1.0f / 16.0f;
+
+struct MomF {
+    float c1s, c2s, c1c, c2c;      // self lines (4/3, 1/3), cross lines (9/8, 1/8)
+    float dx1, dx2, dx3, dy1, dy2, dy3;  // 1/d, 1/(2d), 1/(3d)
+    int lavisc;
+};
+
+template <class F>
+__device__ __forceinline__ float visc5(F f, int i)
+{
+    return -f(i + 2) + 4.0f * f(i + 1) - 6.0f * f(i) + 4.0f * f(i - 1) - f(i - 2);
+}
+
+// advux_fd / advvy_fd (windadv.f90:494-554): self advection along a line
+template <class MF, class TF>
+__device__ __forceinline__ float self_line(int i, float d1, float d2, const MomF &c, MF M, TF T)
+{
+    auto flux = [&](int ii) { return (T(ii + 1) + T(ii)) * (M(ii + 1) + M(ii)) * 0.25f; };
+    auto advf = [&](int ii) { return (T(ii + 2) + T(ii)) * (M(ii + 2) + M(ii)) * 0.25f; };
+    float t = c.c1s * (flux(i) - flux(i - 1)) * d1 - c.c2s * (advf(i) - advf(i - 2)) * d2;
+    if (c.lavisc) t -= AVISC * (fabsf(T(i)) * d1) * visc5(M, i);
+    return t;
+}
+
+// advuy_fd / advvx_fd (windadv.f90:556-626): cross advection along a line;
+// V(jj, o) is the advecting component at transverse offset o (-2..1)
+template <class MF, class VF>
+__device__ __forceinline__ float cross_line(int i, float d1, float d3, const MomF &c, MF M, VF V)
+{
+    auto flux = [&](int ii) {
+        const float vv = (c.c1c * (V(ii + 1, -1) + V(ii + 1, 0)) - c.c2c * (V(ii + 1, -2) + V(ii + 1, 1))) * 0.5f;
+        return vv * ((M(ii + 1) + M(ii)) * 0.5f);
+    };
+    float t = c.c1c * (flux(i) - flux(i - 1)) * d1 - c.c2c * (flux(i + 1) - flux(i - 2)) * d3;
+    if (c.lavisc) {
+        const float mu = 0.25f * fabsf(V(i, -1) + V(i, 0) + V(i + 1, -1) + V(i + 1, 0)) * d1;
+        t -= AVISC * mu * visc5(M, i);
+    }
+    return t;
+}
+
+// advuz_fd / advvz_fd with 'nnf' floor and lid (windadv.f90:628-694):
+// W(kk, o) the vertical velocity at level kk and transverse offset o;
+// m2k the level of dz1w on the k = 2 line (1 for u, 2 for v)
+template <class MF, class WF>
+__device__ __forceinline__ float cross_z(int k, int nz, const float *mz, int m2k, const MomF &c, MF M, WF W)
+{
+    auto Zc = [&](int kk, int n) { return mz[MOMZ_N * (kk - 1) + n]; };
+    auto flux = [&](int kk) -> float {
+        if (kk >= nz || kk < 1) return 0.0f;  // 'nnf' lid: Flux(nz) = 0
+        const float ww = (c.c1c * (W(kk + 1, -1) + W(kk + 1, 0)) - c.c2c * (W(kk + 1, -2) + W(kk + 1, 1))) * 0.5f;
+        return ww * (Zc(kk, 0) * M(kk + 1) + Zc(kk, 1) * M(kk));
+    };
+    float t;
+    if (k >= 3 && k <= nz - 1) {
+        t = c.c1c * (flux(k) - flux(k - 1)) * Zc(k, 2) - c.c2c * (flux(k + 1) - flux(k - 2)) * Zc(k, 4);
+    } else if (k == 1) {
+        t = flux(1) * Zc(1, 2);
+    } else if (k == 2) {
+        t = c.c1c * (flux(2) - flux(1)) * Zc(m2k, 2) - c.c2c * flux(3) * Zc(2, 4);
+    } else {
+        t = -flux(nz - 1) * Zc(nz, 2);
+    }
+    if (c.lavisc && k >= 3 && k <= nz - 2) {
+        const float mu = 0.25f * fabsf(W(k, -1) + W(k, 0) + W(k + 1, -1) + W(k + 1, 0)) * Zc(k, 2);
+        t -= AVISC * mu * visc5(M, k);
+    }
+    return t;
+}
+
+// advwx_fd / advwy_fd at level k in 2..nz (windadv.f90:896-1056): M(ii)
+// the vertical momentum at level k, T(ii, kk) the advecting component
+template <class MF, class TF>
+__device__ __forceinline__ float w_line(int i, int k, int nz, float d1, float d3, const MomF &c, MF M, TF T)
+{
+    auto uu = [&](int ii) -> float {
+        if (k == 2) return (T(ii, 2) + T(ii, 1)) * 0.5f;
+        if (k == nz) return (T(ii, nz) + T(ii, nz - 1)) * 0.5f;
+        return (c.c1c * (T(ii, k) + T(ii, k - 1)) - c.c2c * (T(ii, k + 1) + T(ii, k - 2))) * 0.5f;
+    };
+    auto flux = [&](int ii) { return uu(ii + 1) * ((M(ii + 1) + M(ii)) * 0.5f); };
+    float t = c.c1c * (flux(i) - flux(i - 1)) * d1 - c.c2c * (flux(i + 1) - flux(i - 2)) * d3;
+    if (c.lavisc) {
+        const float mu = 0.25f * fabsf(T(i, k) + T(i, k - 1) + T(i + 1, k) + T(i + 1, k - 1)) * d1;
+        t -= AVISC * mu * visc5(M, i);
+    }
+    return t;
+}
+
+// advwz_fd with 'nnf' floor and lid (windadv.f90:1058-1127)
+template <class MF, class TF>
+__device__ __forceinline__ float w_z(int k, int nz, const float *mz, const MomF &c, MF M, TF T)
+{
+    auto Zc = [&](int kk, int n) { return mz[MOMZ_N * (kk - 1) + n]; };
+    auto flux = [&](int kk) -> float {
+        if (kk >= nz || kk < 1) return 0.0f;
+        return (T(kk + 1) + T(kk)) * (M(kk + 1) + M(kk)) * 0.25f;
+    };
+    auto advf = [&](int kk) -> float {
+        if (kk < 1 || kk >= nz) return 0.0f;
+        if (kk == nz - 1) return Zc(nz, 6) * T(nz - 1) * M(nz - 1);
+        return (Zc(kk, 0) * T(kk + 2) + Zc(kk, 1) * T(kk)) * (Zc(kk, 0) * M(kk + 2) + Zc(kk, 1) * M(kk));
+    };
+    float t = 0.0f;
+    if (k >= 3 && k <= nz - 1) {
+        t = c.c1s * (flux(k) - flux(k - 1)) * Zc(k - 1, 3) - c.c2s * (advf(k) - advf(k - 2)) * Zc(k, 5);
+    } else if (k == 2) {
+        t = c.c1s * (flux(2) - flux(1)) * Zc(1, 3) - c.c2s * advf(2) * Zc(2, 5);
+    } else if (k == nz) {
+        t = -flux(nz - 1) * Zc(nz - 1, 3);
+    }
+    if (c.lavisc && k >= 3 && k <= nz - 2) t -= AVISC * (fabsf(T(k)) * Zc(k - 1, 3)) * visc5(M, k);
+    return t;
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) void momentum_batch_kernel(
+    GridDev g, MomF c, MomBatchArgs a, int kchunk, int ntx, int nwork)
+{
+    constexpr bool THREE_D = TY > 1;
+    constexpr int NT = TX * TY;
+    constexpr int H = 2, LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1, PL = LW * LH;
+    constexpr int NR = 5, TS = (PL + NT - 1) / NT;
+    __shared__ float ring[6][NR][PL];  // mu, mv, mw, ut, vt, wt
+
+    const int nx = g.nx, ny = g.ny, nz = g.nz, mx = nx - 5, my = ny - 5;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const int nchunk = (nz + kchunk - 1) / kchunk;
+    const float *src[6] = {a.m[0], a.m[1], a.m[2], a.ut, a.vt, a.wt};
+    const float *mz = g.zm.momz;
+
+    if (const int wi = blockIdx.x; wi < nwork) {
+        const int tile = wi / nchunk, kc = wi - tile * nchunk;
+        const int i0 = 4 + (tile % ntx) * TX;
+        const int j0 = THREE_D ? 4 + (tile / ntx) * TY : 1;
+        const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
+        const int i = i0 + tx, j = j0 + ty;
+        const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
+        // plane offset of ring entry s of this thread (ghost columns read as
+        // the caller filled them; positions past the array, the padding of a
+        // partial tile, take a periodic image that feeds no interior cell)
+        auto t_off = [&](int s) -> int {
+            const int e = tid + s * NT;
+            if (e >= PL) return -1;
+            const int r = e / LW, cc = e - r * LW;
+            int ii = i0 - H + cc;
+            if (ii > nx) ii = (ii - 4) % mx + 4;
+            int jj = 1;
+            if (THREE_D) {
+                jj = j0 - H + r;
+                if (jj > ny) jj = (jj - 4) % my + 4;
+            }
+            return (ii - 1) + nx * (jj - 1);
+        };
+        float pre[6][TS];
+        auto fetch = [&](int kk) {
+            const bool in = kk >= 1 && kk <= nz;
+            const size_t kb = in ? sz * (kk - 1) : 0;
+#pragma unroll
+            for (int s = 0; s < TS; ++s) {
+                const int off = t_off(s);
+#pragma unroll
+                for (int n = 0; n < 6; ++n) pre[n][s] = (in && off >= 0) ? src[n][kb + off] : 0.0f;
+            }
+        };
+        auto store = [&](int kk) {
+            const int slot = (kk + 2 * NR) % NR;
+#pragma unroll
+            for (int s = 0; s < TS; ++s) {
+                if (tid + s * NT >= PL) continue;
+#pragma unroll
+                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = pre[n][s];
+            }
+        };
+#pragma unroll 1
+        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) {
+            fetch(kk);
+            store(kk);
+        }
+        fetch(k0 + 2);
+#pragma unroll 1
+        for (int k = k0; k <= k1; ++k) {
+            store(k + 2);
+            __syncthreads();
+            if (k < k1) fetch(k + 3);
+            if (valid) {
+                const size_t kb = sz * (k - 1);
+                const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
+                float m0v[3] = {0.0f, 0.0f, 0.0f};
+                if (a.epi != 0 && a.c0 != 0.0) {
+#pragma unroll
+                    for (int n = 0; n < 3; ++n)
+                        if (n != 1 || THREE_D) m0v[n] = a.m0[n][id];
+                }
+                // ring element of operand n at column offsets (di, dj) from
+                // this thread's column and level kk (k-2 <= kk <= k+2)
+                auto R = [&](int n, int di, int dj, int kk) {
+                    const int e = ((kk + 2 * NR) % NR) * PL + (THREE_D ? (ty + H + dj) * LW : 0) + (tx + H + di);
+                    return ring[n][0][e];
+                };
+                float L[3];
+                // ---- u (uadv_fd): x self, y cross, z cross
+                {
+                    float t = self_line(0, c.dx1, c.dx2, c, [&](int di) { return R(0, di, 0, k); },
+                                        [&](int di) { return R(3, di, 0, k); });
+                    if (THREE_D)
+                        t += cross_line(0, c.dy1, c.dy3, c, [&](int dj) { return R(0, 0, dj, k); },
+                                        [&](int dj, int o) { return R(4, o, dj, k); });
+                    t += cross_z(k, nz, mz, 1, c, [&](int kk) { return R(0, 0, 0, kk); },
+                                 [&](int kk, int o) { return R(5, o, 0, kk); });
+                    L[0] = t;
+                }
+                // ---- v (vadv_fd), 3D only: x cross, y self, z cross
+                L[1] = 0.0f;
+                if (THREE_D) {
+                    float t = cross_line(0, c.dx1, c.dx3, c, [&](int di) { return R(1, di, 0, k); },
+                                         [&](int di, int o) { return R(3, di, o, k); });
+                    t += self_line(0, c.dy1, c.dy2, c, [&](int dj) { return R(1, 0, dj, k); },
+                                   [&](int dj) { return R(4, 0, dj, k); });
+                    t += cross_z(k, nz, mz, 2, c, [&](int kk) { return R(1, 0, 0, kk); },
+                                 [&](int kk, int o) { return R(5, 0, o, kk); });
+                    L[1] = t;
+                }
+                // ---- w (wadv_fd), levels 2..nz: x, y, z
+                L[2] = 0.0f;
+                if (k >= 2) {
+                    float t = w_line(0, k, nz, c.dx1, c.dx3, c, [&](int di) { return R(2, di, 0, k); },
+                                     [&](int di, int kk) { return R(3, di, 0, kk); });
+                    if (THREE_D)
+                        t += w_line(0, k, nz, c.dy1, c.dy3, c, [&](int dj) { return R(2, 0, dj, k); },
+                                    [&](int dj, int kk) { return R(4, 0, dj, kk); });
+                    t += w_z(k, nz, mz, c, [&](int kk) { return R(2, 0, 0, kk); }, [&](int kk) { return R(5, 0, 0, kk); });
+                    L[2] = t;
+                }
+#pragma unroll
+                for (int n = 0; n < 3; ++n) {
+                    if (n == 1 && !THREE_D) continue;
+                    const double Ln = -(double)L[n];
+                    float qv;
+                    if (a.epi == 0) {
+                        qv = (float)Ln;
+                    } else {
+                        const double m = (double)R(n, 0, 0, k);
+                        const double qn = a.c0 != 0.0 ? a.c0 * (double)m0v[n] + a.c1 * (m + a.dt * Ln)
+                                                      : a.c1 * (m + a.dt * Ln);
+                        qv = (float)qn;
+                    }
+                    float *o = a.out[n];
+                    o[id] = qv;
+                    if (a.epi != 0 && a.fill) {
+                        // periodic images (lbcu_p4 for the periodic case)
+                        int ix = 0, jy = 0;
+                        if (i >= nx - 4) ix = i - mx;
+                        else if (i <= 5) ix = i + mx;
+                        if (THREE_D) {
+                            if (j >= ny - 4) jy = j - my;
+                            else if (j <= 5) jy = j + my;
+                        }
+                        if (ix) o[kb + (size_t)(ix - 1) + sy * (j - 1)] = qv;
+                        if (jy) {
+                            o[kb + (size_t)(i - 1) + sy * (jy - 1)] = qv;
+                            if (ix) o[kb + (size_t)(ix - 1) + sy * (jy - 1)] = qv;
+                        }
+                    }
+                }
+            }
+            // the next level's store overwrites the ring slot of level k-2
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s)
+{
+    MomF c;
+    if (mom_ord == 4) {
+        c.c1s = 4.0f / 3.0f;
+        c.c2s = 1.0f / 3.0f;
+        c.c1c = 9.0f / 8.0f;
+        c.c2c = 1.0f / 8.0f;
+    } else {
+        c.c1s = 1.0f;
+        c.c2s = 0.0f;
+        c.c1c = 1.0f;
+        c.c2c = 0.0f;
+    }
+    c.dx1 = (float)(1.0 / g.dx);
+    c.dx2 = (float)(1.0 / (2.0 * g.dx));
+    c.dx3 = (float)(1.0 / (3.0 * g.dx));
+    c.dy1 = (float)(1.0 / g.dy);
+    c.dy2 = (float)(1.0 / (2.0 * g.dy));
+    c.dy3 = (float)(1.0 / (3.0 * g.dy));
+    c.lavisc = lavisc;
+    const int kchunk = g.nz < 16 ? g.nz : 16;
+    const int nchunk = (g.nz + kchunk - 1) / kchunk;
+    if (g.ny > 1) {
+        constexpr int TX = 32, TY = 8;
+        const int ntx = (g.nx - 5 + TX - 1) / TX, nty = (g.ny - 5 + TY - 1) / TY;
+        const int nwork = ntx * nty * nchunk;
+        hipLaunchKernelGGL((momentum_batch_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, c, a, kchunk, ntx,
+                           nwork);
+    } else {
+        constexpr int TX = 64, TY = 1;
+        const int ntx = (g.nx - 5 + TX - 1) / TX;
+        const int nwork = ntx * nchunk;
+        hipLaunchKernelGGL((momentum_batch_kernel<TX, TY>), dim3(nwork), dim3(TX), 0, s, g, c, a, kchunk, ntx,
+                           nwork);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mimadv

@@ -289,162 +289,6 @@ __global__ __launch_bounds__(256) void momentum_fp64_kernel(GridDev g, MomCoefs 
     wa[id] = tw;
 }
 
-// Batched momentum stage: FP32 momenta advected by the FP32 velocities of
-// the batch, FP64 arithmetic on the loaded values.  epi 0: out = -(div), the
-// tendency the caller of qnadv_uvw forms (momentum.f90 subtracts winda);
-// epi 1: out = c0*m0 + c1*(m + dt*(-div)) rounded once to FP32, with the
-// periodic images of the cell.
-//
-// Column-marching form: a workgroup owns a TX x TY tile of columns and a
-// chunk of levels.  Every operand the stencils of level k read (the three
-// momenta and the three advecting velocities at levels k-2..k+2, columns
-// within two cells of the tile) sits in a five-level LDS ring of FP32
-// tiles with a 2-cell halo; each level is loaded once, coalesced,
-// and serves five output levels.  The per-cell formulas are those of the
-// FP64 kernel (mom_cell), so results do not depend on the mapping.  One
-// work item per workgroup.  Tiles of 32x8 columns keep the ring at 51.8 KB,
-// so three groups share a CU; the 168-VGPR cap that gives each SIMD three
-// waves spills a few values and still wins (2.9 -> 2.5 ms per stage at
-// 512x512x160 against 64x4 tiles at two waves).  The ring offsets are
-// recomputed per load rather than held, which halves the spills.
-template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) void momentum_march_kernel(
-    GridDev g, MomCoefs c, int lavisc, MomBatchArgs a, int kchunk, int ntx, int nwork)
-{
-    constexpr bool THREE_D = TY > 1;
-    constexpr int NT = TX * TY;
-    constexpr int H = 2, LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1, PL = LW * LH;
-    constexpr int NR = 5, TS = (PL + NT - 1) / NT;
-    __shared__ float ring[6][NR][PL];  // mu, mv, mw, ut, vt, wt
-
-    const int nx = g.nx, ny = g.ny, nz = g.nz, mx = nx - 5, my = ny - 5;
-    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
-    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-    const int nchunk = (nz + kchunk - 1) / kchunk;
-    const float *src[6] = {a.m[0], a.m[1], a.m[2], a.ut, a.vt, a.wt};
-
-    if (const int wi = blockIdx.x; wi < nwork) {
-        const int tile = wi / nchunk, kc = wi - tile * nchunk;
-        const int i0 = 4 + (tile % ntx) * TX;
-        const int j0 = THREE_D ? 4 + (tile / ntx) * TY : 1;
-        const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
-        const int i = i0 + tx, j = j0 + ty;
-        const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
-        // plane offset of ring entry s of this thread (periodic images for
-        // positions outside the interior; -1: no entry), recomputed where it
-        // is used rather than held in registers through the level loop
-        auto t_off = [&](int s) -> int {
-            const int e = tid + s * NT;
-            if (e >= PL) return -1;
-            const int r = e / LW, cc = e - r * LW;
-            // ghost columns are read as the caller filled them, like the
-            // reference; positions past the array (padding of a partial
-            // tile) take a periodic image that feeds no interior cell
-            int ii = i0 - H + cc;
-            if (ii > nx) ii = (ii - 4) % mx + 4;
-            int jj = 1;
-            if (THREE_D) {
-                jj = j0 - H + r;
-                if (jj > ny) jj = (jj - 4) % my + 4;
-            }
-            return (ii - 1) + nx * (jj - 1);
-        };
-        // level kk of the six operands into registers (zeros outside 1..nz),
-        // then into its ring slot
-        float pre[6][TS];
-        auto fetch = [&](int kk) {
-            const bool in = kk >= 1 && kk <= nz;
-            const size_t kb = in ? sz * (kk - 1) : 0;
-#pragma unroll
-            for (int s = 0; s < TS; ++s) {
-                const int off = t_off(s);
-#pragma unroll
-                for (int n = 0; n < 6; ++n) pre[n][s] = (in && off >= 0) ? src[n][kb + off] : 0.0f;
-            }
-        };
-        auto store = [&](int kk) {
-            const int slot = (kk + 2 * NR) % NR;
-#pragma unroll
-            for (int s = 0; s < TS; ++s) {
-                if (tid + s * NT >= PL) continue;
-#pragma unroll
-                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = pre[n][s];
-            }
-        };
-#pragma unroll 1
-        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) {
-            fetch(kk);
-            store(kk);
-        }
-        fetch(k0 + 2);
-#pragma unroll 1
-        for (int k = k0; k <= k1; ++k) {
-            // level k+2 arrived during the previous level's arithmetic; the
-            // slot it takes (level k-3) was last read before the barrier
-            // that ended that level
-            store(k + 2);
-            __syncthreads();
-            if (k < k1) fetch(k + 3);
-            if (valid) {
-                const size_t kb = sz * (k - 1);
-                const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
-                // the step-start momenta, requested before the cell arithmetic
-                float m0v[3] = {0.0f, 0.0f, 0.0f};
-                if (a.epi != 0 && a.c0 != 0.0) {
-#pragma unroll
-                    for (int n = 0; n < 3; ++n)
-                        if (n != 1 || THREE_D) m0v[n] = a.m0[n][id];
-                }
-                // ring offset of level kk (k-2 <= kk <= k+2)
-                auto lev = [&](int kk) { return ((kk + 2 * NR) % NR) * PL; };
-                auto acc = [&](int n) {
-                    return [&, n](int ii, int jj, int kk) {
-                        const int e = lev(kk) + (THREE_D ? (jj - j0 + H) * LW : 0) + (ii - i0 + H);
-                        return (double)ring[n][0][e];
-                    };
-                };
-                double d[3];
-                mom_cell(g, c, lavisc, true, i, j, k, acc(0), acc(1), acc(2), acc(3), acc(4), acc(5), d[0], d[1],
-                         d[2]);
-                const int ce = lev(k) + (THREE_D ? (ty + H) * LW : 0) + tx + H;
-#pragma unroll
-                for (int n = 0; n < 3; ++n) {
-                    if (n == 1 && !THREE_D) continue;
-                    const double L = -d[n];
-                    float qv;
-                    if (a.epi == 0) {
-                        qv = (float)L;
-                    } else {
-                        const double m = (double)ring[n][0][ce];
-                        const double qn = a.c0 != 0.0 ? a.c0 * (double)m0v[n] + a.c1 * (m + a.dt * L)
-                                                      : a.c1 * (m + a.dt * L);
-                        qv = (float)qn;
-                    }
-                    float *o = a.out[n];
-                    o[id] = qv;
-                    if (a.epi != 0 && a.fill) {
-                        // periodic images (qnbc_p4 / lbcu_p4 for the periodic case)
-                        int ix = 0, jy = 0;
-                        if (i >= nx - 4) ix = i - mx;
-                        else if (i <= 5) ix = i + mx;
-                        if (THREE_D) {
-                            if (j >= ny - 4) jy = j - my;
-                            else if (j <= 5) jy = j + my;
-                        }
-                        if (ix) o[kb + (size_t)(ix - 1) + sy * (j - 1)] = qv;
-                        if (jy) {
-                            o[kb + (size_t)(i - 1) + sy * (jy - 1)] = qv;
-                            if (ix) o[kb + (size_t)(ix - 1) + sy * (jy - 1)] = qv;
-                        }
-                    }
-                }
-            }
-            // the next level's load overwrites the ring slot of level k-2
-            __syncthreads();
-        }
-    }
-}
-
 MomCoefs mom_coefs(int mom_ord)
 {
     MomCoefs c;
@@ -469,27 +313,6 @@ hipError_t launch_momentum_fp64(const GridDev &g, int mom_ord, int lavisc, int n
     dim3 grid((g.nx + 255) / 256, g.ny, g.nz);
     hipLaunchKernelGGL(momentum_fp64_kernel, grid, block, 0, s, g, c, lavisc, ntime, mu, mv, mw, ut, vt, wt, ua, va,
                        wa);
-    return hipGetLastError();
-}
-
-hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, const MomBatchArgs &a, hipStream_t s)
-{
-    const MomCoefs c = mom_coefs(mom_ord);
-    const int kchunk = g.nz < 16 ? g.nz : 16;
-    const int nchunk = (g.nz + kchunk - 1) / kchunk;
-    if (g.ny > 1) {
-        constexpr int TX = 32, TY = 8;
-        const int ntx = (g.nx - 5 + TX - 1) / TX, nty = (g.ny - 5 + TY - 1) / TY;
-        const int nwork = ntx * nty * nchunk;
-        hipLaunchKernelGGL((momentum_march_kernel<TX, TY>), dim3(nwork), dim3(TX * TY), 0, s, g, c, lavisc, a, kchunk,
-                           ntx, nwork);
-    } else {
-        constexpr int TX = 64, TY = 1;
-        const int ntx = (g.nx - 5 + TX - 1) / TX;
-        const int nwork = ntx * nchunk;
-        hipLaunchKernelGGL((momentum_march_kernel<TX, TY>), dim3(nwork), dim3(TX), 0, s, g, c, lavisc, a, kchunk, ntx,
-                           nwork);
-    }
     return hipGetLastError();
 }
 

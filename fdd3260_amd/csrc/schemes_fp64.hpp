@@ -17,6 +17,7 @@ namespace mimadv {
 struct ZMetrics {
     const double *fdz0, *dzw, *dz1w, *dz1;
     const double *ppmc;  // [nz][12] PPM non-uniform coefficients per level
+    const float *momz;   // [nz][MOMZ_N] level coefficients of the FP32 batched momentum operator
     double dz;
 };
 

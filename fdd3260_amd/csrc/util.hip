@@ -1,4 +1,7 @@
 // util.hip -- ghost fill, stage combine and interior reductions.
+#include <algorithm>
+#include <cmath>
+
 #include "mimadv_internal.hpp"
 
 namespace mimadv {
@@ -84,6 +87,50 @@ __global__ void mirror_y_kernel(int nx, int ny, int nz, int bc_s, int bc_n, doub
         c[sy * (ny - 1)] = c[sy * (ny - 5)];
         c[sy * (ny - 2)] = c[sy * (ny - 4)];
     }
+}
+
+// lbcu_p4 side fills of the normal wind component (bc.f90:255-281 for u in
+// x, :307-333 for v in y): 'nnf' mirrors with a sign change, 'ope' copies.
+// stride 1 for u along x, nx for v along y; one thread per line.
+__global__ void normal_fill_kernel(int n, size_t stride, size_t nlines, size_t line_step_a, int line_count_a,
+                                   size_t line_step_b, int bc_lo, int bc_hi, double *x)
+{
+    size_t l = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    const size_t la = l % (size_t)line_count_a, lb = l / (size_t)line_count_a;
+    double *r = x + la * line_step_a + lb * line_step_b;
+    auto X = [&](int i) -> double & { return r[stride * (size_t)(i - 1)]; };
+    if (bc_lo == 1 || bc_lo == 2) {
+        const double sg = bc_lo == 1 ? -1.0 : 1.0;
+        X(3) = sg * X(4);
+        X(2) = sg * X(5);
+        X(1) = sg * X(6);
+    }
+    if (bc_hi == 1 || bc_hi == 2) {
+        const double sg = bc_hi == 1 ? -1.0 : 1.0;
+        X(n) = sg * X(n - 3);
+        X(n - 1) = sg * X(n - 2);
+    }
+}
+
+// integ_mom (integpack.f90:54-135) for one component: the integ_scal
+// combine, with the step scaled by the sponge factor tw for w (tw = 1 for
+// u and v)
+__global__ void integ_mom_kernel(long n, int sorder, double a_lm, double b_lm, double dt, double tw,
+                                 const double *xold, const double *__restrict__ a, double *__restrict__ tend,
+                                 double *x)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double upd;
+    if (sorder > 1) {
+        double t = a[i] + b_lm * tend[i];
+        tend[i] = t;
+        upd = xold[i] + a_lm * dt * t;
+    } else {
+        upd = xold[i] + dt * a[i];
+    }
+    x[i] = tw == 1.0 ? upd : tw * upd;
 }
 
 __global__ void integ_kernel(long n, int sorder, double a_lm, double b_lm, double dt, const double *__restrict__ xold,
@@ -205,6 +252,86 @@ hipError_t launch_fill_scalar(const GridDev &g, double *x, hipStream_t s)
     if (three_d && (g.bcl[2] != 0 || g.bcl[3] != 0))
         hipLaunchKernelGGL(mirror_y_kernel, dim3((nx + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, g.bcl[2], g.bcl[3], x);
     return hipGetLastError();
+}
+
+hipError_t launch_fill_winds(const GridDev &g, double *u, double *v, double *w, hipStream_t s)
+{
+    int nx = g.nx, ny = g.ny, nz = g.nz;
+    bool three_d = ny > 1;
+    double *f[3] = {u, three_d ? v : nullptr, w};
+    // periodic copies in x, y, z (each pass sees the previous ones), then
+    // the normal-component fills of the non-periodic sides
+    for (double *x : f)
+        if (x && (g.bcl[0] == 0 || g.bcl[1] == 0))
+            hipLaunchKernelGGL(fill_x_kernel, dim3((ny + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, 0, 0, x);
+    for (double *x : f)
+        if (x && three_d && (g.bcl[2] == 0 || g.bcl[3] == 0))
+            hipLaunchKernelGGL(fill_y_kernel, dim3((nx + 63) / 64, nz), dim3(64), 0, s, nx, ny, nz, x);
+    for (double *x : f)
+        if (x && (g.bcl[4] == 0 || g.bcl[5] == 0)) {
+            size_t n = (size_t)nx * ny;
+            hipLaunchKernelGGL(fill_z_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, nx, ny, nz, x);
+        }
+    if (g.bcl[0] != 0 || g.bcl[1] != 0) {
+        // u along x: one line per (j, k)
+        size_t nl = (size_t)ny * nz;
+        hipLaunchKernelGGL(normal_fill_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, nx, (size_t)1,
+                           nl, (size_t)nx, ny, (size_t)nx * ny, g.bcl[0], g.bcl[1], u);
+    }
+    if (three_d && (g.bcl[2] != 0 || g.bcl[3] != 0)) {
+        // v along y: one line per (i, k)
+        size_t nl = (size_t)nx * nz;
+        hipLaunchKernelGGL(normal_fill_kernel, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, ny, (size_t)nx,
+                           nl, (size_t)1, nx, (size_t)nx * ny, g.bcl[2], g.bcl[3], v);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_integ_mom_one(const GridDev &g, int comp, int sorder, int im, double dt, double tw,
+                                const double *old, const double *a, double *tend, double *x, hipStream_t s)
+{
+    (void)comp;
+    double a_lm = 1.0, b_lm = 0.0;
+    if (sorder > 1 && im != 1) {
+        a_lm = 0.5;
+        b_lm = 1.0;
+    }
+    long n = (long)g.nx * g.ny * g.nz;
+    unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(integ_mom_kernel, dim3(nb), dim3(256), 0, s, n, sorder, a_lm, b_lm, dt, tw, old, a, tend, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_integ_mom(const GridDev &g, int sorder, int im, double dt, double tw, const double *const *old,
+                            const double *const *a, double *const *tend, double *const *x, hipStream_t s)
+{
+    for (int c = 0; c < 3; ++c) {
+        if (c == 1 && g.ny == 1) continue;
+        hipError_t e = launch_integ_mom_one(g, c, sorder, im, dt, c == 2 ? tw : 1.0, old[c], a[c],
+                                            tend ? tend[c] : nullptr, x[c], s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+double sponge_factor(int nz, int sorder, int im, double dt, const double *z0, double zmax, double zdamp, int sponge)
+{
+    // integpack.f90:83-96 as the reference runs it: the level loop assigns
+    // the whole tauw array, so the last level at or above zdamp sets one
+    // factor for every level (pi and taud are the reference's constants)
+    const double pi = 3.14159265, taud = 0.25;
+    const double a_lm = (sorder > 1 && im != 1) ? 0.5 : 1.0;
+    double tauw = 1.0;
+    if (sponge == 2) {
+        const double H = zmax - zdamp;
+        for (int k = 0; k < nz; ++k)
+            if (z0[k] >= zdamp) {
+                const double c = std::cos(0.5 * pi * (zmax - z0[k]) / H);
+                tauw = taud * std::max(std::min(std::pow(c, 2.0), 1.0), 0.0);
+            }
+        tauw = 1.0 / (1.0 + a_lm * dt * tauw);
+    }
+    return tauw;
 }
 
 hipError_t launch_integ_scal(long n, int sorder, int im, double dt, const double *xold, const double *a,

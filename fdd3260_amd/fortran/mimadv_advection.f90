@@ -28,6 +28,7 @@ module mimadv_advection
   public :: advection_x, advection_y, advection_z, advection_ac_arrays, qnadv_uvw_arrays
   public :: advection_ac_1d_arrays
   public :: mimadv_set_density, get_momentum_arrays, get_velocities_arrays, get_advective_arrays, get_conserved_array
+  public :: field_addr, fields_tendency
 
   integer(c_int), parameter :: MIMADV_DIR_X = 1, MIMADV_DIR_Y = 2, MIMADV_DIR_Z = 4
   integer(c_int), parameter :: MIMADV_HOST_PTRS = 16
@@ -200,6 +201,11 @@ contains
       o%muscl_limiter = 3
     end select
     o%ppm_limiter = merge(1, 0, trim(ppmlimiter) == 'PPM')
+    ! advection_init: limit = .false. also sets ppmlimiter = 'no-limit'
+    ! (advection.f90:1056)
+    if (present(limit)) then
+      if (.not. limit) o%ppm_limiter = 0
+    endif
     o%lim_tol = lim_tol
     o%weno_bottom = 0
     o%mom_ord = mom_ord
@@ -247,6 +253,37 @@ contains
     if (dirflag == MIMADV_DIR_Z) pw = c_loc(vel)
     call check(c_scalar_tendency(ctx, dtl, pu, pv, pw, 1_c_int, c_loc(qp), c_loc(xp), c_loc(af), &
                                  ior(ior(dirflag, MIMADV_HOST_PTRS), line_flags(lim))), 'advection')
+  end subroutine
+
+  ! Address of a whole field (the actual argument must be contiguous: an
+  ! allocatable array or an array component of one).
+  type(c_ptr) function field_addr(x)
+    real(8), target, intent(in) :: x(g_nx * g_ny * g_nz)
+    field_addr = c_loc(x)
+  end function
+
+  ! nf fields in one call: xs_f = ((xs_f - X_f) - Y_f) - Z_f over the
+  ! directions of dirmask (overwrite: xs_f = ((0 - X_f) ...)), host arrays
+  ! given by address; adv(f) the advective form of field f.  u, v, w are
+  ! the face velocities of the directions used (the others may be any
+  ! field of the grid's shape).
+  subroutine fields_tendency(dtl, u, v, w, nf, q, xs, adv, dirmask, lim, overwrite)
+    real(8), intent(in) :: dtl
+    real(8), target, intent(in) :: u(g_nx, g_ny, g_nz), v(g_nx, g_ny, g_nz), w(g_nx, g_ny, g_nz)
+    integer, intent(in) :: nf, dirmask
+    type(c_ptr), target, intent(in) :: q(nf), xs(nf)
+    logical, intent(in) :: adv(nf), lim, overwrite
+    integer(c_int), target :: af(max(nf, 1))
+    integer(c_int) :: flags
+    integer :: f
+    if (nf <= 0) return
+    do f = 1, nf
+      af(f) = merge(1, 0, adv(f))
+    enddo
+    flags = ior(ior(int(dirmask, c_int), MIMADV_HOST_PTRS), line_flags(lim))
+    if (overwrite) flags = ior(flags, 8_c_int)
+    call check(c_scalar_tendency(ctx, dtl, c_loc(u), c_loc(v), c_loc(w), int(nf, c_int), c_loc(q), c_loc(xs), &
+                                 c_loc(af), flags), 'advection')
   end subroutine
 
   ! advection_x (advection.f90:556): xs = xs - (x flux divergence of xtmp)

@@ -160,6 +160,28 @@ int mimadv_integ_scal(mimadv_ctx *ctx, int sorder, int im, double dt, int nf,
                       const double *const *xold, const double *const *a, double *const *tend,
                       double *const *x, unsigned flags);
 
+/* Ghost fill of the winds u, v, w (lbcu_p4, bc.f90:81-344, serial branch
+ * of a single tile): periodic images in x, y and z per the context's bcl,
+ * then on 'nnf' / 'ope' sides the normal component (u in x, v in y) is
+ * mirrored with a sign change / copied into the ghosts (u(3:1) =
+ * +-u(4:6), u(nx-1:nx) = +-u(nx-2), +-u(nx-3)); the tangential components
+ * keep their ghost values there, as in the reference's single-tile run.
+ * v is unused when ny == 1. */
+int mimadv_fill_winds(mimadv_ctx *ctx, double *u, double *v, double *w, unsigned flags);
+
+/* Momentum stage combine integ_mom (integpack.f90:54-135) with integ.h, on
+ * the component triples old/a/tend/x = (u, v, w) (whole arrays; v unused in
+ * 2D; x may alias old): sorder > 1: tend = a + b_lm*tend, x = old +
+ * a_lm*dt*tend; sorder == 1: x = old + dt*a; w is then multiplied by the
+ * implicit sponge factor when sponge == 2: tauw = 0.25*clamp(cos(pi/2*(zmax -
+ * z0(k))/(zmax - zdamp))^2) of the LAST level k with z0(k) >= zdamp (the
+ * reference's level loop assigns the whole tauw array), factor 1/(1 +
+ * a_lm*dt*tauw) for every level (1/(1 + a_lm*dt) when no level reaches
+ * zdamp), pi = 3.14159265.  z0 is a host array of nz level heights. */
+int mimadv_integ_mom(mimadv_ctx *ctx, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                     int sponge, const double *const *old, const double *const *a, double *const *tend,
+                     double *const *x, unsigned flags);
+
 /* totalav (averages.f90:507-551): unweighted interior mean of x; also the
  * dzw-weighted interior sum (the quantity flux form conserves). */
 int mimadv_totalav(mimadv_ctx *ctx, const double *x, double *mean, double *dzw_sum, unsigned flags);
@@ -223,9 +245,14 @@ int mimadv_advection_1d(mimadv_ctx *ctx, const double *w0, const double *dens, i
 /* q[0], q[1], q[2] (MIMADV_F32) are the momenta rho0*u, rho0*v, rho0*w,
  * advanced with the momentum operator of qnadv_uvw (windadv.f90:46-1127;
  * the context's mom_ord and lavisc, directions in the even-ntime order) by
- * the batch's velocities instead of with the scalar scheme; FP64 arithmetic
- * on the FP32 values, one rounding per stage.  In 2D q[1] is not touched. */
+ * the batch's velocities instead of with the scalar scheme; FP32
+ * arithmetic with the level metrics as precomputed factors, the stage
+ * update in FP64 rounded once; 'nnf' floor and lid.  In 2D q[1] is not
+ * touched. */
 #define MIMADV_MOMENTUM 64u
+/* Record an event pair around every launch of the call (eager launches
+ * only; not with MIMADV_GRAPH); mimadv_batch_kernel_ms reads them. */
+#define MIMADV_PROFILE 1024u
 
 /* Tuning of the batched launches; 0 selects the default of each field. */
 typedef struct mimadv_tuning {
@@ -255,6 +282,12 @@ int mimadv_batch_tendency(mimadv_ctx *ctx, const float *u, const float *v, const
  * MIMADV_GRAPH. */
 int mimadv_batch_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const float *u, const float *v,
                          const float *w, int nf, const int *precision, void *const *q, unsigned flags);
+
+/* Kernel time of the launches of the last batched call made with
+ * MIMADV_PROFILE, summed per kind: [0] momenta, [1] FP64 fields, [2] FP32
+ * fields (ms, from the events on the context stream), and the number of
+ * launches of each kind.  Synchronises with the last launch. */
+int mimadv_batch_kernel_ms(mimadv_ctx *ctx, double ms[3], int count[3]);
 
 /* Domain budgets: dzw-weighted interior sums of nf fields, accumulated in
  * FP64 (out[f]). */

@@ -1464,6 +1464,12 @@ void orc_fill_scalar(const orc_grid *g, double *x)
 #undef X3
 }
 
+/* lbcu_p4 (bc.f90:81-344), serial branch of a single-tile run (it_start =
+ * jt_start = 4, no nesting, no CHANNEL): periodic copies of u, v, w in x,
+ * y and z, then the no-flux ('nnf', negated) and open ('ope') fills of the
+ * normal component on non-periodic sides.  The tangential fills of those
+ * sides need it_start == 3 (jt_start == 3), which a single tile never has,
+ * so v, w (u, w) keep their ghost values there, as in the reference. */
 void orc_fill_winds(const orc_grid *g, double *u, double *v, double *w)
 {
     int nx = g->nx, ny = g->ny, nz = g->nz;
@@ -1495,6 +1501,50 @@ void orc_fill_winds(const orc_grid *g, double *u, double *v, double *w)
                     x[IDX(g, i, ny - 1, k)] = x[IDX(g, i, 4, k)];
                 }
         }
+    if (g->bcl[4] == ORC_BC_PER || g->bcl[5] == ORC_BC_PER)
+        for (int c = 0; c < 3; ++c) {
+            double *x = f[c];
+            if (!x) continue;
+            for (int j = 1; j <= ny; ++j)
+                for (int i = 1; i <= nx; ++i) {
+                    x[IDX(g, i, j, 1)] = x[IDX(g, i, j, nz - 4)];
+                    x[IDX(g, i, j, 2)] = x[IDX(g, i, j, nz - 3)];
+                    x[IDX(g, i, j, 3)] = x[IDX(g, i, j, nz - 2)];
+                    x[IDX(g, i, j, nz)] = x[IDX(g, i, j, 5)];
+                    x[IDX(g, i, j, nz - 1)] = x[IDX(g, i, j, 4)];
+                }
+        }
+    /* west / east sides: u is the normal component (bc.f90:255-281) */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = 1; j <= ny; ++j) {
+            double s = g->bcl[0] == ORC_BC_NNF ? -1.0 : 1.0;
+            if (g->bcl[0] == ORC_BC_NNF || g->bcl[0] == ORC_BC_OPE) {
+                u[IDX(g, 3, j, k)] = s * u[IDX(g, 4, j, k)];
+                u[IDX(g, 2, j, k)] = s * u[IDX(g, 5, j, k)];
+                u[IDX(g, 1, j, k)] = s * u[IDX(g, 6, j, k)];
+            }
+            s = g->bcl[1] == ORC_BC_NNF ? -1.0 : 1.0;
+            if (g->bcl[1] == ORC_BC_NNF || g->bcl[1] == ORC_BC_OPE) {
+                u[IDX(g, nx, j, k)] = s * u[IDX(g, nx - 3, j, k)];
+                u[IDX(g, nx - 1, j, k)] = s * u[IDX(g, nx - 2, j, k)];
+            }
+        }
+    /* south / north sides: v is the normal component (bc.f90:307-333) */
+    if (three_d)
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 1; i <= nx; ++i) {
+                double s = g->bcl[2] == ORC_BC_NNF ? -1.0 : 1.0;
+                if (g->bcl[2] == ORC_BC_NNF || g->bcl[2] == ORC_BC_OPE) {
+                    v[IDX(g, i, 3, k)] = s * v[IDX(g, i, 4, k)];
+                    v[IDX(g, i, 2, k)] = s * v[IDX(g, i, 5, k)];
+                    v[IDX(g, i, 1, k)] = s * v[IDX(g, i, 6, k)];
+                }
+                s = g->bcl[3] == ORC_BC_NNF ? -1.0 : 1.0;
+                if (g->bcl[3] == ORC_BC_NNF || g->bcl[3] == ORC_BC_OPE) {
+                    v[IDX(g, i, ny, k)] = s * v[IDX(g, i, ny - 3, k)];
+                    v[IDX(g, i, ny - 1, k)] = s * v[IDX(g, i, ny - 2, k)];
+                }
+            }
 }
 
 /* ======================================================================
@@ -1515,6 +1565,57 @@ void orc_integ_scal(long n, int sorder, int im, double dt, const double *xold,
     } else {
         for (long i = 0; i < n; ++i) x[i] = xold[i] + dt * a[i];
     }
+}
+
+/* integ_mom (integpack.f90:54-135) with integ.h: the momentum stage of the
+ * winds, old/x/a/tend = (u, v, w) triples of whole arrays (v unused in 2D).
+ * The implicit sponge (sponge == 2) follows the reference literally: the
+ * loop over levels at or above zdamp assigns the WHOLE tauw array (tauw =
+ * taud*clamp(cos(pi/2 (zmax - z0(k))/H)^2), pi = 3.14159265, taud = 0.25),
+ * so the last such level sets one factor 1/(1 + a_lm dt tauw) for every
+ * level, and with no level above zdamp tauw stays 1 before that step. */
+double orc_sponge_factor(int nz, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                         int sponge)
+{
+    const double pi = 3.14159265, taud = 0.25;
+    double a_lm = (sorder > 1 && im != 1) ? 0.5 : 1.0;
+    double tauw = 1.0;
+    if (sponge == 2) {
+        double H = zmax - zdamp;
+        for (int k = 1; k <= nz; ++k)
+            if (L(z0, k) >= zdamp) {
+                double c = cos(0.5 * pi * (zmax - L(z0, k)) / H);
+                tauw = taud * dmax(dmin(pow(c, 2.0), 1.0), 0.0);
+            }
+        tauw = 1.0 / (1.0 + a_lm * dt * tauw);
+    }
+    return tauw;
+}
+
+void orc_integ_mom(const orc_grid *g, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                   int sponge, double *const *old, const double *const *a, double *const *tend, double *const *x)
+{
+    size_t n = (size_t)g->nx * g->ny * g->nz, plane = (size_t)g->nx * g->ny;
+    double a_lm, b_lm;
+    if (sorder > 1) {
+        if (im == 1) { a_lm = 1.0; b_lm = 0.0; }
+        else { a_lm = 0.5; b_lm = 1.0; }
+    } else {
+        a_lm = 1.0;
+        b_lm = 0.0;
+    }
+    double tw = orc_sponge_factor(g->nz, sorder, im, dt, z0, zmax, zdamp, sponge);
+    int three_d = g->ny > 1;
+    for (int c = 0; c < 3; ++c) {
+        if (c == 1 && !three_d) continue;
+        if (sorder > 1)
+            for (size_t i = 0; i < n; ++i) tend[c][i] = a[c][i] + b_lm * tend[c][i];
+        for (size_t i = 0; i < n; ++i) {
+            double upd = sorder > 1 ? old[c][i] + a_lm * dt * tend[c][i] : old[c][i] + dt * a[c][i];
+            x[c][i] = c == 2 ? tw * upd : upd;
+        }
+    }
+    (void)plane;
 }
 
 double orc_totalav(const orc_grid *g, const double *x)

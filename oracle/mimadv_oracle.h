@@ -13,7 +13,7 @@
  *   src/TOOLS/advq_muscl.f90  advs{x,y,z}_muscl, limiter_muscl
  *   src/TOOLS/advection.f90   advection_{x,y,z} (:556-798), advection_ac (:801-869)
  *   src/INTEG/windadv.f90     qnadv_uvw and adv{u,v,w}{x,y,z}_fd
- *   src/TOOLS/bc.f90          qnbc_p4 (:378-503), lbcu_p4 (:81-144)
+ *   src/TOOLS/bc.f90          qnbc_p4 (:378-503), lbcu_p4 (:81-344)
  *   src/INTEG/integpack.f90 + INCLUDE/integ.h   Heun / Euler stage combine
  *   src/TOOLS/averages.f90    totalav (:507-551)
  *   src/TOOLS/funcpack.f90    get_momentum/get_velocities/get_advective/
@@ -106,8 +106,12 @@ void orc_qnadv_uvw(const orc_grid *g, const orc_opts *o, int ntime,
 
 /* Ghost fill for one scalar (qnbc_p4, serial branch). */
 void orc_fill_scalar(const orc_grid *g, double *x);
-/* Ghost fill for winds, periodic lateral part of lbcu_p4 (serial branch). */
+/* Ghost fill for winds: lbcu_p4 of a single-tile run (bc.f90:81-344). */
 void orc_fill_winds(const orc_grid *g, double *u, double *v, double *w);
+double orc_sponge_factor(int nz, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                         int sponge);
+void orc_integ_mom(const orc_grid *g, int sorder, int im, double dt, const double *z0, double zmax, double zdamp,
+                   int sponge, double *const *old, const double *const *a, double *const *tend, double *const *x);
 
 /* Stage combine of integ_scal for one field:
  *   sorder > 1:  tend = a + b_lm*tend ; x = xold + a_lm*dt*tend

@@ -69,6 +69,10 @@ def load(path=None):
     lib.orc_fill_scalar.argtypes = [G, dp]
     lib.orc_fill_winds.argtypes = [G, dp, dp, dp]
     lib.orc_integ_scal.argtypes = [C.c_long, i, i, d, dp, dp, dp, dp]
+    ppd = C.POINTER(dp)
+    lib.orc_integ_mom.argtypes = [G, i, i, d, dp, d, d, i, ppd, ppd, ppd, ppd]
+    lib.orc_sponge_factor.argtypes = [i, i, i, d, dp, d, d, i]
+    lib.orc_sponge_factor.restype = d
     lib.orc_totalav.argtypes = [G, dp]
     lib.orc_totalav.restype = d
     lib.orc_tlm_weno_tendency.argtypes = [G, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
@@ -209,6 +213,18 @@ class Oracle:
 
     def integ_scal(self, sorder, im, dt, xold, a, tend, x):
         self.lib.orc_integ_scal(xold.size, int(sorder), int(im), float(dt), _dp(xold), _dp(a), _dp(tend), _dp(x))
+
+    def integ_mom(self, sorder, im, dt, z0, zmax, zdamp, sponge, old, a, tend, x):
+        """integ_mom on (u, v, w) triples; x and tend are updated in place."""
+        z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        arr = lambda L: (C.POINTER(C.c_double) * 3)(*[_dp(t) for t in L])
+        self.lib.orc_integ_mom(C.byref(self.g), int(sorder), int(im), float(dt), _dp(z0), float(zmax), float(zdamp),
+                               int(sponge), arr(old), arr(a), arr(tend), arr(x))
+
+    def sponge_factor(self, sorder, im, dt, z0, zmax, zdamp, sponge):
+        z0 = np.ascontiguousarray(z0, dtype=np.float64)
+        return self.lib.orc_sponge_factor(len(z0), int(sorder), int(im), float(dt), _dp(z0), float(zmax),
+                                          float(zdamp), int(sponge))
 
     def totalav(self, x):
         return self.lib.orc_totalav(C.byref(self.g), _dp(x))

@@ -180,9 +180,9 @@ def _momenta(g, u, v, w, seed=41):
 def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
     """The three FP32 momenta in the batch (MIMADV_MOMENTUM) against the
     oracle's qnadv_uvw on the same FP32-rounded inputs: the kernel evaluates
-    the reference formulas in FP64, so only the final rounding to FP32
-    separates them (held to 1e-7 of the field maximum).  The scalar members
-    of the same batch keep their own bounds."""
+    the reference formulas in FP32 with precomputed level factors (held to
+    2e-6 of the field maximum, 5e-7 relative RMS).  The scalar members of
+    the same batch keep their own bounds."""
     print(f"case batch_momentum_tendency ord={mom_ord} lavisc={lavisc}", flush=True)
     g, u, v, w, f64, f32 = _case(nspec=3)
     mom = _momenta(g, u, v, w)
@@ -203,7 +203,9 @@ def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
         a.stream_sync()
         for c, r in enumerate((ua, va, wa)):
             got = outs[c].double().cpu().numpy()
-            assert _relmax(got, -r, g) <= 1e-7, f"momentum component {c}"
+            mx, rms = _relmax(got, -r, g), _rel_rms(got, -r, g)
+            print(f"  momentum component {c}: rel max {mx:.3e} rel rms {rms:.3e}", flush=True)
+            assert mx <= 2e-6 and rms <= 5e-7, f"momentum component {c}"
         for n, (o, r) in enumerate(zip(outs[3:], srefs)):
             got = o.double().cpu().numpy()
             if n < len(f64):
@@ -214,16 +216,15 @@ def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
 
 @pytest.mark.parametrize("nz", [24, 17, 9])
 @pytest.mark.parametrize("mom_ord,lavisc", [(4, True), (2, False)])
-def test_batch_momentum_bitwise_fp64_kernel(cuda, nz, mom_ord, lavisc):
-    """The batched momentum kernel carries each z-face flux up the column from
-    the level that owns it and reads FP32 operands; the per-cell FP64 kernel
-    of mimadv_momentum_tendency evaluates every flux it needs in place.  On
-    FP32-representable inputs both run the same FP64 operations, so the batch
-    tendency is the FP64 one negated and rounded once to FP32, bit for bit.
-    Columns march in chunks of 16 levels: nz = 17 leaves a last chunk of one
-    level, nz = 9 a single chunk.  The batched path takes the no-flux floor
-    and lid only."""
-    print(f"case batch_momentum_bitwise nz={nz} ord={mom_ord} lavisc={lavisc}", flush=True)
+def test_batch_momentum_matches_fp64_kernel(cuda, nz, mom_ord, lavisc):
+    """The batched FP32 momentum kernel against the FP64 kernel of
+    mimadv_momentum_tendency (itself bit-identical to the oracle's
+    qnadv_uvw) on FP32-representable inputs.  Columns march in chunks of 16
+    levels: nz = 17 leaves a last chunk of one level, nz = 9 a single chunk,
+    so every vertical closure line (k = 1, 2, nz-1, nz) meets chunk edges.
+    Held to 2e-6 of the field maximum at every cell; the batched path takes
+    the no-flux floor and lid only."""
+    print(f"case batch_momentum_vs_fp64 nz={nz} ord={mom_ord} lavisc={lavisc}", flush=True)
     g = gm.stretched(40, 36, nz)
     u, v, w = cases.turbulent_wind(g, seed=5, rms=3.0)
     u, v, w = (a.astype(np.float32).astype(np.float64) for a in (u, v, w))
@@ -238,12 +239,12 @@ def test_batch_momentum_bitwise_fp64_kernel(cuda, nz, mom_ord, lavisc):
         outs = [torch.zeros_like(f) for f in fields]
         a.batch_tendency(du, dv, dw, fields, outs, momentum=True)
         a.stream_sync()
-        I = g.interior()
         for c in range(3):
-            want = (-ref[c]).float().cpu().numpy()[I]
-            got = outs[c].cpu().numpy()[I]
-            nbit = int(np.count_nonzero(want.view(np.uint32) != got.view(np.uint32)))
-            assert nbit == 0, f"momentum component {c}: {nbit} cells differ"
+            want = (-ref[c]).cpu().numpy()
+            got = outs[c].double().cpu().numpy()
+            mx = _relmax(got, want, g)
+            print(f"  component {c}: rel max {mx:.3e}", flush=True)
+            assert mx <= 2e-6, f"momentum component {c}"
 
 
 @pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
@@ -459,7 +460,7 @@ def test_batch_2d_with_momentum(cuda, scheme):
         a.stream_sync()
         for c, r in ((0, ua), (2, wa)):
             got = outs[c].double().cpu().numpy()
-            assert _relmax(got, -r, g) <= 1e-6, f"momentum component {c}"
+            assert _relmax(got, -r, g) <= 2e-6, f"momentum component {c}"
         assert not bool(outs[1].any()), "q[1] is not touched in 2D"
         for n, (o, r) in enumerate(zip(outs[3:], srefs)):
             got = o.double().cpu().numpy()

@@ -360,3 +360,54 @@ def check_weno5_indicators(lib=None):
 
 def test_weno5_smoothness_indicators_closed_form():
     check_weno5_indicators()
+
+
+# ---------------------------------------------------------- integ_mom -----
+
+def check_integ_mom(lib=None):
+    """integ_mom (integpack.f90:54-135): Heun stages on q' = lambda q give
+    1 + z + z^2/2 for u and v; the implicit sponge (sponge = 2) scales w at
+    EVERY level by 1/(1 + a_lm dt tauw), tauw = 0.25 cos^2(pi/2 (zmax -
+    z0(k*))/H) of the last level k* at or above zdamp (the reference assigns
+    the whole tauw array inside its level loop), pi = 3.14159265; with no
+    level above zdamp the factor is 1/(1 + a_lm dt); sponge /= 2 leaves w
+    unscaled."""
+    import math
+
+    g = gm.stretched(10, 9, 8)
+    o = Oracle(g, lib_path=lib)
+    z0 = np.cumsum(g.dzw) - 0.5 * g.dzw
+    zmax, zdamp, dt = float(np.sum(g.dzw)), 0.6 * float(np.sum(g.dzw)), 3.0
+    H = zmax - zdamp
+    kstar = max(k for k in range(g.nz) if z0[k] >= zdamp)
+    c = math.cos(0.5 * 3.14159265 * (zmax - z0[kstar]) / H)
+    for im, a_lm in ((1, 1.0), (2, 0.5)):
+        want = 1.0 / (1.0 + a_lm * dt * (0.25 * min(max(c * c, 0.0), 1.0)))
+        assert o.sponge_factor(2, im, dt, z0, zmax, zdamp, 2) == pytest.approx(want, rel=1e-15)
+        assert o.sponge_factor(2, im, dt, z0, zmax, 10 * zmax, 2) == pytest.approx(1.0 / (1.0 + a_lm * dt),
+                                                                                   rel=1e-15)
+        assert o.sponge_factor(2, im, dt, z0, zmax, zdamp, 1) == 1.0
+    # two Heun stages of q' = lam q on every component
+    lam = -0.01
+    rng = np.random.default_rng(3)
+    q0 = [rng.standard_normal(g.shape) for _ in range(3)]
+    x = [q.copy() for q in q0]
+    tend = [np.zeros(g.shape) for _ in range(3)]
+    for im in (1, 2):
+        a = [lam * xx for xx in x]
+        o.integ_mom(2, im, dt, z0, zmax, zdamp, 0, q0, a, tend, x)
+    z = lam * dt
+    for c_ in range(3):
+        assert np.allclose(x[c_], q0[c_] * (1 + z + z * z / 2), rtol=1e-14, atol=0)
+    # sponge on: w of one Euler-like first stage scaled at every level
+    x = [q.copy() for q in q0]
+    tend = [np.zeros(g.shape) for _ in range(3)]
+    a = [lam * xx for xx in x]
+    o.integ_mom(2, 1, dt, z0, zmax, zdamp, 2, q0, a, tend, x)
+    f = o.sponge_factor(2, 1, dt, z0, zmax, zdamp, 2)
+    assert np.allclose(x[0], q0[0] * (1 + z), rtol=1e-14, atol=0)
+    assert np.allclose(x[2], f * (q0[2] * (1 + z)), rtol=1e-14, atol=0)
+
+
+def test_integ_mom_stages_and_sponge():
+    check_integ_mom()

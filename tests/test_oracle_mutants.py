@@ -89,6 +89,16 @@ MUTANTS = [
      "if (((xx - r.phil) * (r.phil - xm) < 0.0 &&", ka.check_ppm_face_fix, ()),
     ("ppm left fix to xx", "if (fabs(r.phil - xx) < fabs(r.phil - xm)) r.phil = xx;\n            else r.phil = xm;",
      "r.phil = xx;", ka.check_ppm_face_fix, ()),
+    # integ_mom and its sponge
+    ("sponge pi", "const double pi = 3.14159265, taud = 0.25;", "const double pi = 3.141592653589793, taud = 0.25;",
+     ka.check_integ_mom, ()),
+    ("sponge a_lm", "tauw = 1.0 / (1.0 + a_lm * dt * tauw);", "tauw = 1.0 / (1.0 + dt * tauw);",
+     ka.check_integ_mom, ()),
+    ("sponge first level wins", "            if (L(z0, k) >= zdamp) {\n                double c",
+     "            if (L(z0, k) >= zdamp && tauw == 1.0) {\n                double c", ka.check_integ_mom, ()),
+    ("integ_mom b_lm", "    double tw = orc_sponge_factor(g->nz, sorder, im, dt, z0, zmax, zdamp, sponge);",
+     "    b_lm = 0.5 * b_lm;\n    double tw = orc_sponge_factor(g->nz, sorder, im, dt, z0, zmax, zdamp, sponge);",
+     ka.check_integ_mom, ()),
     # WENO5 smoothness indicators
     ("weno5 beta0 second term",
      "0.25 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c)", "0.25 * (a - 4.0 * b + c) * (a - 4.0 * b + c)",

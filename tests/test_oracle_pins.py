@@ -718,8 +718,44 @@ def test_fill_winds_2d_and_open_sides():
     ref[:, g.ny - 2:, :] = ref[:, 3:5, :]
     arrs = [x.copy() for _ in range(3)]
     Oracle(g).fill_winds(*arrs)
-    for a in arrs:
+    # u, the normal component of the open x sides, is copied outwards:
+    # u(3:1) = u(4:6), u(nx-1:nx) = u(nx-2), u(nx-3) (bc.f90:255-281);
+    # v and w keep their x ghosts (the tangential fill needs it_start = 3)
+    ru = ref.copy()
+    ru[:, :, 2], ru[:, :, 1], ru[:, :, 0] = ru[:, :, 3], ru[:, :, 4], ru[:, :, 5]
+    ru[:, :, g.nx - 2], ru[:, :, g.nx - 1] = ru[:, :, g.nx - 3], ru[:, :, g.nx - 4]
+    assert np.array_equal(arrs[0], ru, equal_nan=True)
+    for a in arrs[1:]:
         assert np.array_equal(a, ref, equal_nan=True)
+
+
+def test_fill_winds_no_flux_sides_and_periodic_levels():
+    """'nnf' sides negate the normal component into the ghosts (u in x, v
+    in y: u(3) = -u(4), u(2) = -u(5), u(1) = -u(6), u(nx-1) = -u(nx-2),
+    u(nx) = -u(nx-3)), so the face average of u over the boundary face
+    pair vanishes; a periodic vertical takes levels 1:3 = nz-4:nz-2 and
+    nz-1:nz = 4:5 before the side fills."""
+    g = gm.uniform(15, 14, 11, bcl=("nnf", "nnf", "nnf", "nnf", "per", "per"))
+    rng = np.random.default_rng(13)
+    u, v, w = (rng.standard_normal(g.shape) for _ in range(3))
+    u0, v0, w0 = u.copy(), v.copy(), w.copy()
+    Oracle(g).fill_winds(u, v, w)
+    nx, ny, nz = g.nx, g.ny, g.nz
+    # vertical images first, for every component
+    for a, a0 in ((u, u0), (v, v0), (w, w0)):
+        b = a0.copy()
+        b[:3] = a0[nz - 5:nz - 2]
+        b[nz - 2:] = a0[3:5]
+        if a is w:
+            assert np.array_equal(a, b)
+        a0[...] = b
+    for i_g, i_s in ((2, 3), (1, 4), (0, 5), (nx - 2, nx - 3), (nx - 1, nx - 4)):
+        assert np.array_equal(u[:, :, i_g], -u0[:, :, i_s])
+    for j_g, j_s in ((2, 3), (1, 4), (0, 5), (ny - 2, ny - 3), (ny - 1, ny - 4)):
+        assert np.array_equal(v[:, j_g, :], -v0[:, j_s, :])
+    inner = (slice(None), slice(None), slice(3, nx - 2))
+    assert np.array_equal(u[inner], u0[inner])
+    assert np.array_equal(v[:, 3:ny - 2, :], v0[:, 3:ny - 2, :])
 
 
 def test_totalav_is_interior_mean():
