@@ -25,6 +25,8 @@ struct mimadv_ctx {
     std::string err;
     double *d_metrics = nullptr;  // fdz0, dzw, dz1w, dz1, ppmc
     double *d_red = nullptr;      // reduction scratch [2 + nz]
+    double *d_budget = nullptr;   // budget results and per-block partials
+    size_t budget_elems = 0;
     // staging buffers for MIMADV_HOST_PTRS calls
     double *stage = nullptr;
     size_t stage_elems = 0;
@@ -311,6 +313,7 @@ void mimadv_destroy(mimadv_ctx *c)
     if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     if (c->stream && c->stream != c->own_stream) (void)hipStreamSynchronize(c->stream);
     if (c->d_metrics) (void)hipFree(c->d_metrics);
+    if (c->d_budget) (void)hipFree(c->d_budget);
     if (c->d_red) (void)hipFree(c->d_red);
     if (c->stage) (void)hipFree(c->stage);
     if (c->work) (void)hipFree(c->work);
@@ -687,8 +690,8 @@ int check_batch_flags(mimadv_ctx *c, unsigned flags)
 
 int check_batch(mimadv_ctx *c, const float *u, const float *v, const float *w)
 {
-    if (c->sp.scheme != MIMADV_WENO && c->sp.scheme != MIMADV_MUSCL)
-        return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path implements the WENO and MUSCL schemes");
+    if (c->sp.scheme != MIMADV_WENO && c->sp.scheme != MIMADV_MUSCL && c->sp.scheme != MIMADV_WENO5)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path implements the WENO, WENO5 and MUSCL schemes");
     for (int b = 0; b < 4; ++b)
         if (c->g.bcl[b] != MIMADV_BC_PER && !(c->g.ny == 1 && b >= 2))
             return fail(c, MIMADV_ERR_UNSUPPORTED, "the batched path needs periodic lateral boundaries");
@@ -1005,10 +1008,23 @@ int mimadv_batch_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, c
         if (!c->graph_exec || key != c->graph_key) {
             drop_graph(c);
             hipGraph_t graph = nullptr;
-            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            // capture on the context's own stream (the caller's stream may
+            // be the null stream, which cannot capture); the graph is then
+            // launched on the caller's stream
+            hipStream_t user = c->stream;
+            c->stream = c->own_stream;
+            hipError_t be = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+            if (be != hipSuccess) {
+                c->stream = user;
+                return hipfail(c, be, "hipStreamBeginCapture");
+            }
             hipError_t le = one_step();
             hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-            if (le != hipSuccess) return hipfail(c, le, "launch during capture");
+            c->stream = user;
+            if (le != hipSuccess) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return hipfail(c, le, "launch during capture");
+            }
             if (ce != hipSuccess) return hipfail(c, ce, "hipStreamEndCapture");
             hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
@@ -1030,10 +1046,37 @@ int mimadv_batch_budget(mimadv_ctx *c, int nf, const int *precision, const void 
 {
     if (!c) return MIMADV_ERR_ARG;
     if (nf < 0 || (nf > 0 && (!precision || !q || !out))) return fail(c, MIMADV_ERR_ARG, "bad field list");
+    if (nf == 0) return MIMADV_OK;
     for (int f = 0; f < nf; ++f) {
         if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
-        HIPCHK(c, launch_wsum(c->g, q[f], precision[f] == MIMADV_F32, c->d_red + 2, c->d_red, c->stream));
-        HIPCHK(c, hipMemcpyAsync(out + f, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (precision[f] != MIMADV_F32 && precision[f] != MIMADV_F64)
+            return fail(c, MIMADV_ERR_ARG, "precision must be MIMADV_F32 or MIMADV_F64");
+    }
+    BudgetArgs a{};
+    a.nx = c->g.nx;
+    a.ny = c->g.ny;
+    a.nz = c->g.nz;
+    a.nb = budget_blocks(c->g);
+    a.dzw = c->g.zm.dzw;
+    const size_t need = (size_t)std::min(nf, BUDGET_MAXF) * (a.nb + 1);
+    if (need > c->budget_elems) {
+        if (c->d_budget) (void)hipFree(c->d_budget);
+        c->d_budget = nullptr;
+        c->budget_elems = 0;
+        if (hipMalloc(&c->d_budget, need * sizeof(double)) != hipSuccess)
+            return fail(c, MIMADV_ERR_NOMEM, "budget scratch allocation failed");
+        c->budget_elems = need;
+    }
+    // one pass over up to BUDGET_MAXF fields, one copy back, one wait
+    for (int f0 = 0; f0 < nf; f0 += BUDGET_MAXF) {
+        const int n = std::min(BUDGET_MAXF, nf - f0);
+        for (int f = 0; f < n; ++f) {
+            a.q[f] = q[f0 + f];
+            a.fp32[f] = precision[f0 + f] == MIMADV_F32;
+        }
+        a.part = c->d_budget + n;
+        HIPCHK(c, launch_budgets(a, n, c->d_budget, c->stream));
+        HIPCHK(c, hipMemcpyAsync(out + f0, c->d_budget, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return MIMADV_OK;

@@ -21,8 +21,9 @@
 //
 // Schemes: the reference WENO ('weno' of advq_weno.f90, including its
 // limiter, the k = 3..nz-3 range of nonlinear vertical weights and the
-// bottom/top closures) and MUSCL (advq_muscl.f90) in an operation-reduced
-// form: the positivity rescale is applied to the weighted face value
+// bottom/top closures), MUSCL (advq_muscl.f90) and the WENO5-JS extension
+// (3-cell halo, in z where the upwind stencil fits the column) in an
+// operation-reduced form: the positivity rescale is applied to the weighted face value
 // (b + phi*(F - b), equal to rescaling each candidate since the weights
 // sum to one), only the upwind side's weights are evaluated, and divisions
 // are reciprocals.  It is checked against the FP64 oracle with tolerances,
@@ -39,11 +40,11 @@ using namespace faces;
 
 // Geometry of one work item (a TX x TY column tile and a chunk of at most
 // KC levels) shared by every field it advects.
-template <int TX, int TY>
+template <int TX, int TY, int H>
 struct Item {
     static constexpr bool THREE_D = TY > 1;
     static constexpr int NT = TX * TY;
-    static constexpr int LW = TX + 4, LH = THREE_D ? TY + 4 : 1;  // tile + 2-cell halo
+    static constexpr int LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1;  // tile + H-cell halo
     static constexpr int NXF = TY * (TX + 1), NYF = THREE_D ? (TY + 1) * TX : 0, NF = NXF + NYF;
     static constexpr int FS = (NF + NT - 1) / NT;        // face slots per thread
     static constexpr int TS = (LH * LW + NT - 1) / NT;   // tile slots per thread
@@ -57,22 +58,22 @@ constexpr int FUSED_KC = 8;
 // Face e of the flattened list [x faces row-major | y faces] of a tile:
 // kind (1 x, 2 y), LDS offset/step of its stencil and the column of its
 // velocity (u(f+1) of x face f + 1/2, v(g+1) of y face g + 1/2).
-template <int TX, int TY>
+template <int TX, int TY, int H>
 __device__ __forceinline__ int face_slot_geom(int e, int i0, int j0, const FusedArgs &A, int &base, int &step,
                                               size_t &vcol)
 {
-    using I = Item<TX, TY>;
+    using I = Item<TX, TY, H>;
     const int nx = A.nx, ny = A.ny;
     if (e < I::NXF) {
         const int r = e / (TX + 1), fl = e - r * (TX + 1);
-        base = (I::THREE_D ? r + 2 : 0) * I::LW + fl;
+        base = (I::THREE_D ? r + H : 0) * I::LW + fl;
         step = 1;
         const int jr = I::THREE_D ? wrap(j0 + r, ny, A.my) : 1;
         vcol = (size_t)(wrap(i0 + fl, nx, A.mx) - 1) + (size_t)nx * (jr - 1);
         return 1;
     }
     const int e2 = e - I::NXF, fr = e2 / TX, c = e2 - fr * TX;
-    base = fr * I::LW + c + 2;
+    base = fr * I::LW + c + H;
     step = I::LW;
     vcol = (size_t)(wrap(i0 + c, nx, A.mx) - 1) + (size_t)nx * (wrap(j0 + fr, ny, A.my) - 1);
     return 2;
@@ -84,9 +85,9 @@ __device__ __forceinline__ int face_slot_geom(int e, int i0, int j0, const Fused
 // [x faces row-major | y faces] (LDS offset of the stencil q(f-1..f+2),
 // step 1 for x and LW for y, 0 for an empty slot).  Plane offsets fit in
 // 32 bits.
-template <int TX, int TY>
+template <int TX, int TY, int H>
 struct Slots {
-    using I = Item<TX, TY>;
+    using I = Item<TX, TY, H>;
     int t_off[I::TS];
     int f_base[I::FS], f_step[I::FS];
     __device__ __forceinline__ void init(int tid, int i0, int j0, const FusedArgs &A)
@@ -98,8 +99,8 @@ struct Slots {
             t_off[s] = -1;
             if (e < I::LH * I::LW) {
                 const int r = e / I::LW, c = e - r * I::LW;
-                const int ii = wrap(i0 - 2 + c, nx, A.mx);
-                const int jj = I::THREE_D ? wrap(j0 - 2 + r, ny, A.my) : 1;
+                const int ii = wrap(i0 - H + c, nx, A.mx);
+                const int jj = I::THREE_D ? wrap(j0 - H + r, ny, A.my) : 1;
                 t_off[s] = (ii - 1) + nx * (jj - 1);
             }
         }
@@ -110,11 +111,11 @@ struct Slots {
             f_step[s] = 0;
             if (e < I::NXF) {
                 const int r = e / (TX + 1), fl = e - r * (TX + 1);  // x face f = i0-1+fl, row j0+r
-                f_base[s] = (I::THREE_D ? r + 2 : 0) * I::LW + fl;
+                f_base[s] = (I::THREE_D ? r + H : 0) * I::LW + fl;
                 f_step[s] = 1;
             } else if (e < I::NF) {
                 const int e2 = e - I::NXF, fr = e2 / TX, c = e2 - fr * TX;  // y face g = j0-1+fr, column i0+c
-                f_base[s] = fr * I::LW + c + 2;
+                f_base[s] = fr * I::LW + c + H;
                 f_step[s] = I::LW;
             }
         }
@@ -132,9 +133,10 @@ template <int SCHEME, int TX, int TY, class T>
 __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restrict__ q, const T *__restrict__ q0,
                                             T *__restrict__ out, T *lt, T *lf, const float *lv, int k0, int nk,
                                             int i0, int j0, int tid, int tx, int ty, bool valid, int col,
-                                            const Slots<TX, TY> &sl)
+                                            const Slots<TX, TY, stencil_h<SCHEME>()> &sl)
 {
-    using I = Item<TX, TY>;
+    constexpr int H = stencil_h<SCHEME>();
+    using I = Item<TX, TY, H>;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const size_t plane = (size_t)nx * ny;
     auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
@@ -144,9 +146,10 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
     const bool zdir = (A.dirmask & 4) != 0;
     const int kstart = (k0 > 1 && zdir) ? -1 : 0;
     const int ks = k0 + kstart;
-    // window q(ks-1..ks+2) with q(0) := q(1)
-    T zw[4] = {q[col + lev(ks - 1)], q[col + lev(ks)], q[col + lev(ks + 1)], q[col + lev(ks + 2)]};
-    if (ks == 1) zw[0] = zw[1];
+    // window q(ks-H+1..ks+H), levels below 1 read as q(1)
+    T zw[2 * H];
+#pragma unroll
+    for (int m = 0; m < 2 * H; ++m) zw[m] = q[col + lev(ks - H + 1 + m)];
     const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
     T fzb = T(0);
     T tv[I::TS];
@@ -161,19 +164,18 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
         for (int s = 0; s < I::TS; ++s)
             if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
         if (need_q0 && valid) q0v = q0[col + kb];
-        zn = q[col + lev(k + 2)];
+        zn = q[col + lev(k + H)];
         d1v = A.dz1w[k - 1];
     };
     fetch(k0);
 #pragma unroll 1
     for (int kk = kstart; kk < nk; ++kk) {
         const int k = k0 + kk;
-        // window q(k-1..k+2): the level's top value arrived one level ahead
+        // window q(k-H+1..k+H): the level's top value arrived one level ahead
         if (kk > kstart) {
-            zw[0] = zw[1];
-            zw[1] = zw[2];
-            zw[2] = zw[3];
-            zw[3] = zn;
+#pragma unroll
+            for (int m = 0; m < 2 * H - 1; ++m) zw[m] = zw[m + 1];
+            zw[2 * H - 1] = zn;
         }
         const float *lvk = lv + (kk < 0 ? 0 : kk) * I::NV;  // this level's staged velocities
         const T fzt = zdir ? zface<SCHEME, T>(k, nz, zw, T(kk < 0 ? wz0 : lvk[I::NF + tid]), A) : T(0);
@@ -193,8 +195,10 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
             const int st = sl.f_step[s];
             if (st == 0) continue;
             const int b = sl.f_base[s];
-            lf[tid + s * I::NT] = hface<SCHEME, T>(lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st],
-                                                   T(lvk[tid + s * I::NT]), A, st == 1 ? A.dx1 : A.dy1);
+            T sv[2 * H];
+#pragma unroll
+            for (int m = 0; m < 2 * H; ++m) sv[m] = lt[b + m * st];
+            lf[tid + s * I::NT] = hface<SCHEME, T>(sv, T(lvk[tid + s * I::NT]), A, st == 1 ? A.dx1 : A.dy1);
         }
         __syncthreads();
         if (valid) {
@@ -214,8 +218,8 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
                 out[id] = T(a);
             } else {
                 double qn;
-                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[1]) + A.dt * a);
-                else qn = A.c1 * (double(zw[1]) + A.dt * a);
+                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[H - 1]) + A.dt * a);
+                else qn = A.c1 * (double(zw[H - 1]) + A.dt * a);
                 const T qv = T(qn);
                 out[id] = qv;
                 if (A.fill) {
@@ -246,18 +250,20 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 // Waves per SIMD the register allocation must allow.  The FP32 64x4 tile
 // takes 32 KB of LDS, so five groups fit a CU; at 97 VGPRs it would be held
 // to four waves, and 96 reach five without spills.  The other instances are
-// bound by LDS or need their registers.
-template <int TY, class T>
+// bound by LDS or need their registers (WENO5's six-value stencils among
+// them: held to 96 VGPRs it spills).
+template <int SCHEME, int TY, class T>
 constexpr int fused_min_waves()
 {
-    return (TY == 4 && sizeof(T) == 4) ? 5 : 1;
+    return (SCHEME != 3 && TY == 4 && sizeof(T) == 4) ? 5 : 1;
 }
 
 template <int SCHEME, int TX, int TY, class T>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(fused_min_waves<TY, T>(), 8))) void
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(fused_min_waves<SCHEME, TY, T>(), 8))) void
 fused_kernel(FusedArgs A)
 {
-    using I = Item<TX, TY>;
+    constexpr int H = stencil_h<SCHEME>();
+    using I = Item<TX, TY, H>;
     __shared__ T tile_lds[I::LH * I::LW];
     __shared__ T face_lds[I::FS * I::NT];
     __shared__ float vel_lds[FUSED_KC * I::NV];
@@ -285,7 +291,7 @@ fused_kernel(FusedArgs A)
         const bool valid = i <= nx - 2 && (!I::THREE_D || j <= ny - 2);
         const int il = wrap(i, nx, A.mx), jl = I::THREE_D ? wrap(j, ny, A.my) : 1;
         const int col = (il - 1) + nx * (jl - 1);
-        Slots<TX, TY> sl;
+        Slots<TX, TY, H> sl;
         sl.init(tid, i0, j0, A);
         // the chunk's face velocities and column w(k+1), read once for every field
         {
@@ -299,7 +305,7 @@ fused_kernel(FusedArgs A)
                     if (e < I::NF) {
                         int base, step;
                         size_t vcol;
-                        const int kind = face_slot_geom<TX, TY>(e, i0, j0, A, base, step, vcol);
+                        const int kind = face_slot_geom<TX, TY, H>(e, i0, j0, A, base, step, vcol);
                         vel = kind == 1 ? A.u[kb + vcol] : A.v[kb + vcol];
                     } else if (k < nz) {
                         const int c = e - I::NF, cx = c % TX, cy = c / TX;
@@ -384,6 +390,7 @@ hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStr
     if (n64 < 0 || n64 > FUSED_MAX64 || n32 < 0 || n32 > FUSED_MAX32) return hipErrorInvalidValue;
     if (scheme == 0) return launch_scheme<0>(A, s, blocks_per_cu);
     if (scheme == 2) return launch_scheme<2>(A, s, blocks_per_cu);
+    if (scheme == 3) return launch_scheme<3>(A, s, blocks_per_cu);
     return hipErrorInvalidValue;
 }
 

@@ -71,6 +71,34 @@ __device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
     return uu * (b + phi * Fb);
 }
 
+// WENO5-JS (Jiang & Shu 1996) value at the right face of c from a..e =
+// q(i-2..i+2): candidates (2,-7,11)/6, (-1,5,2)/6, (2,5,-1)/6, linear weights
+// 0.1, 0.6, 0.3, smoothness indicators beta_r, eps = 1e-6, p = 2; the
+// divisions are reciprocals.
+template <class T>
+__device__ __forceinline__ T weno5_left(T a, T b, T c, T d, T e)
+{
+    const T eps = T(1e-6), k13 = T(13.0 / 12.0), sixth = T(1.0 / 6.0);
+    const T q0 = (T(2) * a - T(7) * b + T(11) * c) * sixth;
+    const T q1 = (-b + T(5) * c + T(2) * d) * sixth;
+    const T q2 = (T(2) * c + T(5) * d - e) * sixth;
+    const T s0 = a - T(2) * b + c, t0 = a - T(4) * b + T(3) * c;
+    const T s1 = b - T(2) * c + d, t1 = b - d;
+    const T s2 = c - T(2) * d + e, t2 = T(3) * c - T(4) * d + e;
+    T b0 = eps + (k13 * s0 * s0 + T(0.25) * t0 * t0);
+    T b1 = eps + (k13 * s1 * s1 + T(0.25) * t1 * t1);
+    T b2 = eps + (k13 * s2 * s2 + T(0.25) * t2 * t2);
+    const T a0 = T(0.1) * rcp(b0 * b0), a1 = T(0.6) * rcp(b1 * b1), a2 = T(0.3) * rcp(b2 * b2);
+    return (a0 * q0 + a1 * q1 + a2 * q2) * rcp(a0 + a1 + a2);
+}
+
+// WENO5 face flux from q(f-2..f+3), upwind side only
+template <class T>
+__device__ __forceinline__ T weno5_face(T a, T b, T c, T d, T e, T f, T uu)
+{
+    return uu > T(0) ? uu * weno5_left(a, b, c, d, e) : uu * weno5_left(f, e, d, c, b);
+}
+
 // top closure k = nz-1 of advsz_weno (no limiter, linear weights)
 template <class T>
 __device__ __forceinline__ T weno_top_face(T a, T b, T c, T uu)
@@ -160,19 +188,36 @@ __device__ __forceinline__ T muscl_zface(int f, int nz, T a, T b, T c, T d, T ww
     return ww * F;
 }
 
-template <int SCHEME, class T>
-__device__ __forceinline__ T hface(T a, T b, T c, T d, T uu, const FusedArgs &A, double d1)
+// Stencil half-width of a scheme: the x/y halo of the LDS tile and the
+// offset of q(f) in a face's stencil (WENO5 reads q(f-2..f+3), the others
+// q(f-1..f+2)).
+template <int SCHEME>
+constexpr int stencil_h()
 {
-    if constexpr (SCHEME == 0) return weno_face(a, b, c, d, uu, true);
-    else return muscl_face(a, b, c, d, uu, A.muscl_limiter, T(A.lim_tol), T(A.lim_tol_inv), T(d1));
+    return SCHEME == 3 ? 3 : 2;
+}
+
+// x/y face from the stencil s[0..2H-1] = q(f-H+1..f+H)
+template <int SCHEME, class T>
+__device__ __forceinline__ T hface(const T *s, T uu, const FusedArgs &A, double d1)
+{
+    if constexpr (SCHEME == 0) return weno_face(s[0], s[1], s[2], s[3], uu, true);
+    else if constexpr (SCHEME == 3) return weno5_face(s[0], s[1], s[2], s[3], s[4], s[5], uu);
+    else return muscl_face(s[0], s[1], s[2], s[3], uu, A.muscl_limiter, T(A.lim_tol), T(A.lim_tol_inv), T(d1));
 }
 
 template <int SCHEME, class T>
 __device__ __forceinline__ T zface(int f, int nz, const T *zw, T ww, const FusedArgs &A)
 {
-    // zw = q(f-1), q(f), q(f+1), q(f+2) (q(0) := q(1))
+    // zw = q(f-H+1..f+H) (H = stencil_h), levels below 1 read as q(1)
     if (f >= nz) return T(0);
-    if constexpr (SCHEME == 0) {
+    if constexpr (SCHEME == 3) {
+        // WENO5 where the 5-cell upwind stencil lies in the column, the
+        // reference face elsewhere
+        if (ww > T(0) && f >= 3 && f <= nz - 2) return ww * weno5_left(zw[0], zw[1], zw[2], zw[3], zw[4]);
+        if (ww < T(0) && f >= 2 && f <= nz - 3) return ww * weno5_left(zw[5], zw[4], zw[3], zw[2], zw[1]);
+        return zface<0, T>(f, nz, zw + 1, ww, A);
+    } else if constexpr (SCHEME == 0) {
         if (f == nz - 1) return weno_top_face(zw[0], zw[1], zw[2], ww);
         if (f == 1 && A.weno_bottom == 1) {
             T ffip1 = T(0.5) * (zw[1] + zw[2]);

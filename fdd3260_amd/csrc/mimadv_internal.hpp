@@ -109,6 +109,21 @@ hipError_t launch_level_scale(const GridDev &g, const double *prof, const double
 hipError_t launch_field_scale(const GridDev &g, double *q, const double *d, int divide, hipStream_t s);
 hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_part, double *d_out, hipStream_t s);
 
+// Domain budgets (dzw-weighted interior sums) of up to BUDGET_MAXF fields in
+// two launches: per-block partials reduced with wavefront shuffles, then one
+// wave per field (deterministic order).  part holds nf*nb doubles.
+constexpr int BUDGET_MAXF = 32;
+struct BudgetArgs {
+    int nx, ny, nz, nb;
+    const void *q[BUDGET_MAXF];
+    int fp32[BUDGET_MAXF];
+    const double *dzw;
+    double *part;
+};
+constexpr int BUDGET_MAXNB = 1024;  // first-pass blocks per field
+int budget_blocks(const GridDev &g);
+hipError_t launch_budgets(const BudgetArgs &a, int nf, double *d_out, hipStream_t s);
+
 // Host-side PPM per-level coefficient table (12 doubles per level).
 std::vector<double> ppm_coefficients(int nz, const double *dz1w);
 

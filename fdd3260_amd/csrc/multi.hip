@@ -7,7 +7,8 @@
 // workgroup finds its subdomain by a binary search of the first work id of
 // every subdomain, reads that subdomain's argument block from a device
 // table, and runs the same per-field column march as the single-grid
-// batched kernel (fused.hip): tile plus 2-cell halo in LDS, flattened x/y
+// batched kernel (fused.hip): tile plus a 2-cell (WENO5: 3-cell) halo in
+// LDS, flattened x/y
 // face list, z face carried up the column, FP64 divergence and stage
 // update rounded once to FP32.
 //
@@ -28,18 +29,19 @@ using namespace faces;
 constexpr int NT = MULTI_NT;
 constexpr int FS = 3;                  // 2*NT + TX + TY faces per tile
 constexpr int TS = 3;                  // (TX+4)*(TY+4) tile values
-constexpr int TILE_MAX = 68 * 8;       // largest (TX+4)*(TY+4) of the shapes
+constexpr int TILE_MAX = 70 * 10;      // largest (TX+2H)*(TY+2H) of the shapes (64x4, H = 3)
 constexpr int KC_MAX = 8;              // levels of a work item (velocities staged in LDS)
 constexpr int NV_MAX = 2 * NT + 68 + NT;  // face velocities and column w of one level
 
 struct Geom {
-    int TX, TY, LW, LH, NXF, NF, NV;
-    __device__ __forceinline__ void set(int tx_, int ty_)
+    int TX, TY, H, LW, LH, NXF, NF, NV;
+    __device__ __forceinline__ void set(int tx_, int ty_, int h_)
     {
         TX = tx_;
         TY = ty_;
-        LW = TX + 4;
-        LH = TY + 4;
+        H = h_;
+        LW = TX + 2 * H;
+        LH = TY + 2 * H;
         NXF = TY * (TX + 1);
         NF = NXF + (TY + 1) * TX;
         NV = NF + NT;
@@ -58,7 +60,7 @@ struct SlotsR {
             t_off[s] = -1;
             if (e < G.LH * G.LW) {
                 const int r = e / G.LW, c = e - r * G.LW;
-                t_off[s] = (wrap(i0 - 2 + c, nx, A.mx) - 1) + nx * (wrap(j0 - 2 + r, ny, A.my) - 1);
+                t_off[s] = (wrap(i0 - G.H + c, nx, A.mx) - 1) + nx * (wrap(j0 - G.H + r, ny, A.my) - 1);
             }
         }
 #pragma unroll
@@ -68,27 +70,20 @@ struct SlotsR {
             f_step[s] = 0;
             if (e < G.NXF) {
                 const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);  // x face f = i0-1+fl, row j0+r
-                f_base[s] = (r + 2) * G.LW + fl;
+                f_base[s] = (r + G.H) * G.LW + fl;
                 f_step[s] = 1;
             } else if (e < G.NF) {
                 const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;  // y face g = j0-1+fr
-                f_base[s] = fr * G.LW + c + 2;
+                f_base[s] = fr * G.LW + c + G.H;
                 f_step[s] = G.LW;
             }
         }
     }
 };
 
-__device__ __forceinline__ float face_any(const FusedArgs &A, float a, float b, float c, float d, float uu, double d1)
-{
-    return A.scheme == 0 ? hface<0, float>(a, b, c, d, uu, A, d1) : hface<2, float>(a, b, c, d, uu, A, d1);
-}
-
-__device__ __forceinline__ float zface_any(const FusedArgs &A, int f, int nz, const float *zw, float ww)
-{
-    return A.scheme == 0 ? zface<0, float>(f, nz, zw, ww, A) : zface<2, float>(f, nz, zw, ww, A);
-}
-
+// One field of one work item; SCHEME is the subdomain's scheme (uniform
+// over the work item).
+template <int SCHEME>
 __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, const float *__restrict__ q,
                                             const float *__restrict__ q0, float *__restrict__ out, float *lt,
                                             float *lf, const float *lv, int k0, int nk, int i0, int j0, int tid,
@@ -99,10 +94,13 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
     auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
     // a chunk above the floor first evaluates its bottom face in a z-only
     // pass of the level loop: every z face comes from the same code
+    constexpr int H = stencil_h<SCHEME>();
     const int kstart = k0 > 1 ? -1 : 0;
     const int ks = k0 + kstart;
-    float zw[4] = {q[col + lev(ks - 1)], q[col + lev(ks)], q[col + lev(ks + 1)], q[col + lev(ks + 2)]};
-    if (ks == 1) zw[0] = zw[1];
+    // window q(ks-H+1..ks+H), levels below 1 read as q(1)
+    float zw[2 * H];
+#pragma unroll
+    for (int m = 0; m < 2 * H; ++m) zw[m] = q[col + lev(ks - H + 1 + m)];
     const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
     float fzb = 0.0f;
     float tv[TS], q0v = 0.0f, zn = 0.0f;
@@ -115,7 +113,7 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
         for (int s = 0; s < TS; ++s)
             if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
         if (need_q0 && valid) q0v = q0[col + kb];
-        zn = q[col + lev(k + 2)];
+        zn = q[col + lev(k + H)];
         d1v = A.dz1w[k - 1];
     };
     const int ex = ty * (G.TX + 1) + tx, ey = G.NXF + ty * G.TX + tx;
@@ -124,13 +122,12 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
     for (int kk = kstart; kk < nk; ++kk) {
         const int k = k0 + kk;
         if (kk > kstart) {
-            zw[0] = zw[1];
-            zw[1] = zw[2];
-            zw[2] = zw[3];
-            zw[3] = zn;
+#pragma unroll
+            for (int m = 0; m < 2 * H - 1; ++m) zw[m] = zw[m + 1];
+            zw[2 * H - 1] = zn;
         }
         const float *lvk = lv + (kk < 0 ? 0 : kk) * G.NV;  // this level's staged velocities
-        const float fzt = zface_any(A, k, nz, zw, kk < 0 ? wz0 : lvk[G.NF + tid]);
+        const float fzt = zface<SCHEME, float>(k, nz, zw, kk < 0 ? wz0 : lvk[G.NF + tid], A);
         if (kk < 0) {
             fzb = fzt;
             continue;
@@ -147,8 +144,10 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
             const int st = sl.f_step[s];
             if (st == 0) continue;
             const int b = sl.f_base[s];
-            lf[tid + s * NT] = face_any(A, lt[b], lt[b + st], lt[b + 2 * st], lt[b + 3 * st], lvk[tid + s * NT],
-                                        st == 1 ? A.dx1 : A.dy1);
+            float sv[2 * H];
+#pragma unroll
+            for (int m = 0; m < 2 * H; ++m) sv[m] = lt[b + m * st];
+            lf[tid + s * NT] = hface<SCHEME, float>(sv, lvk[tid + s * NT], A, st == 1 ? A.dx1 : A.dy1);
         }
         __syncthreads();
         if (valid) {
@@ -161,8 +160,8 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
                 out[id] = float(a);
             } else {
                 double qn;
-                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[1]) + A.dt * a);
-                else qn = A.c1 * (double(zw[1]) + A.dt * a);
+                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[H - 1]) + A.dt * a);
+                else qn = A.c1 * (double(zw[H - 1]) + A.dt * a);
                 const float qv = float(qn);
                 out[id] = qv;
                 if (A.fill) {
@@ -204,7 +203,7 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
         // memory); the field pointers are read from the table once per field
         const FusedArgs A = table[lo];
         Geom G;
-        G.set(A.tile_x, A.tile_y);
+        G.set(A.tile_x, A.tile_y, A.scheme == 3 ? 3 : 2);
         const int local = wi - A.work_begin;
         const int tiles = A.ntx * A.nty;
         const int chunk = local / tiles, t = local - chunk * tiles;
@@ -245,9 +244,16 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
             __syncthreads();
         }
 #pragma unroll 1
-        for (int f = 0; f < A.n32_active; ++f)
-            march_multi(A, G, table[lo].q32[f], table[lo].q0_32[f], table[lo].out32[f], tile_lds, face_lds, vel_lds, k0,
-                        nk, i0, j0, tid, tx, ty, valid, col, sl);
+        for (int f = 0; f < A.n32_active; ++f) {
+            const float *qf = table[lo].q32[f], *q0f = table[lo].q0_32[f];
+            float *of = table[lo].out32[f];
+            if (A.scheme == 0)
+                march_multi<0>(A, G, qf, q0f, of, tile_lds, face_lds, vel_lds, k0, nk, i0, j0, tid, tx, ty, valid, col, sl);
+            else if (A.scheme == 3)
+                march_multi<3>(A, G, qf, q0f, of, tile_lds, face_lds, vel_lds, k0, nk, i0, j0, tid, tx, ty, valid, col, sl);
+            else
+                march_multi<2>(A, G, qf, q0f, of, tile_lds, face_lds, vel_lds, k0, nk, i0, j0, tid, tx, ty, valid, col, sl);
+        }
     }
 }
 

@@ -223,6 +223,74 @@ __global__ void sum_levels_kernel(int nz, const double *__restrict__ part, doubl
 
 }  // namespace
 
+namespace {
+
+// Sum over the 64 lanes of a wavefront with cross-lane shuffles (no LDS,
+// no barrier); the order of the butterfly is fixed, so the result is
+// deterministic.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Domain budgets of nf fields, first pass: block (b, f) sums dzw(k) x over
+// the interior rows r = b, b + nb, ... of field f (a row is the mx interior
+// values of one (j, k), read coalesced), reduces its four waves with
+// shuffles and one LDS exchange, and writes part[f*nb + b].
+__global__ __launch_bounds__(256) void budget_part_kernel(BudgetArgs a)
+{
+    const int f = blockIdx.y, b = blockIdx.x;
+    const bool three_d = a.ny > 1;
+    const int j0 = three_d ? 4 : 1, my = three_d ? a.ny - 5 : 1, mx = a.nx - 5;
+    const long rows = (long)my * a.nz;
+    const bool f32 = a.fp32[f] != 0;
+    const float *x32 = (const float *)a.q[f];
+    const double *x64 = (const double *)a.q[f];
+    double s = 0.0;
+    for (long r = b; r < rows; r += a.nb) {
+        const int k = (int)(r / my), jj = (int)(r - (long)k * my);
+        const size_t base = (size_t)a.nx * ((size_t)(jj + j0 - 1) + (size_t)a.ny * k) + 3;
+        double rs = 0.0;
+        for (int i = threadIdx.x; i < mx; i += blockDim.x) rs += f32 ? (double)x32[base + i] : x64[base + i];
+        s += a.dzw[k] * rs;
+    }
+    s = wave_sum(s);
+    __shared__ double w[4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) w[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) a.part[(size_t)f * a.nb + b] = ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+// Second pass: one wave per field sums its nb partials (strided lanes,
+// then the shuffle tree).
+__global__ __launch_bounds__(64) void budget_final_kernel(int nb, const double *__restrict__ part,
+                                                          double *__restrict__ out)
+{
+    const int f = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 64) s += part[(size_t)f * nb + b];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[f] = s;
+}
+
+}  // namespace
+
+int budget_blocks(const GridDev &g)
+{
+    const long rows = (long)(g.ny > 1 ? g.ny - 5 : 1) * g.nz;
+    return (int)std::min<long>(rows, BUDGET_MAXNB);
+}
+
+hipError_t launch_budgets(const BudgetArgs &a, int nf, double *d_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(budget_part_kernel, dim3(a.nb, nf), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(budget_final_kernel, dim3(nf), dim3(64), 0, s, a.nb, a.part, d_out);
+    return hipGetLastError();
+}
+
 hipError_t launch_wsum(const GridDev &g, const void *x, bool fp32, double *d_part, double *d_out, hipStream_t s)
 {
     if (fp32)

@@ -1,5 +1,5 @@
 // weno_march.hip -- plain per-field FP64 kernels, column-marching form: the
-// reference 'weno' scheme, the WENO5-JS extension in x/y, and the line
+// reference 'weno' scheme, the WENO5-JS extension, and the line
 // schemes MUSCL, QUICK and Lax-Wendroff with their unlimited forms (face
 // formulas of schemes_fp64.hpp).
 //
@@ -178,8 +178,9 @@ __device__ __forceinline__ double line_zface(int f, int nz, const ZMetrics &zm, 
     else return lw_zface<true>(f, nz, zm, sp, bc51, bc61, Q, W);
 }
 
-// S (scheme code): 0 the reference 'weno', 3 WENO5-JS in x/y with the
-// reference 'weno' in z, 2 MUSCL, 4 QUICK, 5 Lax-Wendroff, 6 and 7 their
+// S (scheme code): 0 the reference 'weno', 3 WENO5-JS (in z where the
+// upwind stencil fits the column, the reference 'weno' face elsewhere),
+// 2 MUSCL, 4 QUICK, 5 Lax-Wendroff, 6 and 7 their
 // unlimited forms
 template <int TX, int TY, int S>
 __device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a,
@@ -274,6 +275,8 @@ __device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams 
         // z-only pass, so that every z face comes from the same code.
         const int kstart = (k0 > 1 && zdir) ? k0 - 1 : k0;
         double z0 = Qz(kstart - 1), z1 = Qz(kstart), z2 = Qz(kstart + 1), z3 = Qz(kstart + 2);
+        // WENO5 widens the window to q(k-2..k+3)
+        double zm1 = S == 3 ? Qz(kstart - 2) : 0.0, z4 = S == 3 ? Qz(kstart + 3) : 0.0;
         double wz = kstart < nz ? a.w[col + (size_t)kstart * plane] : 0.0;
         double fzb = 0.0;
         // operands of level k, requested one level ahead
@@ -291,7 +294,7 @@ __device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams 
                 vv[s] = kind == 1 ? uk[f_off(s)] : (kind == 2 ? vk[f_off(s)] : 0.0);
             }
             wv = k < nz ? a.w[col + kb + plane] : 0.0;
-            zn = Qz(k + 2);
+            zn = Qz(S == 3 ? k + 3 : k + 2);
             if (valid) {
                 if (need_q0) q0v = a.q0[col + kb];
                 if (need_tend) tdv = a.tend[col + kb];
@@ -302,15 +305,33 @@ __device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams 
 #pragma unroll 1
         for (int k = kstart; k <= k1; ++k) {
             if (k > kstart) {
-                z0 = z1;
-                z1 = z2;
-                z2 = z3;
-                z3 = zn;
+                if constexpr (S == 3) {
+                    zm1 = z0;
+                    z0 = z1;
+                    z1 = z2;
+                    z2 = z3;
+                    z3 = z4;
+                    z4 = zn;
+                } else {
+                    z0 = z1;
+                    z1 = z2;
+                    z2 = z3;
+                    z3 = zn;
+                }
             }
             const double wc = k < k0 ? wz : wv;
             double fzt = 0.0;
             if (zdir && k <= nz - 1) {
-                if constexpr (S == 0 || S == 3) {
+                if constexpr (S == 3) {
+                    // WENO5-JS where the 5-cell upwind stencil lies in the
+                    // column, the reference face elsewhere
+                    if (wc > 0.0 && k >= 3 && k <= nz - 2)
+                        fzt = 0.5 * weno5_left(zm1, z0, z1, z2, z3) * (wc + fabs(wc));
+                    else if (wc < 0.0 && k >= 2 && k <= nz - 3)
+                        fzt = 0.5 * weno5_left(z4, z3, z2, z1, z0) * (wc - fabs(wc));
+                    else
+                        fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
+                } else if constexpr (S == 0) {
                     fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
                 } else {
                     // q(k-1..k+2) from the window, other levels (the periodic

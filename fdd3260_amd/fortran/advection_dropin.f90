@@ -201,6 +201,9 @@ contains
     ! some field feeds a diagnostic: one call per direction into
     ! temporaries, added in the reference's order
     allocate(part(ip_start:ip_end, jp_start:jp_end, 1:nz, fl%n))
+    ! the library writes interior cells only: the ghosts of each temporary
+    ! must hold zero, which the accumulators then receive unchanged
+    part = 0.d0
     do f = 1, fl%n
       pp(f) = field_addr(part(:, :, :, f))
     enddo
@@ -268,6 +271,7 @@ contains
       call fields_tendency(dtl, vel, vel, vel, 1, qp, xp, af, ishft(1, d - 1), lim, .false.)
     else
       allocate(part(ip_start:ip_end, jp_start:jp_end, 1:nz))
+      part = 0.d0  ! ghosts are not written by the library
       xp(1) = field_addr(part)
       call fields_tendency(dtl, vel, vel, vel, 1, qp, xp, af, ishft(1, d - 1), lim, .true.)
       call apply_part(xs, part, slot, d)
@@ -334,6 +338,7 @@ contains
       return
     endif
     allocate(part(ip_start:ip_end, jp_start:jp_end, 1:nz))
+    part = 0.d0  ! ghosts are not written by the library
     xp(1) = field_addr(part)
     do d = 1, 3
       if (d == 2 .and. jp_end == jp_start) cycle

@@ -49,7 +49,9 @@ extern "C" {
 #define MIMADV_WENO 0  /* 'weno': the reference's 3rd-order switch WENO */
 #define MIMADV_PPM 1   /* 'ppm' */
 #define MIMADV_MUSCL 2 /* 'muscl' */
-#define MIMADV_WENO5 3 /* WENO5-JS in x/y (no reference counterpart) */
+#define MIMADV_WENO5 3 /* WENO5-JS (no reference counterpart); in z on faces whose
+                          5-cell upwind stencil lies in the column, the 'weno'
+                          face elsewhere */
 #define MIMADV_QUICK 4 /* 'quick' (advq_quick.f90) */
 #define MIMADV_LW 5    /* Lax-Wendroff with the MC limiter (advq_lw.f90; also any other scal_adv) */
 
@@ -265,7 +267,7 @@ typedef struct mimadv_tuning {
 int mimadv_set_tuning(mimadv_ctx *ctx, const mimadv_tuning *t);
 
 /* Tendency xs_f = -div(q_f) of nf fields sharing one FP32 velocity, with
- * the context's scheme (MIMADV_WENO or MIMADV_MUSCL) in the batched kernel:
+ * the context's scheme (MIMADV_WENO, MIMADV_WENO5 or MIMADV_MUSCL) in the batched kernel:
  * FP32 reconstruction for MIMADV_F32 fields, FP64 for MIMADV_F64 ones (at
  * most 2), flux divergence accumulated in FP64.  Device arrays; xs_f has the
  * precision of q_f.  Lateral boundaries periodic, vertical no-flux.  flags
@@ -296,7 +298,7 @@ int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const voi
 /* ---- many independent subdomains in one launch (no reference counterpart:
  * the reference runs one domain per MPI job) ----
  * Subdomain d is described by a context ctx[d] (its grid, periodic lateral
- * and 'nnf' vertical boundaries, scheme MIMADV_WENO or MIMADV_MUSCL, 3D),
+ * and 'nnf' vertical boundaries, scheme MIMADV_WENO, MIMADV_WENO5 or MIMADV_MUSCL, 3D),
  * its FP32 velocity (u[d], v[d], w[d]) and nf[d] FP32 fields q[d][0..nf-1]
  * (device arrays, at most 18 per subdomain).  Every stage of every
  * subdomain is one work list of one persistent launch; the tile shape

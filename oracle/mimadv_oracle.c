@@ -129,9 +129,16 @@ static void weno3_h(int n, double d, const double *x, const double *u, double *x
     free(flux);
 }
 
-/* advsz_weno (advq_weno.f90:202-301) with k1 = 1, k2 = nz. */
+static double weno5_left(double a, double b, double c, double d, double e);
+
+/* advsz_weno (advq_weno.f90:202-301) with k1 = 1, k2 = nz.  weno5 != 0 is
+ * the WENO5-JS extension in z (no reference counterpart): a face of the
+ * main loop whose 5-cell upwind stencil lies inside the column (w > 0:
+ * 3 <= k <= nz-2, w < 0: 2 <= k <= nz-3) takes the Jiang-Shu face with the
+ * uniform-grid coefficients, as the reference scheme also uses in z; every
+ * other face, the closures included, stays the reference face. */
 static void weno3_z(const orc_grid *g, const orc_opts *o, const double *x, double x0,
-                    const double *w, double *xa, int adv)
+                    const double *w, double *xa, int adv, int weno5)
 {
     int nz = g->nz;
     const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
@@ -154,9 +161,20 @@ static void weno3_z(const orc_grid *g, const orc_opts *o, const double *x, doubl
      * weights only for k = 3..nz-3 (:435). */
     int kstart = (o->weno_bottom == ORC_WENO_BOTTOM_CLOSURE) ? 2 : 1;
     for (int k = kstart; k <= nz - 2; ++k) {
+        double wk = L(w, k + 1);
+        if (weno5 && wk > 0.0 && k >= 3 && k <= nz - 2) {
+            double fl = weno5_left(XZ(k - 2), XZ(k - 1), XZ(k), XZ(k + 1), XZ(k + 2));
+            L(flux, k) = 0.5 * fl * (wk + fabs(wk));
+            continue;
+        }
+        if (weno5 && wk < 0.0 && k >= 2 && k <= nz - 3) {
+            double fr = weno5_left(XZ(k + 3), XZ(k + 2), XZ(k + 1), XZ(k), XZ(k - 1));
+            L(flux, k) = 0.5 * fr * (wk - fabs(wk));
+            continue;
+        }
         double ww[4] = {g1, g2, g2, g1};
         if (k >= 3 && k <= nz - 3) weno3_weights(XZ(k - 1), XZ(k), XZ(k + 1), XZ(k + 2), ww);
-        L(flux, k) = weno3_face(XZ(k - 1), XZ(k), XZ(k + 1), XZ(k + 2), L(w, k + 1), ww, 1);
+        L(flux, k) = weno3_face(XZ(k - 1), XZ(k), XZ(k + 1), XZ(k + 2), wk, ww, 1);
     }
     /* top closure, k = nz-1 (:262-271): no limiter, linear left weights */
     {
@@ -861,7 +879,8 @@ int orc_line_scheme(int scheme, int unlimited, int sorder)
 
 /* ======================================================================
  * Extension (not in the reference): WENO5-JS on x/y lines, periodic data
- * with the reference ghost layout (3 west, 2 east).  Jiang & Shu (1996):
+ * with the reference ghost layout (3 west, 2 east), and in z inside
+ * weno3_z where the upwind stencil fits the column.  Jiang & Shu (1996):
  * candidate stencils (2,-7,11)/6, (-1,5,2)/6, (2,5,-1)/6; linear weights
  * (0.1, 0.6, 0.3); smoothness indicators beta_r; eps = 1e-6; p = 2.
  * The x(n+1) value needed by the last face for negative velocity is taken
@@ -936,10 +955,8 @@ void orc_advs_z(const orc_grid *g, const orc_opts *o, const double *x, double x0
                 const double *w, double *xa, int adv)
 {
     switch (o->scheme) {
-    case ORC_WENO:
-    case ORC_WENO5: /* the extension keeps the reference scheme in z */
-        weno3_z(g, o, x, x0, w, xa, adv);
-        break;
+    case ORC_WENO: weno3_z(g, o, x, x0, w, xa, adv, 0); break;
+    case ORC_WENO5: weno3_z(g, o, x, x0, w, xa, adv, 1); break;
     case ORC_PPM: ppm_z(g, o, x, w, xa, adv); break;
     case ORC_MUSCL: muscl_z(g, o, x, w, xa, adv); break;
     case ORC_QUICK: quick_z(g, o, 0, x, w, xa, adv); break;

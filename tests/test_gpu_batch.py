@@ -41,7 +41,7 @@ def _relmax(got, ref, g):
     return float(np.abs(got - ref)[I].max() / max(np.abs(ref[I]).max(), 1e-300))
 
 
-@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
 @pytest.mark.parametrize("group32", [8, 2])
 def test_batch_tendency(cuda, scheme, group32):
     print(f"case batch_tendency {scheme} group32={group32}", flush=True)
@@ -390,7 +390,7 @@ def test_batch_graph_follows_option_changes(cuda, scheme):
             assert torch.equal(x, y), "graph replay kept the old tiles"
 
 
-@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
 def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
     """Every public tile instance (64x2, 64x4, 64x8 for both precisions) and
     level chunks of 8 and 5 on nz = 17, where the last chunk is ragged
@@ -429,7 +429,7 @@ def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
             assert torch.equal(x, y), f"tile_y/kchunk {key}: field {n} differs from the default shape"
 
 
-@pytest.mark.parametrize("scheme", ["weno", "muscl"])
+@pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
 def test_batch_2d_with_momentum(cuda, scheme):
     """A 2D batch (ny = 1, the 64x1 instances of both kernels): two FP32
     momenta (rho0 u, rho0 w; q[1] is left alone), θ in FP64 and FP32
@@ -468,3 +468,24 @@ def test_batch_2d_with_momentum(cuda, scheme):
                 assert _relmax(got, r, g) <= 1e-10, "fp64 theta"
             else:
                 assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
+
+
+def test_batch_graph_on_the_null_stream(cuda):
+    """With the library on the caller's null stream (torch's default), a
+    graph advance captures on the context's own stream and replays on the
+    null stream; its result equals the eager launches bit for bit."""
+    print("case batch graph on the null stream", flush=True)
+    g, u, v, w, f64, f32 = _case(nspec=2, zero_frac=0.0)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    with Advection(g, scal_adv="weno") as a:
+        a.set_stream(torch.cuda.current_stream(cuda).cuda_stream)
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        res = []
+        for graph in (True, False):
+            fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
+                                                                   for q in f32]
+            a.batch_advance(_lib.INT_SSPRK3, 2, dt, du, dv, dw, fields, graph=graph)
+            torch.cuda.synchronize()
+            res.append([f.clone() for f in fields])
+        for x, y in zip(*res):
+            assert torch.equal(x, y)

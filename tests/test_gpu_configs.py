@@ -296,3 +296,50 @@ def test_config5_dot_product_40_steps_checkpoint_20(cuda):
     scale = float(np.sum(np.abs(lam[I] * d[I]))) + sum(float(np.sum(np.abs(t))) for t in terms)
     _say(f"  <lam, TLM d> = {lhs:.15e}, <ADJ lam, d> = {rhs:.15e}, rel {abs(lhs - rhs) / scale:.2e}")
     assert abs(lhs - rhs) <= 1e-13 * scale
+
+
+def test_config3_fp32_error_sources(cuda, config3):
+    """Where the FP32 species error of the batch comes from, at 512x512x160
+    over 1 and 20 SSP-RK3 steps, each against the FP64 path (plain per-field
+    kernels, bit-identical to the oracle) from the same FP32-rounded state:
+      A  the species of the batch (30 % exact zeros), FP32;
+      B  the same species without the zero mask, FP32;
+      C  species of A carried as FP64 members of the batch (the batch's
+         operation-reduced formulas in FP64).
+    C isolates the formula from the precision, B the precision from the
+    zero cells.  Held: C to 1e-9 of the field's RMS, B to 1e-5 RMS."""
+    g, u, v, w, fields, prec, dt = config3
+    specs = [5, 9, 13]  # mu = -20, -16, -12
+    _say("case config3 FP32 error sources")
+    nostrip = []
+    for n in specs:
+        s = n - 5
+        q = cases.lognormal(g, seed=10 + s, mu=-20 + 14 * s / 14, sigma=1.5, zero_frac=0.0)
+        nostrip.append(q.astype(np.float32).astype(np.float64))
+    with Advection(g, scal_adv="weno") as a:
+        du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
+        U, V, W = (torch.from_numpy(x).to(cuda) for x in (u, v, w))
+        I = g.interior()
+
+        def rel_rms(x, r):
+            d = (x - r).cpu().numpy()[I]
+            rr = r.cpu().numpy()[I]
+            return float(np.sqrt(np.mean(d ** 2)) / max(np.sqrt(np.mean(rr ** 2)), 1e-300))
+
+        for nsteps in (1, 20):
+            for label, data, as64 in (("A zeros FP32", [fields[n] for n in specs], False),
+                                      ("B no zeros FP32", nostrip, False),
+                                      ("C zeros FP64 batch", [fields[n] for n in specs], True)):
+                errs = []
+                for q in data:
+                    ref = torch.from_numpy(q).to(cuda)
+                    a.advance(_lib.INT_SSPRK3, nsteps, dt, U, V, W, [ref])
+                    t = torch.from_numpy(q if as64 else q.astype(np.float32)).to(cuda)
+                    a.batch_advance(_lib.INT_SSPRK3, nsteps, dt, du, dv, dw, [t])
+                    a.stream_sync()
+                    errs.append(rel_rms(t.double(), ref))
+                _say(f"  {nsteps:2d} steps  {label:20s} rel rms " + "  ".join(f"{e:.3e}" for e in errs))
+                if as64:
+                    assert max(errs) <= 1e-9, label
+                elif label.startswith("B"):
+                    assert max(errs) <= 1e-5, label

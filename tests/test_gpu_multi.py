@@ -19,7 +19,7 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 SIZES = [(20, 16, 12, "weno"), (33, 17, 14, "muscl"), (48, 40, 16, "weno"), (16, 50, 10, "muscl"),
-         (70, 24, 12, "weno")]
+         (70, 24, 12, "weno"), (37, 21, 13, "weno5")]
 
 
 def _batch(cuda, seed=201):

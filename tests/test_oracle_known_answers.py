@@ -411,3 +411,40 @@ def check_integ_mom(lib=None):
 
 def test_integ_mom_stages_and_sponge():
     check_integ_mom()
+
+
+def check_weno5_z(lib=None):
+    """WENO5 in z (extension): on cell averages of a cubic in height (uniform
+    levels), a face k+1/2 whose 5-cell upwind stencil lies in the column
+    (w > 0: 3 <= k <= nz-2; w < 0: 2 <= k <= nz-3) has the Jiang-Shu closed
+    form value; the other faces are the reference 'weno' faces."""
+    nz = 14
+    g = gm.uniform(9, 0, nz)
+    o5 = Oracle(g, scal_adv="weno5", lib_path=lib)
+    o3 = Oracle(g, scal_adv="weno", lib_path=lib)
+    coef = (Fr(1, 4), Fr(-1, 3), Fr(1, 5), Fr(-1, 90))
+
+    def avg(k):  # average over [k-1/2, k+1/2] in level units
+        P = lambda s: coef[0] * s + coef[1] * s ** 2 / 2 + coef[2] * s ** 3 / 3 + coef[3] * s ** 4 / 4
+        return P(Fr(k) + Fr(1, 2)) - P(Fr(k) - Fr(1, 2))
+
+    xs = {k: avg(k) for k in range(0, nz + 1)}
+    x = np.array([float(xs[k]) for k in range(1, nz + 1)])
+    for k in range(1, nz - 1):
+        for wf in (1.0, -1.0):
+            w = np.zeros(nz)
+            w[k] = wf  # w(k+1): face k+1/2
+            f5 = o5.line_z(0.0, x, w)[k - 1] / g.dz1w[k - 1]
+            f3 = o3.line_z(0.0, x, w)[k - 1] / g.dz1w[k - 1]
+            if wf > 0 and 3 <= k <= nz - 2:
+                want = float(_js_face(*(xs[k + m] for m in (-2, -1, 0, 1, 2))))
+                assert f5 == pytest.approx(wf * want, rel=1e-12), (k, wf)
+            elif wf < 0 and 2 <= k <= nz - 3:
+                want = float(_js_face(*(xs[k + m] for m in (3, 2, 1, 0, -1))))
+                assert f5 == pytest.approx(wf * want, rel=1e-12), (k, wf)
+            else:
+                assert f5 == f3, (k, wf)
+
+
+def test_weno5_vertical_faces():
+    check_weno5_z()

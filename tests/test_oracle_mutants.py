@@ -107,6 +107,12 @@ MUTANTS = [
      ka.check_weno5_indicators, ()),
     ("weno5 beta2 stencil", "0.25 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e)",
      "0.25 * (3.0 * c - 4.0 * d + e) * (c - 4.0 * d + 3.0 * e)", ka.check_weno5_indicators, ()),
+    ("weno5 z downward stencil", "weno5_left(XZ(k + 3), XZ(k + 2), XZ(k + 1), XZ(k), XZ(k - 1));",
+     "weno5_left(XZ(k - 1), XZ(k), XZ(k + 1), XZ(k + 2), XZ(k + 3));", ka.check_weno5_z, ()),
+    ("weno5 z upward range", "if (weno5 && wk > 0.0 && k >= 3 && k <= nz - 2) {",
+     "if (weno5 && wk > 0.0 && k >= 4 && k <= nz - 2) {", ka.check_weno5_z, ()),
+    ("weno5 z downward range", "if (weno5 && wk < 0.0 && k >= 2 && k <= nz - 3) {",
+     "if (weno5 && wk < 0.0 && k >= 3 && k <= nz - 3) {", ka.check_weno5_z, ()),
     ("weno5 power one", "double a0 = 0.1 / ((eps + b0) * (eps + b0));", "double a0 = 0.1 / (eps + b0);",
      ka.check_weno5_indicators, ()),
 ]
