@@ -135,7 +135,7 @@ def test_batch_rejects_unsupported(cuda):
     g = gm.stretched(20, 16, 12)
     with Advection(g, scal_adv="ppm") as a:
         t = torch.zeros(g.shape, dtype=torch.float32, device=cuda)
-        with pytest.raises(Exception, match="WENO and MUSCL"):
+        with pytest.raises(Exception, match="implements the WENO, WENO5 and MUSCL schemes"):
             a.batch_tendency(t, t, t, [t], [t.clone()])
     # the line-kernel selection flags of the plain path have no batched form
     with Advection(g, scal_adv="weno") as a:
