@@ -58,6 +58,7 @@ EXPORTED = (
     "mimadv_batch_advance",
     "mimadv_batch_budget",
     "mimadv_batch_kernel_ms",
+    "mimadv_batch_budget_device",
     "mimadv_adjoint_tendency",
     "mimadv_adjoint_advance",
     "mimadv_advance_record",
@@ -168,6 +169,8 @@ def load() -> C.CDLL:
     lib.mimadv_batch_advance.argtypes = [vp, i, i, d, vp, vp, vp, i, ip, pp, u]
     lib.mimadv_batch_advance.restype = i
     lib.mimadv_batch_budget.argtypes = [vp, i, ip, pp, C.POINTER(d)]
+    lib.mimadv_batch_budget_device.argtypes = [vp, i, ip, pp, vp]
+    lib.mimadv_batch_budget_device.restype = i
     lib.mimadv_batch_kernel_ms.argtypes = [vp, C.POINTER(d), ip]
     lib.mimadv_batch_kernel_ms.restype = i
     lib.mimadv_adjoint_tendency.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]

@@ -302,6 +302,17 @@ class Advection:
                                            len(fields), pr, pq, flags)
         self._check(rc, "mimadv_batch_advance")
 
+    def batch_budget_device(self, fields, out):
+        """Budgets of the fields into the float64 GPU tensor out[len(fields)]
+        with no device allocation (mimadv_batch_budget_device)."""
+        import torch
+
+        pr, pq = self._batch_fields(fields)
+        if not _is_torch(out) or out.dtype != torch.float64 or not out.is_cuda or out.numel() < len(fields):
+            raise MimadvError("out must be a float64 GPU tensor with one entry per field")
+        rc = self.lib.mimadv_batch_budget_device(self._h, len(fields), pr, pq, out.data_ptr())
+        self._check(rc, "mimadv_batch_budget_device")
+
     def batch_kernel_ms(self):
         """Per-kind kernel time of the last profiled batched call: {kind: (ms, launches)}
         for kinds 'momentum', 'f64', 'f32'."""

@@ -27,6 +27,7 @@ struct mimadv_ctx {
     double *d_red = nullptr;      // reduction scratch [2 + nz]
     double *d_budget = nullptr;   // budget results and per-block partials
     size_t budget_elems = 0;
+    double *budget_scratch = nullptr;  // this device's static budget scratch
     // staging buffers for MIMADV_HOST_PTRS calls
     double *stage = nullptr;
     size_t stage_elems = 0;
@@ -295,6 +296,8 @@ int mimadv_create(const mimadv_grid *grid, const mimadv_opts *opts, mimadv_ctx *
     c->g.zm.ppmc = c->d_metrics + 4 * nz;
     c->g.zm.momz = (const float *)(c->d_metrics + 4 * nz + ppmc.size());
     c->g.zm.dz = grid->dz;
+    e = budget_scratch_setup(c->device, &c->budget_scratch);
+    if (e != hipSuccess) return bad(hipfail(c, e, "hipGetSymbolAddress of the budget scratch"));
     e = hipMalloc(&c->d_red, (size_t)(2 + nz) * sizeof(double));
     if (e != hipSuccess) return bad(fail(c, MIMADV_ERR_NOMEM, "reduction allocation failed"));
     e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
@@ -1097,6 +1100,35 @@ int mimadv_batch_kernel_ms(mimadv_ctx *c, double *ms, int *count)
         ms[c->prof_kind[s]] += t;
         count[c->prof_kind[s]] += 1;
     }
+    return MIMADV_OK;
+}
+
+int mimadv_batch_budget_device(mimadv_ctx *c, int nf, const int *precision, const void *const *q, double *d_out)
+{
+    if (!c) return MIMADV_ERR_ARG;
+    if (nf < 0 || (nf > 0 && (!precision || !q || !d_out))) return fail(c, MIMADV_ERR_ARG, "bad field list");
+    if (nf > BUDGET_MAXF)
+        return fail(c, MIMADV_ERR_ARG, "mimadv_batch_budget_device takes at most 32 fields per call (its static "
+                                       "scratch holds the partials of 32 fields)");
+    if (nf == 0) return MIMADV_OK;
+    if (!c->budget_scratch) return fail(c, MIMADV_ERR_HIP, "budget scratch address unknown on this device");
+    BudgetArgs a{};
+    a.nx = c->g.nx;
+    a.ny = c->g.ny;
+    a.nz = c->g.nz;
+    a.nb = budget_blocks(c->g);
+    a.dzw = c->g.zm.dzw;
+    for (int f = 0; f < nf; ++f) {
+        if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
+        if (precision[f] != MIMADV_F32 && precision[f] != MIMADV_F64)
+            return fail(c, MIMADV_ERR_ARG, "precision must be MIMADV_F32 or MIMADV_F64");
+        a.q[f] = q[f];
+        a.fp32[f] = precision[f] == MIMADV_F32;
+    }
+    // every partial the call reads is written by the same call first:
+    // nothing in the shared scratch needs clearing
+    a.part = c->budget_scratch;
+    HIPCHK(c, launch_budgets(a, nf, d_out, c->stream));
     return MIMADV_OK;
 }
 

@@ -121,7 +121,12 @@ struct BudgetArgs {
     double *part;
 };
 constexpr int BUDGET_MAXNB = 1024;  // first-pass blocks per field
+constexpr size_t BUDGET_SCRATCH = (size_t)BUDGET_MAXF * (BUDGET_MAXNB + 1);
+constexpr int BUDGET_MAX_DEVICES = 64;
 int budget_blocks(const GridDev &g);
+// Device address of the budget scratch on `device` (made current by the
+// caller), looked up once per device and kept.
+hipError_t budget_scratch_setup(int device, double **out);
 hipError_t launch_budgets(const BudgetArgs &a, int nf, double *d_out, hipStream_t s);
 
 // Host-side PPM per-level coefficient table (12 doubles per level).

@@ -1,6 +1,7 @@
 // util.hip -- ghost fill, stage combine and interior reductions.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "mimadv_internal.hpp"
 
@@ -282,6 +283,36 @@ int budget_blocks(const GridDev &g)
 {
     const long rows = (long)(g.ny > 1 ? g.ny - 5 : 1) * g.nz;
     return (int)std::min<long>(rows, BUDGET_MAXNB);
+}
+
+// Scratch of the allocation-free budget entry point (mimadv_batch_budget_
+// device): the per-block partials of up to BUDGET_MAXF fields,
+// BUDGET_SCRATCH doubles (262 KB).  Each GPU holds its own copy until the
+// process ends; host code uses only the address hipGetSymbolAddress gives
+// for the current device, never this name.
+static __device__ __align__(256) double scratch[BUDGET_SCRATCH];
+
+namespace {
+
+std::mutex scratch_mutex;
+double *scratch_ptr[BUDGET_MAX_DEVICES] = {};
+
+}  // namespace
+
+hipError_t budget_scratch_setup(int device, double **out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= BUDGET_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[device]) {
+        // the caller made `device` current
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[device] = (double *)p;
+    }
+    *out = scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_budgets(const BudgetArgs &a, int nf, double *d_out, hipStream_t s)

@@ -295,6 +295,19 @@ int mimadv_batch_kernel_ms(mimadv_ctx *ctx, double ms[3], int count[3]);
  * FP64 (out[f]). */
 int mimadv_batch_budget(mimadv_ctx *ctx, int nf, const int *precision, const void *const *q, double *out);
 
+/* mimadv_batch_budget for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc); it has no counterpart in the reference.  It allocates
+ * nothing: the per-block partials go to a static device array of the
+ * library (262 KB per GPU, held until the process ends), so nf is at most
+ * 32 (MIMADV_ERR_ARG beyond), and the results go to the caller's DEVICE
+ * array d_out[nf]; the call is ordered on the context stream and does not
+ * wait.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (two streams, or two host threads) race.  Same values as
+ * mimadv_batch_budget, bit for bit. */
+int mimadv_batch_budget_device(mimadv_ctx *ctx, int nf, const int *precision, const void *const *q,
+                               double *d_out);
+
 /* ---- many independent subdomains in one launch (no reference counterpart:
  * the reference runs one domain per MPI job) ----
  * Subdomain d is described by a context ctx[d] (its grid, periodic lateral
