@@ -33,12 +33,29 @@ __device__ __forceinline__ double tmin(double a, double b) { return __builtin_fm
 __device__ __forceinline__ float tmax(float a, float b) { return __builtin_fmaxf(a, b); }
 __device__ __forceinline__ double tmax(double a, double b) { return __builtin_fmax(a, b); }
 
-// Reference WENO face flux, operation-reduced.  a..d = q(f-1..f+2), uu the
-// face velocity, nonlin: whether the smoothness switch is active at this face
-// (always in x/y, k = 3..nz-3 in z).  The candidates are formed from the
-// cell differences: ffim1 = b + (b-a)/2, ffip1 = b + (c-b)/2, ffip2 =
-// c - (d-c)/2; only the upwind outer candidate and its smoothness enter
-// the weights.
+// Reference WENO face flux (advq_weno.f90:92-110 with weight{x,y,z}
+// :329-350 and limiter_weno :462-491), rearranged so that FP32 resolves
+// the limiter's eps scale.  a..d = q(f-1..f+2), uu the face velocity,
+// nonlin: whether the smoothness switch is active at this face (always in
+// x/y, k = 3..nz-3 in z).  Candidates ffim1 = b + (b-a)/2, ffip1 =
+// b + (c-b)/2, ffip2 = c - (d-c)/2; only the upwind outer candidate fo and
+// its smoothness enter the weights (wo outer, wi inner, wo + wi = 1).
+//
+// limiter_weno rescales every candidate about b by phi = min(|b/D|, 1),
+// D = minp - b + eps, so the face is F = b + phi (Fc - b) with Fc =
+// wi ffip1 + wo fo.  Where a candidate undershoots zero, phi pulls minp to
+// -eps b/|D|: cells next to exact zeros receive values of order 1e-15,
+// and at the next stage those values (comparable to eps) decide phi of
+// their own faces.  b + phi (Fc - b) cannot resolve 1e-15 against
+// b ~ 1e-5 in FP32, so the face is evaluated in the equivalent form
+//   e  = Fc - minp = wi (ffip1 - minp) + wo (fo - minp)   (terms >= 0)
+//   m  = minp + eps,  D = m - b,  t = b (eps - e) / D
+//   F  = t                  if m and b have opposite signs (phi < 1)
+//      = 2b - t             if |m| > 2|b| with the same sign (phi < 1)
+//      = minp + e           otherwise (phi = 1),
+// whose branch tests compare FP32 values exactly and whose limited value
+// carries no cancellation of b.  In exact arithmetic it equals the
+// reference face.
 template <class T>
 __device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
 {
@@ -47,9 +64,7 @@ __device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
     const T ffim1 = b + T(0.5) * dm;
     const T ffip1 = b + T(0.5) * dc;
     const T ffip2 = c - T(0.5) * dp;
-    // limiter_weno: rescale about b (the left/lower cell)
     const T minp = tmin(tmin(ffim1, ffip1), ffip2);
-    const T phi = tmin(fabs(b * rcp((minp - b) + eps)), T(1));
     const bool pos = uu > T(0);
     const T dout = pos ? dm : dp;
     const T fo = pos ? ffim1 : ffip2;
@@ -57,18 +72,24 @@ __device__ __forceinline__ T weno_face(T a, T b, T c, T d, T uu, bool nonlin)
     // switch max(eta) > 20 (min(eta) + eps) in these terms is
     // max(e) > 20 min(e) + eps
     const T ei = dc * dc + eps, eo = dout * dout + eps;
-    T wo = T(1.0 / 3.0);
+    T wo = T(1.0 / 3.0), wi = T(2.0 / 3.0);
     if (nonlin) {
         const T mx = tmax(eo, ei), mn = tmin(eo, ei);
         if (mx > thr * mn + eps) {
+            // (g1/so) / (g1/so + g2/si) with g2 = 2 g1, and its complement
             const T so = eo * eo, si = ei * ei;
-            // (g1/so) / (g1/so + g2/si) with g2 = 2 g1
-            wo = si * rcp(si + T(2) * so);
+            const T r = rcp(si + T(2) * so);
+            wo = si * r;
+            wi = T(2) * so * r;
         }
     }
-    // b + phi (F - b) with F - b = (ffip1 - b) + wo (fo - ffip1)
-    const T Fb = T(0.5) * dc + wo * (fo - ffip1);
-    return uu * (b + phi * Fb);
+    const T e = wi * (ffip1 - minp) + wo * (fo - minp);
+    const T m = minp + eps;
+    const T t = b * (eps - e) * rcp(m - b);
+    T F = minp + e;
+    if ((m < T(0)) != (b < T(0))) F = t;
+    else if (fabs(m) > T(2) * fabs(b)) F = T(2) * b - t;
+    return uu * F;
 }
 
 // WENO5-JS (Jiang & Shu 1996) value at the right face of c from a..e =

@@ -23,3 +23,15 @@ def relmax(got, ref):
     """Norm-wise relative error max|got-ref| / max|ref|."""
     den = max(float(np.abs(ref).max()), 1e-300)
     return float(np.abs(got - ref).max()) / den
+
+
+def split_flips(got, ref, g, thr=1e-4):
+    """Per-cell error relative to the field's interior RMS, split at thr:
+    (number of cells above thr, RMS of the cells at or below thr).  The
+    cells above are those where FP32 storage of a stage value flipped the
+    ratio-20 smoothness switch (a flip moves one cell by 1e-3..1e-1 of the
+    field RMS); the rest carry FP32 round-off."""
+    I = g.interior()
+    e = np.abs(got - ref)[I] / max(float(np.sqrt(np.mean(ref[I] ** 2))), 1e-300)
+    out = e > thr
+    return int(out.sum()), float(np.sqrt(np.mean(np.where(out, 0.0, e) ** 2)))

@@ -13,6 +13,8 @@ from fdd3260_amd import Advection, _lib, cases
 from fdd3260_amd import grid as gm
 from oracle.oracle import Oracle
 
+from helpers import split_flips
+
 pytestmark = pytest.mark.gpu
 
 
@@ -71,19 +73,15 @@ def test_batch_tendency(cuda, scheme, group32):
 @pytest.mark.parametrize("zero_frac", [0.0, 0.3])
 @pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
 def test_batch_advance_and_graph(cuda, integrator, zero_frac):
-    """Three steps against the FP64 oracle trajectory.
-
-    FP32 species without exact zeros are held to 1e-5 relative RMS.  With
-    the 30% exact-zero mask the reference scheme itself is discontinuous at
-    those cells: limiter_weno's phi = |b/(minp-b+eps)| jumps from 0 (b = 0)
-    to O(1) once |b| exceeds ~eps = 1e-15, and the 1e-13-sized values that
-    FP64 stages leave in zero cells are below FP32's resolution of the
-    neighbouring fluxes.  Those fields are held to the per-tendency bound
-    (test_batch_tendency), to mass conservation, and to 1e-2 relative RMS
-    along the trajectory (measured 1e-4 to 1e-3)."""
+    """Three steps against the FP64 oracle trajectory, FP32 species with
+    and without the 30% exact-zero mask held to 1e-5 relative RMS.  Next to
+    exact zeros limiter_weno pulls face values to -eps b/|minp - b + eps|
+    (eps = 1e-15), and at the next stage those 1e-15-sized cell values
+    decide phi; the batched face evaluates the limited value in a form that
+    resolves that scale in FP32, so zeros cost no accuracy."""
     print(f"case batch_advance {integrator} zero_frac={zero_frac}", flush=True)
     g, u, v, w, f64, f32 = _case(nspec=4, zero_frac=zero_frac)
-    traj_tol = 1e-5 if zero_frac == 0.0 else 1e-2
+    traj_tol = 1e-5
     dt = cases.cfl_dt(g, u, v, w, 0.5)
     nsteps = 3
     ora = Oracle(g, scal_adv="weno")
@@ -306,9 +304,10 @@ def test_batch_config3_field_errors(cuda):
     3 FP32 momenta, θ and qv in FP64, 15 FP32 log-normal species with 30 %
     exact zeros, 3 m/s wind), one SSP-RK3 step against the FP64 oracle from
     the same FP32-rounded state.  Prints max and RMS error per field
-    (relative to the field's max and RMS) and holds the momenta and the FP64
-    members to their bounds; species with exact zeros carry the reference
-    scheme's discontinuity (see test_batch_advance_and_graph)."""
+    (relative to the field's max and RMS) and holds the momenta, the FP64
+    members and the species (zeros included) to their bounds: species at
+    FP32 round-off except in the few cells where a stage value stored in
+    FP32 flips the ratio-20 smoothness switch."""
     print("case batch_config3_field_errors", flush=True)
     g = gm.stretched(128, 128, 64)
     u, v, w = cases.turbulent_wind(g, seed=2, rms=3.0)
@@ -344,13 +343,17 @@ def test_batch_config3_field_errors(cuda):
     names = ["rho0u", "rho0v", "rho0w", "theta", "qv"] + [f"species{s}" for s in range(15)]
     for n, (x, r) in enumerate(zip(got, refs)):
         mx, rms = _relmax(x, r, g), _rel_rms(x, r, g)
-        print(f"  {names[n]:10s} max {mx:.3e}  rms {rms:.3e}", flush=True)
+        nflip, rms_in = split_flips(x, r, g)
+        print(f"  {names[n]:10s} max {mx:.3e}  rms {rms:.3e}  flip cells {nflip}  rms of the rest {rms_in:.3e}",
+              flush=True)
         if n < 3:
             assert mx <= 1e-5 and rms <= 1e-6, names[n]
         elif n < 5:
             assert mx <= 1e-9, names[n]
         else:
-            assert rms <= 1e-2, names[n]
+            # FP32 round-off but in the cells where a stage value stored in
+            # FP32 flipped the ratio-20 switch
+            assert rms_in <= 1e-6 and nflip <= 4, names[n]
 
 
 @pytest.mark.parametrize("scheme", ["weno", "muscl"])
