@@ -29,6 +29,7 @@ DIR_X, DIR_Y, DIR_Z, DIR_ALL = 1, 2, 4, 7
 OVERWRITE = 8
 HOST_PTRS = 16
 UNLIMITED = 128
+POSITIVE = 2048
 SORDER1 = 256
 DREF_DEN0 = 512
 GRAPH = 32

@@ -230,7 +230,7 @@ class Advection:
         self._check(rc, "mimadv_totalav")
         return mean.value, wsum.value
 
-    def advance(self, integrator, nsteps, dt, u, v, w, fields, adv=None, limit=True):
+    def advance(self, integrator, nsteps, dt, u, v, w, fields, adv=None, limit=True, positive=False):
         ptrs, hflag = self._ptrs([u, v, w] + list(fields), writable=fields)
         if hflag:
             raise MimadvError("advance works on device tensors")
@@ -239,7 +239,8 @@ class Advection:
         if adv is not None:
             ad = (C.c_int * nf)(*[1 if a else 0 for a in adv])
         rc = self.lib.mimadv_advance(self._h, int(integrator), int(nsteps), float(dt), ptrs[0], ptrs[1], ptrs[2],
-                                     nf, _lib.ptr_array(ptrs[3:]), ad, 0 if limit else _lib.UNLIMITED)
+                                     nf, _lib.ptr_array(ptrs[3:]), ad,
+                                     (0 if limit else _lib.UNLIMITED) | (_lib.POSITIVE if positive else 0))
         self._check(rc, "mimadv_advance")
 
     # -- batched mixed-precision path ---------------------------------------

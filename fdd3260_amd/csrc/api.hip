@@ -150,12 +150,69 @@ int ensure_work(mimadv_ctx *c, size_t count)
     return MIMADV_OK;
 }
 
+// MIMADV_POSITIVE steps: Shu-Osher stages whose outgoing fluxes are scaled
+// per donor cell (launch_positive_stage), two stage arrays per field plus
+// the flux and theta scratch in the work buffer.
+int advance_positive(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
+                     const double *w, int nf, double *const *q, const int *adv_form, unsigned flags)
+{
+    if (integrator != MIMADV_INT_SSPRK3 && integrator != MIMADV_INT_HEUN)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "MIMADV_POSITIVE needs MIMADV_INT_SSPRK3 or MIMADV_INT_HEUN");
+    if (c->sp.scheme != MIMADV_WENO || (flags & MIMADV_UNLIMITED))
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "MIMADV_POSITIVE implements the limited 'weno' scheme");
+    if (c->g.bcl[4] != MIMADV_BC_NNF || c->g.bcl[5] != MIMADV_BC_NNF)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "MIMADV_POSITIVE needs 'nnf' floor and lid");
+    for (int f = 0; f < nf; ++f)
+        if (adv_form && adv_form[f])
+            return fail(c, MIMADV_ERR_UNSUPPORTED, "MIMADV_POSITIVE works on flux-form fields");
+    int rc = ensure_work(c, (size_t)2 * nf + 4);
+    if (rc) return rc;
+    const size_t n3 = c->n3();
+    for (int f = 0; f < nf; ++f) {
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->work + (2 * f + 1) * n3, q[f], n3 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    double *scr = c->work + (size_t)2 * nf * n3;
+    struct Stage {
+        double c0, c1;
+        int in, out;  // 0: q, 1: w1, 2: w2
+    };
+    std::vector<Stage> st;
+    if (integrator == MIMADV_INT_HEUN) st = {{0.0, 1.0, 0, 1}, {0.5, 0.5, 1, 0}};
+    else st = {{0.0, 1.0, 0, 1}, {0.75, 0.25, 1, 2}, {1.0 / 3.0, 2.0 / 3.0, 2, 0}};
+    for (int s = 0; s < nsteps; ++s)
+        for (const Stage &g : st)
+            for (int f = 0; f < nf; ++f) {
+                double *w1 = c->work + (2 * f) * n3, *w2 = c->work + (2 * f + 1) * n3;
+                auto pick = [&](int which) { return which == 0 ? q[f] : (which == 1 ? w1 : w2); };
+                PosArgs a{};
+                a.q = pick(g.in);
+                a.u = u;
+                a.v = v ? v : u;
+                a.w = w;
+                a.q0 = q[f];
+                a.qn = pick(g.out);
+                a.c0 = g.c0;
+                a.c1 = g.c1;
+                a.dt = dt;
+                a.dx1 = 1.0 / c->g.dx;
+                a.dy1 = 1.0 / c->g.dy;
+                a.fx = scr;
+                a.fy = scr + n3;
+                a.fz = scr + 2 * n3;
+                a.th = scr + 3 * n3;
+                HIPCHK(c, launch_positive_stage(c->g, c->sp, a, c->stream));
+            }
+    return MIMADV_OK;
+}
+
 // nsteps steps of the advection-only loop for nf fields (mimadv_advance
 // without its checks and timing): per stage, tendency of every field,
 // stage combine and periodic ghost refresh in one launch per field.
 int advance_steps(mimadv_ctx *c, int integrator, int nsteps, double dt, const double *u, const double *v,
                   const double *w, int nf, double *const *q, const int *adv_form, unsigned flags = 0)
 {
+    if (flags & MIMADV_POSITIVE) return advance_positive(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form, flags);
     int rc = ensure_work(c, (size_t)2 * nf);
     if (rc) return rc;
     size_t n3 = c->n3();
@@ -626,7 +683,8 @@ int mimadv_advance(mimadv_ctx *c, int integrator, int nsteps, double dt, const d
     for (int f = 0; f < nf; ++f)
         if (!q[f]) return fail(c, MIMADV_ERR_ARG, "field pointer is NULL");
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    int rc = advance_steps(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form, flags & MIMADV_UNLIMITED);
+    int rc = advance_steps(c, integrator, nsteps, dt, u, v, w, nf, q, adv_form,
+                           flags & (MIMADV_UNLIMITED | MIMADV_POSITIVE));
     if (rc) return rc;
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));

@@ -65,6 +65,17 @@ hipError_t launch_adjoint_stage(const AdjArgs &a, hipStream_t s);
 // dst = src on interior cells, 0 on ghost cells
 hipError_t launch_interior_copy(const GridDev &g, const double *src, double *dst, hipStream_t s);
 hipError_t launch_scalar_fp64(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, hipStream_t s);
+
+// One positivity-preserving Shu-Osher stage of the flux-form 'weno'
+// tendency: qn = c0*q0 + c1*(q + dt*L~(q)), L~ from face fluxes scaled by
+// their donor cell's theta (scratch fx, fy, fz, th: nx*ny*nz each).
+struct PosArgs {
+    const double *q, *u, *v, *w, *q0;
+    double *qn;
+    double c0, c1, dt, dx1, dy1;
+    double *fx, *fy, *fz, *th;
+};
+hipError_t launch_positive_stage(const GridDev &g, const SchemeParams &sp, const PosArgs &a, hipStream_t s);
 // column-marching kernels (weno_march.hip): 'weno', WENO5, MUSCL, QUICK,
 // Lax-Wendroff and the unlimited forms; kchunk 0 = default
 hipError_t launch_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s);

@@ -26,6 +26,8 @@
 // starting above the floor first evaluates its bottom face.
 //
 // Compiled with -ffp-contract=off.
+#include <algorithm>
+
 #include "mimadv_internal.hpp"
 #include "weno_ratio.hpp"
 
@@ -506,6 +508,136 @@ hipError_t launch_dims(const GridDev &g, const SchemeParams &sp, const ScalarArg
 }
 
 }  // namespace
+
+// ---- positivity-preserving Shu-Osher stage (MIMADV_POSITIVE; no
+// reference counterpart).  Three passes over the grid, one thread per
+// cell: the 'weno' face fluxes (the same face functions as the march
+// kernel, so the same bits), the per-cell outflow factor theta, and the
+// stage from the donor-scaled fluxes.  The arithmetic follows the oracle's
+// orc_positive_stage operation for operation.
+namespace {
+
+__device__ __forceinline__ int wrapi(int i, int n, int m) { return i < 4 ? i + m : (i > n - 2 ? i - m : i); }
+
+__global__ __launch_bounds__(256) void pos_flux_kernel(GridDev g, SchemeParams sp, PosArgs a)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool three_d = ny > 1;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny, n3 = sz * nz;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n3; id += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(id % sy) + 1, j = (int)((id / sy) % ny) + 1, k = (int)(id / sz) + 1;
+        const bool irow = three_d ? (j >= 4 && j <= ny - 2) : true;
+        const bool icol = i >= 4 && i <= nx - 2;
+        const double *q = a.q;
+        // x face i+1/2 of the interior rows
+        double fx = 0.0;
+        if (irow && i >= 3 && i <= nx - 2) fx = weno_face_up(q[id - 1], q[id], q[id + 1], q[id + 2], a.u[id + 1], true);
+        a.fx[id] = fx;
+        // y face j+1/2 of the interior columns
+        double fy = 0.0;
+        if (three_d && icol && j >= 3 && j <= ny - 2)
+            fy = weno_face_up(q[id - sy], q[id], q[id + sy], q[id + 2 * sy], a.v[id + sy], true);
+        a.fy[id] = fy;
+        // z face k+1/2 of the interior columns, q(0) = q(1)
+        double fz = 0.0;
+        if (irow && icol && k <= nz - 1) {
+            const double z0 = k == 1 ? q[id] : q[id - sz];
+            const double z3 = k + 2 <= nz ? q[id + 2 * sz] : 0.0;
+            fz = zface(k, nz, sp.weno_bottom, z0, q[id], q[id + sz], z3, a.w[id + sz]);
+        }
+        a.fz[id] = fz;
+    }
+}
+
+__device__ __forceinline__ double pos_part(double f) { return f > 0.0 ? f : 0.0; }
+__device__ __forceinline__ double neg_part(double f) { return f < 0.0 ? -f : 0.0; }
+
+__global__ __launch_bounds__(256) void pos_theta_kernel(GridDev g, PosArgs a)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool three_d = ny > 1;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny, n3 = sz * nz;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n3; id += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(id % sy) + 1, j = (int)((id / sy) % ny) + 1, k = (int)(id / sz) + 1;
+        if (i < 4 || i > nx - 2 || (three_d && (j < 4 || j > ny - 2))) continue;
+        const double ox = (pos_part(a.fx[id]) + neg_part(a.fx[id - 1])) * a.dx1;
+        double o = ox;
+        if (three_d) o = o + (pos_part(a.fy[id]) + neg_part(a.fy[id - sy])) * a.dy1;
+        const double fzt = k < nz ? a.fz[id] : 0.0, fzb = k > 1 ? a.fz[id - sz] : 0.0;
+        o = o + (pos_part(fzt) + neg_part(fzb)) * g.zm.dz1w[k - 1];
+        o = a.dt * o;
+        const double qp = a.q[id] > 0.0 ? a.q[id] : 0.0;
+        a.th[id] = o > qp ? (qp / o) * (1.0 - 0x1p-40) : 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void pos_update_kernel(GridDev g, PosArgs a)
+{
+    const int nx = g.nx, ny = g.ny, nz = g.nz, mx = nx - 5, my = ny - 5;
+    const bool three_d = ny > 1;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny, n3 = sz * nz;
+    for (size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x; id < n3; id += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(id % sy) + 1, j = (int)((id / sy) % ny) + 1, k = (int)(id / sz) + 1;
+        if (i < 4 || i > nx - 2 || (three_d && (j < 4 || j > ny - 2))) continue;
+        const size_t kb = sz * (size_t)(k - 1);
+        const size_t rowb = kb + sy * (size_t)(j - 1);
+        // flux of x face f+1/2 of this row, scaled by its donor's theta
+        auto fxs = [&](int f) {
+            const double F = a.fx[rowb + (size_t)(f - 1)];
+            return F * (F > 0.0 ? a.th[rowb + (size_t)(wrapi(f, nx, mx) - 1)]
+                                : a.th[rowb + (size_t)(wrapi(f + 1, nx, mx) - 1)]);
+        };
+        auto fys = [&](int f) {
+            const size_t cb = kb + (size_t)(i - 1);
+            const double F = a.fy[cb + sy * (size_t)(f - 1)];
+            return F * (F > 0.0 ? a.th[cb + sy * (size_t)(wrapi(f, ny, my) - 1)]
+                                : a.th[cb + sy * (size_t)(wrapi(f + 1, ny, my) - 1)]);
+        };
+        auto fzs = [&](int f) {
+            const size_t cb = (size_t)(i - 1) + sy * (size_t)(j - 1);
+            const double F = a.fz[cb + sz * (size_t)(f - 1)];
+            return F * (F > 0.0 ? a.th[cb + sz * (size_t)(f - 1)] : a.th[cb + sz * (size_t)f]);
+        };
+        const double X = (fxs(i) - fxs(i - 1)) * a.dx1;
+        const double Y = three_d ? (fys(j) - fys(j - 1)) * a.dy1 : 0.0;
+        double Z;
+        if (k == 1) Z = (fzs(1) - 0.0) * g.zm.dz1w[0];
+        else if (k == nz) Z = (0.0 - fzs(nz - 1)) * g.zm.dz1w[nz - 1];
+        else Z = (fzs(k) - fzs(k - 1)) * g.zm.dz1w[k - 1];
+        double A = 0.0;
+        A = A - X;
+        if (three_d) A = A - Y;
+        A = A - Z;
+        const double qn = a.c0 * a.q0[id] + a.c1 * (a.q[id] + a.dt * A);
+        a.qn[id] = qn;
+        // periodic images (qnbc_p4, bc.f90:409-414, :431-436)
+        int ix = 0, jy = 0;
+        if (i >= nx - 4) ix = i - mx;
+        else if (i <= 5) ix = i + mx;
+        if (three_d) {
+            if (j >= ny - 4) jy = j - my;
+            else if (j <= 5) jy = j + my;
+        }
+        double *qo = a.qn + kb;
+        if (ix) qo[(size_t)(ix - 1) + sy * (j - 1)] = qn;
+        if (jy) {
+            qo[(size_t)(i - 1) + sy * (jy - 1)] = qn;
+            if (ix) qo[(size_t)(ix - 1) + sy * (jy - 1)] = qn;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_positive_stage(const GridDev &g, const SchemeParams &sp, const PosArgs &a, hipStream_t s)
+{
+    const size_t n3 = (size_t)g.nx * g.ny * g.nz;
+    const int blocks = (int)std::min<size_t>((n3 + 255) / 256, (size_t)256 * 64);
+    hipLaunchKernelGGL(pos_flux_kernel, dim3(blocks), dim3(256), 0, s, g, sp, a);
+    hipLaunchKernelGGL(pos_theta_kernel, dim3(blocks), dim3(256), 0, s, g, a);
+    hipLaunchKernelGGL(pos_update_kernel, dim3(blocks), dim3(256), 0, s, g, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_march(const GridDev &g, const SchemeParams &sp, const ScalarArgs &a, int kchunk, hipStream_t s)
 {

@@ -83,6 +83,18 @@ extern "C" {
 #define MIMADV_UNLIMITED 128u
 /* sorder = 1: limited Lax-Wendroff whatever the scheme (advection.f90:598-601) */
 #define MIMADV_SORDER1 256u
+/* Positivity-preserving stages for mimadv_advance (no reference
+ * counterpart): every Euler substep q + dt*L(q) of the Shu-Osher form
+ * scales each cell's outgoing face fluxes by one factor theta = 1, or
+ * (1 - 2^-40) max(q,0) / (dt * outgoing flux per width) where that flux
+ * exceeds the content, applied to the
+ * fluxes the cell donates (left/lower cell for a positive flux, right/upper
+ * for a negative one).  Mass is conserved, non-negative fields stay
+ * non-negative, and where no cell needs it (theta = 1 everywhere) the step
+ * is the reference step bit for bit.  Scheme 'weno' in flux form, periodic
+ * sides, 'nnf' floor and lid, integrators MIMADV_INT_SSPRK3 and
+ * MIMADV_INT_HEUN (Heun in its Shu-Osher form 1/2 q + 1/2 (q* + dt L(q*))). */
+#define MIMADV_POSITIVE 2048u
 
 /* Grid: extents, spacings and vertical metrics (define_grid.f90:179-187).
  * Metric arrays are host arrays of length nz; they are copied at create. */
@@ -196,7 +208,7 @@ int mimadv_totalav(mimadv_ctx *ctx, const double *x, double *mean, double *dzw_s
  * Lax-Wendroff scheme, or the unlimited one with MIMADV_UNLIMITED),
  * MIMADV_INT_HEUN or MIMADV_INT_SSPRK3.  Lateral boundaries must be
  * periodic.  q holds the fields (in place); dt is also the dt0 of PPM and
- * Lax-Wendroff.  flags may contain MIMADV_UNLIMITED. */
+ * Lax-Wendroff.  flags may contain MIMADV_UNLIMITED or MIMADV_POSITIVE. */
 int mimadv_advance(mimadv_ctx *ctx, int integrator, int nsteps, double dt, const double *u,
                    const double *v, const double *w, int nf, double *const *q, const int *adv_form,
                    unsigned flags);

@@ -103,17 +103,24 @@ static double weno3_face(double a, double b, double c, double d, double uu,
     return 0.5 * fl * (uu + fabs(uu)) + 0.5 * fr * (uu - fabs(uu));
 }
 
+/* The face fluxes of advsx_weno / advsy_weno (advq_weno.f90:92-110):
+ * flux(i) at face i+1/2 for i = 3..n-2. */
+static void weno3_h_flux(int n, const double *x, const double *u, double *flux)
+{
+    for (int i = 3; i <= n - 2; ++i) {
+        double ww[4];
+        weno3_weights(L(x, i - 1), L(x, i), L(x, i + 1), L(x, i + 2), ww);
+        L(flux, i) = weno3_face(L(x, i - 1), L(x, i), L(x, i + 1), L(x, i + 2), L(u, i + 1), ww, 1);
+    }
+}
+
 /* advsx_weno / advsy_weno with i1 = 4, i2 = n-2 (advq_weno.f90:61-131). */
 static void weno3_h(int n, double d, const double *x, const double *u, double *xa, int adv)
 {
     int ii1 = 3, ii2 = n - 2;
     double d1 = 1.0 / d;
     double *flux = (double *)calloc((size_t)n + 1, sizeof(double));
-    for (int i = ii1; i <= ii2; ++i) {
-        double ww[4];
-        weno3_weights(L(x, i - 1), L(x, i), L(x, i + 1), L(x, i + 2), ww);
-        L(flux, i) = weno3_face(L(x, i - 1), L(x, i), L(x, i + 1), L(x, i + 2), L(u, i + 1), ww, 1);
-    }
+    weno3_h_flux(n, x, u, flux);
     double fold = L(flux, ii1);
     for (int i = ii1 + 1; i <= ii2; ++i) {
         L(xa, i) = L(xa, i) + (L(flux, i) - fold) * d1;
@@ -137,12 +144,13 @@ static double weno5_left(double a, double b, double c, double d, double e);
  * 3 <= k <= nz-2, w < 0: 2 <= k <= nz-3) takes the Jiang-Shu face with the
  * uniform-grid coefficients, as the reference scheme also uses in z; every
  * other face, the closures included, stays the reference face. */
-static void weno3_z(const orc_grid *g, const orc_opts *o, const double *x, double x0,
-                    const double *w, double *xa, int adv, int weno5)
+/* The face fluxes of advsz_weno: flux(k) at face k+1/2 for k = 1..nz-1
+ * (the closures and the main loop of weno3_z below). */
+static void weno3_z_flux(const orc_grid *g, const orc_opts *o, const double *x, double x0,
+                         const double *w, double *flux, int weno5)
 {
     int nz = g->nz;
     const double g1 = 1.0 / 3.0, g2 = 2.0 / 3.0;
-    double *flux = (double *)calloc((size_t)nz + 1, sizeof(double));
     double lin[4] = {g1, g2, g2, g1};
 #define XZ(k) ((k) == 0 ? x0 : L(x, k))
     /* bottom closure, k = 1 (:232-241): no limiter, linear right weights */
@@ -188,6 +196,14 @@ static void weno3_z(const orc_grid *g, const orc_opts *o, const double *x, doubl
         L(flux, k) = 0.5 * fl * (www + fabs(www)) + 0.5 * fr * (www - fabs(www));
     }
 #undef XZ
+}
+
+static void weno3_z(const orc_grid *g, const orc_opts *o, const double *x, double x0,
+                    const double *w, double *xa, int adv, int weno5)
+{
+    int nz = g->nz;
+    double *flux = (double *)calloc((size_t)nz + 1, sizeof(double));
+    weno3_z_flux(g, o, x, x0, w, flux, weno5);
     double fold = L(flux, 1);
     for (int k = 2; k <= nz - 1; ++k) {
         L(xa, k) = L(xa, k) + (L(flux, k) - fold) * L(g->dz1w, k);
@@ -1036,6 +1052,139 @@ void orc_scalar_tendency(const orc_grid *g, const orc_opts *o, const double *q,
     if (dirmask & 1) orc_advection_dir(g, o, 0, q, u, xs, adv);
     if ((dirmask & 2) && g->ny > 1) orc_advection_dir(g, o, 1, q, v, xs, adv);
     if (dirmask & 4) orc_advection_dir(g, o, 2, q, w, xs, adv);
+}
+
+/* ======================================================================
+ * Positivity-preserving stage (no reference counterpart; parity pinned by
+ * tests/test_oracle_positive.py)
+ * ======================================================================
+ *
+ * One Shu-Osher stage out = c0*q0 + c1*(q + dt*L(q)) of the flux-form
+ * 'weno' tendency in which every cell's outgoing face fluxes are scaled by
+ * one factor theta so that the Euler step q + dt*L(q) cannot take the cell
+ * below zero:
+ *   O     = dt * sum over the cell's faces of the outgoing flux / width
+ *           (x: (max(F_i,0) + max(-F_{i-1},0)) dx1, then y, then z with
+ *           dz1w(k); the floor and lid faces carry no flux),
+ *   theta = O > max(q,0) ? (max(q,0) / O) * (1 - 2^-40) : 1
+ *           (the margin keeps the limited cell at +2^-40 q instead of an
+ *           ulp below zero after the rounding of the divergence),
+ * and a face flux F is multiplied by theta of its donor cell (left/lower
+ * if F > 0, right/upper if F < 0; across the periodic sides the donor is
+ * the periodic image).  Each flux enters both neighbours with one value,
+ * so mass is conserved; inflows are never scaled, so a non-negative q
+ * stays non-negative; with theta = 1 everywhere the stage equals the
+ * reference stage bit for bit (the divergence is formed in its order).
+ * Periodic sides, 'nnf' floor and lid, advective form off.  Returns 0, or
+ * -1 for an unsupported scheme or boundary. */
+int orc_positive_stage(const orc_grid *g, const orc_opts *o, const double *q, const double *u, const double *v,
+                       const double *w, const double *q0, double c0, double c1, double dt, double *out)
+{
+    int nx = g->nx, ny = g->ny, nz = g->nz;
+    int three_d = ny > 1;
+    if (o->scheme != ORC_WENO || g->bcl[4] != ORC_BC_NNF || g->bcl[5] != ORC_BC_NNF) return -1;
+    if (g->bcl[0] != ORC_BC_PER || g->bcl[1] != ORC_BC_PER) return -1;
+    if (three_d && (g->bcl[2] != ORC_BC_PER || g->bcl[3] != ORC_BC_PER)) return -1;
+    int mx = nx - 5, my = three_d ? ny - 5 : 1;
+    int jt0 = three_d ? 4 : 1, jt1 = three_d ? ny - 2 : 1;
+    size_t n3 = (size_t)nx * ny * nz;
+    double *fx = (double *)calloc(n3, sizeof(double));
+    double *fy = (double *)calloc(n3, sizeof(double));
+    double *fz = (double *)calloc(n3, sizeof(double));
+    double *th = (double *)calloc(n3, sizeof(double));
+    double dx1 = 1.0 / g->dx, dy1 = 1.0 / g->dy;
+    /* face fluxes: x faces i+1/2 (i = 3..nx-2) of the interior rows */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = jt0; j <= jt1; ++j) {
+            size_t b = IDX(g, 1, j, k);
+            weno3_h_flux(nx, q + b, u + b, fx + b);
+        }
+    /* y faces j+1/2 (j = 3..ny-2) of the interior columns */
+    if (three_d) {
+        double *lx = (double *)malloc(sizeof(double) * (size_t)ny * 3);
+        double *lv = lx + ny, *lf = lx + 2 * ny;
+        for (int k = 1; k <= nz; ++k)
+            for (int i = 4; i <= nx - 2; ++i) {
+                for (int j = 1; j <= ny; ++j) {
+                    L(lx, j) = q[IDX(g, i, j, k)];
+                    L(lv, j) = v[IDX(g, i, j, k)];
+                    L(lf, j) = 0.0;
+                }
+                weno3_h_flux(ny, lx, lv, lf);
+                for (int j = 1; j <= ny; ++j) fy[IDX(g, i, j, k)] = L(lf, j);
+            }
+        free(lx);
+    }
+    /* z faces k+1/2 (k = 1..nz-1), x(0) = x(1) */
+    {
+        double *lx = (double *)malloc(sizeof(double) * (size_t)nz * 3);
+        double *lw = lx + nz, *lf = lx + 2 * nz;
+        for (int j = jt0; j <= jt1; ++j)
+            for (int i = 4; i <= nx - 2; ++i) {
+                for (int k = 1; k <= nz; ++k) {
+                    L(lx, k) = q[IDX(g, i, j, k)];
+                    L(lw, k) = w[IDX(g, i, j, k)];
+                    L(lf, k) = 0.0;
+                }
+                weno3_z_flux(g, o, lx, L(lx, 1), lw, lf, 0);
+                for (int k = 1; k <= nz; ++k) fz[IDX(g, i, j, k)] = L(lf, k);
+            }
+        free(lx);
+    }
+#define POS(f) ((f) > 0.0 ? (f) : 0.0)
+#define NEG(f) ((f) < 0.0 ? -(f) : 0.0)
+    /* theta of every interior cell */
+    for (int k = 1; k <= nz; ++k)
+        for (int j = jt0; j <= jt1; ++j)
+            for (int i = 4; i <= nx - 2; ++i) {
+                size_t id = IDX(g, i, j, k);
+                double ox = (POS(fx[id]) + NEG(fx[IDX(g, i - 1, j, k)])) * dx1;
+                double o = ox;
+                if (three_d) o = o + (POS(fy[id]) + NEG(fy[IDX(g, i, j - 1, k)])) * dy1;
+                double fzt = k < nz ? fz[id] : 0.0, fzb = k > 1 ? fz[IDX(g, i, j, k - 1)] : 0.0;
+                o = o + (POS(fzt) + NEG(fzb)) * L(g->dz1w, k);
+                o = dt * o;
+                double qp = q[id] > 0.0 ? q[id] : 0.0;
+                th[id] = o > qp ? (qp / o) * (1.0 - 0x1p-40) : 1.0;
+            }
+#undef POS
+#undef NEG
+#define WX(i) ((i) < 4 ? (i) + mx : ((i) > nx - 2 ? (i) - mx : (i)))
+#define WY(j) (!three_d ? 1 : ((j) < 4 ? (j) + my : ((j) > ny - 2 ? (j) - my : (j))))
+    /* flux of x face f+1/2 of row (j, k), scaled by its donor's theta */
+#define FXS(f, j, k) (fx[IDX(g, f, j, k)] * (fx[IDX(g, f, j, k)] > 0.0 ? th[IDX(g, WX(f), j, k)] \
+                                                                       : th[IDX(g, WX((f) + 1), j, k)]))
+#define FYS(i, f, k) (fy[IDX(g, i, f, k)] * (fy[IDX(g, i, f, k)] > 0.0 ? th[IDX(g, i, WY(f), k)] \
+                                                                       : th[IDX(g, i, WY((f) + 1), k)]))
+#define FZS(i, j, f) (fz[IDX(g, i, j, f)] * (fz[IDX(g, i, j, f)] > 0.0 ? th[IDX(g, i, j, f)] : th[IDX(g, i, j, (f) + 1)]))
+    for (size_t n = 0; n < n3; ++n) out[n] = q[n];
+    for (int k = 1; k <= nz; ++k)
+        for (int j = jt0; j <= jt1; ++j)
+            for (int i = 4; i <= nx - 2; ++i) {
+                size_t id = IDX(g, i, j, k);
+                double X = (FXS(i, j, k) - FXS(i - 1, j, k)) * dx1;
+                double Y = three_d ? (FYS(i, j, k) - FYS(i, j - 1, k)) * dy1 : 0.0;
+                double Z;
+                if (k == 1) Z = (FZS(i, j, 1) - 0.0) * L(g->dz1w, 1);
+                else if (k == nz) Z = (0.0 - FZS(i, j, nz - 1)) * L(g->dz1w, nz);
+                else Z = (FZS(i, j, k) - FZS(i, j, k - 1)) * L(g->dz1w, k);
+                double A = 0.0;
+                A = A - X;
+                if (three_d) A = A - Y;
+                A = A - Z;
+                out[id] = c0 * q0[id] + c1 * (q[id] + dt * A);
+            }
+#undef FXS
+#undef FYS
+#undef FZS
+#undef WX
+#undef WY
+    orc_fill_scalar(g, out);
+    free(fx);
+    free(fy);
+    free(fz);
+    free(th);
+    return 0;
 }
 
 /* ======================================================================

@@ -97,6 +97,13 @@ void orc_scalar_tendency(const orc_grid *g, const orc_opts *o, const double *q,
                          const double *u, const double *v, const double *w,
                          double *xs, int adv, int dirmask);
 
+/* Positivity-preserving Shu-Osher stage of the flux-form 'weno' tendency
+ * (no reference counterpart): out = c0*q0 + c1*(q + dt*L~(q)) with each
+ * cell's outgoing fluxes scaled so that q + dt*L~(q) >= 0; periodic sides,
+ * 'nnf' floor and lid.  Returns 0, or -1 if unsupported. */
+int orc_positive_stage(const orc_grid *g, const orc_opts *o, const double *q, const double *u, const double *v,
+                       const double *w, const double *q0, double c0, double c1, double dt, double *out);
+
 /* qnadv_uvw: ua/va/wa are OVERWRITTEN with +div(rho0 u u) (windadv.f90:46-79).
  * mu/mv/mw: momentum (rho0 u); ut/vt/wt: advecting velocities. */
 void orc_qnadv_uvw(const orc_grid *g, const orc_opts *o, int ntime,

@@ -65,6 +65,8 @@ def load(path=None):
     lib.orc_advs_z.argtypes = [G, O, dp, d, dp, dp, i]
     lib.orc_advection_dir.argtypes = [G, O, i, dp, dp, dp, i]
     lib.orc_scalar_tendency.argtypes = [G, O, dp, dp, dp, dp, dp, i, i]
+    lib.orc_positive_stage.argtypes = [G, O, dp, dp, dp, dp, dp, d, d, d, dp]
+    lib.orc_positive_stage.restype = i
     lib.orc_qnadv_uvw.argtypes = [G, O, i, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.orc_fill_scalar.argtypes = [G, dp]
     lib.orc_fill_winds.argtypes = [G, dp, dp, dp]
@@ -131,6 +133,25 @@ class Oracle:
         self.o.dt0 = dt0
         self.lib.orc_scalar_tendency(C.byref(self.g), C.byref(self.o), _dp(q), _dp(u), _dp(v), _dp(w), _dp(xs),
                                      1 if adv else 0, int(dirmask))
+
+    def positive_stage(self, dt, u, v, w, q, q0, c0, c1):
+        """One Shu-Osher stage c0*q0 + c1*(q + dt*L(q)) with the positivity
+        scaling of the outgoing fluxes (flux-form 'weno'); returns the new
+        state."""
+        out = np.empty_like(q)
+        rc = self.lib.orc_positive_stage(C.byref(self.g), C.byref(self.o), _dp(q), _dp(u), _dp(v), _dp(w), _dp(q0),
+                                         float(c0), float(c1), float(dt), _dp(out))
+        if rc != 0:
+            raise ValueError("orc_positive_stage: unsupported scheme or boundary")
+        return out
+
+    def positive_ssprk3_step(self, dt, u, v, w, q):
+        """SSP-RK3 step of positivity stages (in place)."""
+        q0 = q.copy()
+        cur = q0
+        for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
+            cur = self.positive_stage(dt, u, v, w, cur, q0, c0, c1)
+        q[...] = cur
 
     def advection_dir(self, dt0, direction, q, vel, xs, adv=False):
         self.o.dt0 = dt0

@@ -14,6 +14,7 @@ import subprocess
 import pytest
 
 import test_oracle_known_answers as ka
+import test_oracle_positive as pos
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "mimadv_oracle.c")
@@ -22,6 +23,22 @@ CFLAGS = ["-O2", "-fPIC", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-f
 
 # (name, text in the oracle, planted replacement, check, check arguments)
 MUTANTS = [
+    # positivity-preserving stage
+    ("positive donor of a positive x flux", "? th[IDX(g, WX(f), j, k)]", "? th[IDX(g, WX((f) + 1), j, k)]",
+     pos.check_positive_single_cell, ()),
+    ("positive negative cells donate", "double qp = q[id] > 0.0 ? q[id] : 0.0;", "double qp = q[id];",
+     pos.check_positive_invariants, ()),
+    ("positive y outflow dropped", "if (three_d) o = o + (POS(fy[id])", "if (0) o = o + (POS(fy[id])",
+     pos.check_positive_invariants, ()),
+    ("positive z outflow metric", "o = o + (POS(fzt) + NEG(fzb)) * L(g->dz1w, k);",
+     "o = o + (POS(fzt) + NEG(fzb)) * L(g->dz1w, k < nz ? k + 1 : k);", pos.check_positive_invariants, ()),
+    ("positive divergence order",
+     "A = A - X;\n                if (three_d) A = A - Y;\n                A = A - Z;",
+     "A = A - Z;\n                A = A - X;\n                if (three_d) A = A - Y;", pos.check_positive_identity, ()),
+    ("positive theta without the outflow test", "th[id] = o > qp ? (qp / o) * (1.0 - 0x1p-40) : 1.0;",
+     "th[id] = (qp / o) * (1.0 - 0x1p-40);", pos.check_positive_identity, ()),
+    ("positive theta without the margin", "th[id] = o > qp ? (qp / o) * (1.0 - 0x1p-40) : 1.0;",
+     "th[id] = o > qp ? (qp / o) : 1.0;", pos.check_positive_single_cell, ()),
     # momentum artificial viscosity
     ("avisc self mu index", "double mu = fabs(L(ut, i)) * d1;", "double mu = fabs(L(ut, i + 1)) * d1;",
      ka.check_lavisc_site, ("ux",)),
