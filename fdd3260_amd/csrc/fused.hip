@@ -5,9 +5,9 @@
 // increments on large values) and the rest in FP32.  A work item is a
 // TX x TY tile of columns and a chunk of KC levels:
 //
-//   * the chunk's face velocities are read once into registers and serve
-//     every field of the batch (fields are the outer loop, levels the
-//     inner one, so register use does not grow with the field count);
+//   * the chunk's face velocities are staged in LDS once and serve every
+//     field of the batch (fields are the outer loop, levels the inner one,
+//     so register use does not grow with the field count);
 //   * per field and level the tile plus a 2-cell halo goes through LDS
 //     (the next level's tile is already in flight), the TY*(TX+1) x faces
 //     and (TY+1)*TX y faces of the tile are evaluated once each by a
@@ -23,9 +23,9 @@
 // limiter, the k = 3..nz-3 range of nonlinear vertical weights and the
 // bottom/top closures), MUSCL (advq_muscl.f90) and the WENO5-JS extension
 // (3-cell halo, in z where the upwind stencil fits the column) in an
-// operation-reduced form: the positivity rescale is applied to the weighted face value
-// (b + phi*(F - b), equal to rescaling each candidate since the weights
-// sum to one), only the upwind side's weights are evaluated, and divisions
+// operation-reduced form: limiter_weno's rescale is applied to the
+// weighted face value in a form that resolves its eps scale in FP32 (see
+// weno_face), only the upwind side's weights are evaluated, and divisions
 // are reciprocals.  It is checked against the FP64 oracle with tolerances,
 // not bit for bit.  Vertical boundaries must be no-flux ('nnf'), the
 // lateral ones periodic.
@@ -122,6 +122,20 @@ struct Slots {
     }
 };
 
+// Element at a 32-bit byte offset from a wave-uniform base pointer: the
+// global load/store takes the base from scalar registers and the offset
+// from one vector register, with no 64-bit vector address arithmetic.
+template <class T>
+__device__ __forceinline__ T load_at(const T *base, uint32_t off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void store_at(T *base, uint32_t off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 // One field marched up the item's levels.  Per level: the prefetched tile
 // goes to LDS, the next level's tile, face velocities, window value and
 // step-start value are requested, every thread evaluates its face slots
@@ -141,106 +155,112 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
     const size_t plane = (size_t)nx * ny;
     auto lev = [&](int kk) { return (size_t)(min(max(kk, 1), nz) - 1) * plane; };
     // A chunk above the floor first evaluates its bottom face k0-1/2 in a
-    // z-only pass of the level loop, so that every z face of the column is
-    // evaluated by the same code whatever the chunking.
+    // z-only pass of the level loop (kk = -1), so that every z face of the
+    // column is evaluated by the same code whatever the chunking.
     const bool zdir = (A.dirmask & 4) != 0;
     const int kstart = (k0 > 1 && zdir) ? -1 : 0;
     const int ks = k0 + kstart;
-    // window q(ks-H+1..ks+H), levels below 1 read as q(1)
+    // byte offsets in a level plane: this thread's column, its tile slots
+    const uint32_t colb = (uint32_t)col * (uint32_t)sizeof(T);
+    uint32_t tb[I::TS];
+#pragma unroll
+    for (int s = 0; s < I::TS; ++s) tb[s] = (uint32_t)sl.t_off[s] * (uint32_t)sizeof(T);
+    // window q(ks-H+1..ks+H), levels outside 1..nz clamped
     T zw[2 * H];
 #pragma unroll
-    for (int m = 0; m < 2 * H; ++m) zw[m] = q[col + lev(ks - H + 1 + m)];
-    const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
+    for (int m = 0; m < 2 * H; ++m) zw[m] = load_at(q + lev(ks - H + 1 + m), colb);
+    const float wz0 = load_at(A.w + lev(ks + 1), (uint32_t)col * 4u);
     T fzb = T(0);
     T tv[I::TS];
     T q0v = T(0), zn = T(0);
     double d1v = 0.0;
     const bool need_q0 = A.epi != 0 && A.c0 != 0.0;
-    // everything level k needs from memory, requested one level ahead
+    // everything level k needs from memory, requested one level ahead; zn
+    // = q(k+H) enters the window after level k-1
     auto fetch = [&](int k) {
         const size_t kb = lev(k);
         const T *qk = q + kb;
 #pragma unroll
         for (int s = 0; s < I::TS; ++s)
-            if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
-        if (need_q0 && valid) q0v = q0[col + kb];
-        zn = q[col + lev(k + H)];
+            if (sl.t_off[s] >= 0) tv[s] = load_at(qk, tb[s]);
+        if (need_q0 && valid) q0v = load_at(q0 + kb, colb);
+        zn = load_at(q + lev(k + H), colb);
         d1v = A.dz1w[k - 1];
     };
     fetch(k0);
 #pragma unroll 1
     for (int kk = kstart; kk < nk; ++kk) {
         const int k = k0 + kk;
-        // window q(k-H+1..k+H): the level's top value arrived one level ahead
-        if (kk > kstart) {
-#pragma unroll
-            for (int m = 0; m < 2 * H - 1; ++m) zw[m] = zw[m + 1];
-            zw[2 * H - 1] = zn;
-        }
-        const float *lvk = lv + (kk < 0 ? 0 : kk) * I::NV;  // this level's staged velocities
+        // window q(k-H+1..k+H); this level's staged velocities
+        const float *lvk = lv + (kk < 0 ? 0 : kk) * I::NV;
         const T fzt = zdir ? zface<SCHEME, T>(k, nz, zw, T(kk < 0 ? wz0 : lvk[I::NF + tid]), A) : T(0);
-        if (kk < 0) {
-            fzb = fzt;
-            continue;
-        }
+        if (kk >= 0) {
 #pragma unroll
-        for (int s = 0; s < I::TS; ++s)
-            if (sl.t_off[s] >= 0) lt[tid + s * I::NT] = tv[s];
-        const T q0c = q0v;
-        const double dz1w_k = d1v;
-        __syncthreads();
-        if (kk + 1 < nk) fetch(k + 1);
+            for (int s = 0; s < I::TS; ++s)
+                if (sl.t_off[s] >= 0) lt[tid + s * I::NT] = tv[s];
+            const T q0c = q0v;
+            const double dz1w_k = d1v;
+            __syncthreads();
+            if (kk + 1 < nk) fetch(k + 1);
 #pragma unroll
-        for (int s = 0; s < I::FS; ++s) {
-            const int st = sl.f_step[s];
-            if (st == 0) continue;
-            const int b = sl.f_base[s];
-            T sv[2 * H];
+            for (int s = 0; s < I::FS; ++s) {
+                const int st = sl.f_step[s];
+                if (st == 0) continue;
+                const int b = sl.f_base[s];
+                T sv[2 * H];
 #pragma unroll
-            for (int m = 0; m < 2 * H; ++m) sv[m] = lt[b + m * st];
-            lf[tid + s * I::NT] = hface<SCHEME, T>(sv, T(lvk[tid + s * I::NT]), A, st == 1 ? A.dx1 : A.dy1);
-        }
-        __syncthreads();
-        if (valid) {
-            const size_t id = col + (size_t)(k - 1) * plane;
-            double div = 0.0;
-            if (A.dirmask & 1) {
-                const int e = ty * (TX + 1) + tx;
-                div += (double(lf[e + 1]) - double(lf[e])) * A.dx1;
+                for (int m = 0; m < 2 * H; ++m) sv[m] = lt[b + m * st];
+                lf[tid + s * I::NT] = hface<SCHEME, T>(sv, T(lvk[tid + s * I::NT]), A, st == 1 ? A.dx1 : A.dy1);
             }
-            if (I::THREE_D && (A.dirmask & 2)) {
-                const int e = I::NXF + ty * TX + tx;
-                div += (double(lf[e + TX]) - double(lf[e])) * A.dy1;
-            }
-            if (A.dirmask & 4) div += (double(fzt) - double(fzb)) * dz1w_k;
-            const double a = -div;
-            if (A.epi == 0) {
-                out[id] = T(a);
-            } else {
-                double qn;
-                if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[H - 1]) + A.dt * a);
-                else qn = A.c1 * (double(zw[H - 1]) + A.dt * a);
-                const T qv = T(qn);
-                out[id] = qv;
-                if (A.fill) {
-                    const int i = i0 + tx, j = j0 + ty;
-                    int ix = 0, jy = 0;
-                    if (i >= nx - 4) ix = i - A.mx;
-                    else if (i <= 5) ix = i + A.mx;
-                    if (I::THREE_D) {
-                        if (j >= ny - 4) jy = j - A.my;
-                        else if (j <= 5) jy = j + A.my;
-                    }
-                    const size_t kb = (size_t)(k - 1) * plane;
-                    if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (j - 1)] = qv;
-                    if (jy) {
-                        out[kb + (size_t)(i - 1) + (size_t)nx * (jy - 1)] = qv;
-                        if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qv;
+            __syncthreads();
+            if (valid) {
+                // flux divergence in FP64: the differences of FP32 face
+                // values are exact there, and the three terms are summed
+                // with one rounding per term (a compensated FP32 sum)
+                double div = 0.0;
+                if (A.dirmask & 1) {
+                    const int e = ty * (TX + 1) + tx;
+                    div += (double(lf[e + 1]) - double(lf[e])) * A.dx1;
+                }
+                if (I::THREE_D && (A.dirmask & 2)) {
+                    const int e = I::NXF + ty * TX + tx;
+                    div += (double(lf[e + TX]) - double(lf[e])) * A.dy1;
+                }
+                if (A.dirmask & 4) div += (double(fzt) - double(fzb)) * dz1w_k;
+                const double a = -div;
+                T *outk = out + (size_t)(k - 1) * plane;
+                if (A.epi == 0) {
+                    store_at(outk, colb, T(a));
+                } else {
+                    // Shu-Osher stage in FP64, rounded once
+                    double qn;
+                    if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[H - 1]) + A.dt * a);
+                    else qn = A.c1 * (double(zw[H - 1]) + A.dt * a);
+                    const T qv = T(qn);
+                    store_at(outk, colb, qv);
+                    if (A.fill) {
+                        const int i = i0 + tx, j = j0 + ty;
+                        int ix = 0, jy = 0;
+                        if (i >= nx - 4) ix = i - A.mx;
+                        else if (i <= 5) ix = i + A.mx;
+                        if (I::THREE_D) {
+                            if (j >= ny - 4) jy = j - A.my;
+                            else if (j <= 5) jy = j + A.my;
+                        }
+                        const size_t kb = (size_t)(k - 1) * plane;
+                        if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (j - 1)] = qv;
+                        if (jy) {
+                            out[kb + (size_t)(i - 1) + (size_t)nx * (jy - 1)] = qv;
+                            if (ix) out[kb + (size_t)(ix - 1) + (size_t)nx * (jy - 1)] = qv;
+                        }
                     }
                 }
             }
         }
         fzb = fzt;
+#pragma unroll
+        for (int m = 0; m < 2 * H - 1; ++m) zw[m] = zw[m + 1];
+        zw[2 * H - 1] = zn;
     }
 }
 

@@ -180,7 +180,14 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
     const float *mz = g.zm.momz;
 
     if (const int wi = blockIdx.x; wi < nwork) {
-        const int tile = wi / nchunk, kc = wi - tile * nchunk;
+        // Work order: level chunk outermost, tiles x-fastest within it.  The
+        // hardware deals workgroups round-robin over the 8 XCDs; with
+        // nwork % 8 == 0 every XCD takes a contiguous range of that list,
+        // so a tile's x and y neighbours of the same chunk run on the same
+        // XCD at about the same time and its halo rows come from that L2.
+        const int ntile = nwork / nchunk;
+        const int w = (nwork % 8 == 0) ? (wi % 8) * (nwork / 8) + wi / 8 : wi;
+        const int kc = w / ntile, tile = w - kc * ntile;
         const int i0 = 4 + (tile % ntx) * TX;
         const int j0 = THREE_D ? 4 + (tile / ntx) * TY : 1;
         const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);

@@ -8,9 +8,8 @@
 // every subdomain, reads that subdomain's argument block from a device
 // table, and runs the same per-field column march as the single-grid
 // batched kernel (fused.hip): tile plus a 2-cell (WENO5: 3-cell) halo in
-// LDS, flattened x/y
-// face list, z face carried up the column, FP64 divergence and stage
-// update rounded once to FP32.
+// LDS, flattened x/y face list, z face carried up the column, FP64
+// divergence and stage update rounded once to FP32.
 //
 // The tile shape is chosen per subdomain (TX x TY with TX*TY = 256:
 // 16x16, 32x8, 64x4) so that small or narrow subdomains waste fewer lanes.
