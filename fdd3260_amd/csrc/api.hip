@@ -1210,7 +1210,8 @@ namespace {
 
 int check_adjoint(mimadv_ctx *c, const double *u, const double *v, const double *w)
 {
-    if (c->sp.scheme != MIMADV_WENO) return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint implements the WENO scheme");
+    if (c->sp.scheme != MIMADV_WENO && c->sp.scheme != MIMADV_WENO5)
+        return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint implements the WENO and WENO5 schemes");
     if (c->sp.weno_bottom != MIMADV_WENO_BOTTOM_X0_IS_X1)
         return fail(c, MIMADV_ERR_UNSUPPORTED, "the adjoint implements the x(0) = x(1) floor convention");
     for (int b = 0; b < 4; ++b)
@@ -1239,6 +1240,7 @@ AdjArgs adj_base(mimadv_ctx *c, const double *u, const double *v, const double *
     A.vb = c->g.ny > 1 ? vb : ub;
     A.wb = wb;
     A.kchunk = c->tuning.kchunk;
+    A.weno5 = c->sp.scheme == MIMADV_WENO5 ? 1 : 0;
     return A;
 }
 

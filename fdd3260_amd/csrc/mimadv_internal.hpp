@@ -57,6 +57,7 @@ struct AdjArgs {
     int accumulate;  // out += instead of out =
     int acc0_mode;   // 0: untouched, 1: acc0 = c0*qbn, 2: acc0 += c0*qbn
     int kchunk;      // levels per workgroup (0: default)
+    int weno5;       // the WENO5-JS extension instead of the reference 'weno'
     int ntx, nty, nwork;  // set by the launcher
 };
 

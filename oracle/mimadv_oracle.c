@@ -1879,15 +1879,52 @@ static dual tl_face(dual a, dual b, dual c, dual d, dual u, int kind, int nonlin
     return dadd(dmul(dscale(0.5, fl), dadd(u, au)), dmul(dscale(0.5, fr), dsub(u, au)));
 }
 
-void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
-                           const double *v, const double *w, const double *du, const double *dv,
-                           const double *dw, double *xs, double *dxs)
+/* weno5_left in dual numbers, same operation order for the value (the
+ * WENO5-JS extension: no reference counterpart). */
+static dual tl_weno5_left(dual a, dual b, dual c, dual d, dual e)
+{
+    const double eps = 1e-6;
+    dual q0 = dscale(1.0 / 6.0, dadd(dsub(dscale(2.0, a), dscale(7.0, b)), dscale(11.0, c)));
+    dual q1 = dscale(1.0 / 6.0, dadd(dadd(dscale(-1.0, b), dscale(5.0, c)), dscale(2.0, d)));
+    dual q2 = dscale(1.0 / 6.0, dsub(dadd(dscale(2.0, c), dscale(5.0, d)), e));
+    dual s0 = dadd(dsub(a, dscale(2.0, b)), c), t0 = dadd(dsub(a, dscale(4.0, b)), dscale(3.0, c));
+    dual s1 = dadd(dsub(b, dscale(2.0, c)), d), t1 = dsub(b, d);
+    dual s2 = dadd(dsub(c, dscale(2.0, d)), e), t2 = dadd(dsub(dscale(3.0, c), dscale(4.0, d)), e);
+    dual b0 = dadd(dscale(13.0 / 12.0, dsq(s0)), dscale(0.25, dsq(t0)));
+    dual b1 = dadd(dscale(13.0 / 12.0, dsq(s1)), dscale(0.25, dsq(t1)));
+    dual b2 = dadd(dscale(13.0 / 12.0, dsq(s2)), dscale(0.25, dsq(t2)));
+    dual epsd = dd_(eps, 0.0);
+    dual a0 = ddiv(dd_(0.1, 0.0), dsq(dadd(epsd, b0)));
+    dual a1 = ddiv(dd_(0.6, 0.0), dsq(dadd(epsd, b1)));
+    dual a2 = ddiv(dd_(0.3, 0.0), dsq(dadd(epsd, b2)));
+    return ddiv(dadd(dadd(dmul(a0, q0), dmul(a1, q1)), dmul(a2, q2)), dadd(dadd(a0, a1), a2));
+}
+
+/* WENO5 face flux 0.5 fl (u + |u|) + 0.5 fr (u - |u|), d|u| = du at u = 0 */
+static dual tl_weno5_face(dual a, dual b, dual c, dual d, dual e, dual f, dual u)
+{
+    dual fl = tl_weno5_left(a, b, c, d, e);
+    dual fr = tl_weno5_left(f, e, d, c, b);
+    dual au = u.v < 0.0 ? dd_(-u.v, -u.d) : u;
+    return dadd(dmul(dscale(0.5, fl), dadd(u, au)), dmul(dscale(0.5, fr), dsub(u, au)));
+}
+
+/* Tangent-linear tendency of 'weno' (weno5 = 0) or of the WENO5-JS
+ * extension (weno5 = 1: x/y faces WENO5 with the periodic image x(6) for
+ * x(n+1), z faces WENO5 where the upwind stencil lies in the column and
+ * the reference face elsewhere, as weno5_h / weno3_z(weno5 = 1)). */
+static void tlm_tendency(const orc_grid *g, int weno5, const double *q, const double *dq, const double *u,
+                         const double *v, const double *w, const double *du, const double *dv, const double *dw,
+                         double *xs, double *dxs)
 {
     int nx = g->nx, ny = g->ny, nz = g->nz;
     int three_d = ny > 1;
     int jt0 = three_d ? 4 : 1, jt1 = three_d ? ny - 2 : 1;
     double dx1 = 1.0 / g->dx, dy1 = 1.0 / g->dy;
 #define DQ(i, j, k) dd_(q[IDX(g, i, j, k)], dq[IDX(g, i, j, k)])
+    /* x(n+1) of the last face for negative velocity: its periodic image x(6) */
+#define PX(i) ((i) == nx + 1 ? 6 : (i))
+#define PY(j) ((j) == ny + 1 ? 6 : (j))
 #pragma omp parallel for collapse(2) num_threads(g_threads) schedule(static)
     for (int k = 1; k <= nz; ++k)
         for (int j = jt0; j <= jt1; ++j)
@@ -1895,14 +1932,28 @@ void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq,
                 size_t id = IDX(g, i, j, k);
                 dual t = dd_(0.0, 0.0);
                 /* x */
-                {
+                if (weno5) {
+                    dual fp = tl_weno5_face(DQ(i - 2, j, k), DQ(i - 1, j, k), DQ(i, j, k), DQ(i + 1, j, k),
+                                            DQ(i + 2, j, k), DQ(PX(i + 3), j, k),
+                                            dd_(u[IDX(g, i + 1, j, k)], du[IDX(g, i + 1, j, k)]));
+                    dual fm = tl_weno5_face(DQ(i - 3, j, k), DQ(i - 2, j, k), DQ(i - 1, j, k), DQ(i, j, k),
+                                            DQ(i + 1, j, k), DQ(i + 2, j, k), dd_(u[id], du[id]));
+                    t = dsub(t, dscale(dx1, dsub(fp, fm)));
+                } else {
                     dual fp = tl_face(DQ(i - 1, j, k), DQ(i, j, k), DQ(i + 1, j, k), DQ(i + 2, j, k),
                                       dd_(u[IDX(g, i + 1, j, k)], du[IDX(g, i + 1, j, k)]), 0, 1);
                     dual fm = tl_face(DQ(i - 2, j, k), DQ(i - 1, j, k), DQ(i, j, k), DQ(i + 1, j, k),
                                       dd_(u[id], du[id]), 0, 1);
                     t = dsub(t, dscale(dx1, dsub(fp, fm)));
                 }
-                if (three_d) {
+                if (three_d && weno5) {
+                    dual fp = tl_weno5_face(DQ(i, j - 2, k), DQ(i, j - 1, k), DQ(i, j, k), DQ(i, j + 1, k),
+                                            DQ(i, j + 2, k), DQ(i, PY(j + 3), k),
+                                            dd_(v[IDX(g, i, j + 1, k)], dv[IDX(g, i, j + 1, k)]));
+                    dual fm = tl_weno5_face(DQ(i, j - 3, k), DQ(i, j - 2, k), DQ(i, j - 1, k), DQ(i, j, k),
+                                            DQ(i, j + 1, k), DQ(i, j + 2, k), dd_(v[id], dv[id]));
+                    t = dsub(t, dscale(dy1, dsub(fp, fm)));
+                } else if (three_d) {
                     dual fp = tl_face(DQ(i, j - 1, k), DQ(i, j, k), DQ(i, j + 1, k), DQ(i, j + 2, k),
                                       dd_(v[IDX(g, i, j + 1, k)], dv[IDX(g, i, j + 1, k)]), 0, 1);
                     dual fm = tl_face(DQ(i, j - 2, k), DQ(i, j - 1, k), DQ(i, j, k), DQ(i, j + 1, k),
@@ -1916,7 +1967,15 @@ void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq,
                         int f = k - 1 + s; /* face f+1/2 */
                         if (f < 1 || f > nz - 1) { fz[s] = dd_(0.0, 0.0); continue; }
                         dual wf = dd_(w[IDX(g, i, j, f + 1)], dw[IDX(g, i, j, f + 1)]);
-                        if (f == nz - 1) {
+                        if (weno5 && wf.v > 0.0 && f >= 3 && f <= nz - 2) {
+                            dual fl = tl_weno5_left(DQ(i, j, f - 2), DQ(i, j, f - 1), DQ(i, j, f), DQ(i, j, f + 1),
+                                                    DQ(i, j, f + 2));
+                            fz[s] = dmul(dscale(0.5, fl), dadd(wf, wf));
+                        } else if (weno5 && wf.v < 0.0 && f >= 2 && f <= nz - 3) {
+                            dual fr = tl_weno5_left(DQ(i, j, f + 3), DQ(i, j, f + 2), DQ(i, j, f + 1), DQ(i, j, f),
+                                                    DQ(i, j, f - 1));
+                            fz[s] = dmul(dscale(0.5, fr), dsub(wf, dd_(-wf.v, -wf.d)));
+                        } else if (f == nz - 1) {
                             fz[s] = tl_face(DQ(i, j, f - 1), DQ(i, j, f), DQ(i, j, f + 1), dd_(0, 0), wf, 2, 0);
                         } else {
                             dual a = f == 1 ? DQ(i, j, 1) : DQ(i, j, f - 1);
@@ -1930,6 +1989,22 @@ void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq,
                 dxs[id] = t.d;
             }
 #undef DQ
+#undef PX
+#undef PY
+}
+
+void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
+                           const double *v, const double *w, const double *du, const double *dv,
+                           const double *dw, double *xs, double *dxs)
+{
+    tlm_tendency(g, 0, q, dq, u, v, w, du, dv, dw, xs, dxs);
+}
+
+void orc_tlm_weno5_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
+                            const double *v, const double *w, const double *du, const double *dv,
+                            const double *dw, double *xs, double *dxs)
+{
+    tlm_tendency(g, 1, q, dq, u, v, w, du, dv, dw, xs, dxs);
 }
 
 /* ======================================================================

@@ -138,6 +138,11 @@ double orc_totalav(const orc_grid *g, const double *x);
 void orc_tlm_weno_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
                            const double *v, const double *w, const double *du, const double *dv,
                            const double *dw, double *xs, double *dxs);
+/* The same for the WENO5-JS extension (x/y WENO5 faces, z WENO5 where the
+ * upwind stencil fits the column, the reference face elsewhere). */
+void orc_tlm_weno5_tendency(const orc_grid *g, const double *q, const double *dq, const double *u,
+                            const double *v, const double *w, const double *du, const double *dv,
+                            const double *dw, double *xs, double *dxs);
 
 /* ---- density weighting of the anelastic build (funcpack.f90:65-542) ---- */
 /* avden0(1) = den0(1); avden0(k) = (fdz0(k)den0(k) + fdz0(k-1)den0(k-1)) /

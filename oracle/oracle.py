@@ -78,6 +78,7 @@ def load(path=None):
     lib.orc_totalav.argtypes = [G, dp]
     lib.orc_totalav.restype = d
     lib.orc_tlm_weno_tendency.argtypes = [G, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
+    lib.orc_tlm_weno5_tendency.argtypes = [G, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp]
     lib.orc_avden0.argtypes = [i, dp, dp, dp]
     lib.orc_get_momentum.argtypes = [G, dp, dp, dp, dp, dp, i]
     lib.orc_get_advective.argtypes = [G, dp, dp, d, d, dp, dp, dp, dp, dp, dp]
@@ -202,6 +203,13 @@ class Oracle:
         xs, dxs = np.zeros_like(q), np.zeros_like(q)
         self.lib.orc_tlm_weno_tendency(C.byref(self.g), _dp(q), _dp(dq), _dp(u), _dp(v), _dp(w), _dp(du), _dp(dv),
                                        _dp(dw), _dp(xs), _dp(dxs))
+        return xs, dxs
+
+    def tlm_weno5_tendency(self, q, dq, u, v, w, du, dv, dw):
+        """Tangent-linear of the WENO5-JS flux-form tendency: (xs, dxs)."""
+        xs, dxs = np.zeros_like(q), np.zeros_like(q)
+        self.lib.orc_tlm_weno5_tendency(C.byref(self.g), _dp(q), _dp(dq), _dp(u), _dp(v), _dp(w), _dp(du),
+                                        _dp(dv), _dp(dw), _dp(xs), _dp(dxs))
         return xs, dxs
 
     def avden0(self, den0):

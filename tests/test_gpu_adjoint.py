@@ -1,4 +1,4 @@
-"""The hand-written adjoint of the plain FP64 'weno' path.
+"""The hand-written adjoint of the plain FP64 'weno' and WENO5 paths.
 
 * dot-product tests against the oracle's tangent-linear model (forward-mode
   derivative of the reference scheme, same one-sided conventions at the
@@ -60,16 +60,21 @@ def _check_dot(lhs_terms, rhs_terms, tol=1e-13):
     assert abs(lhs - rhs) <= tol * scale, (lhs, rhs, scale)
 
 
+def _tlm(ora, scheme):
+    return ora.tlm_weno5_tendency if scheme == "weno5" else ora.tlm_weno_tendency
+
+
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
 @pytest.mark.parametrize("three_d", [True, False])
 @pytest.mark.parametrize("kind", ["lognormal", "gaussian"])
-def test_adjoint_tendency_dot_product(cuda, three_d, kind):
-    print(f"case adjoint_tendency three_d={three_d} kind={kind}", flush=True)
+def test_adjoint_tendency_dot_product(cuda, three_d, kind, scheme):
+    print(f"case adjoint_tendency {scheme} three_d={three_d} kind={kind}", flush=True)
     g, u, v, w, q = _case(three_d, kind)
     rng = np.random.default_rng(71)
     dq, du, dv, dw = _dirs(g, rng, np.abs(q).mean())
     lam = _interior_only(g, rng.standard_normal(g.shape))
-    _, tl = Oracle(g, scal_adv="weno").tlm_weno_tendency(q, dq, u, v, w, du, dv, dw)
-    with Advection(g, scal_adv="weno") as a:
+    _, tl = _tlm(Oracle(g, scal_adv=scheme), scheme)(q, dq, u, v, w, du, dv, dw)
+    with Advection(g, scal_adv=scheme) as a:
         U, V, W, Q, L = _dev(cuda, u, v, w, q, lam)
         qb, ub, vb, wb = (torch.zeros_like(Q) for _ in range(4))
         a.adjoint_tendency(U, V, W, Q, L, qb, ub, vb, wb)
@@ -86,14 +91,15 @@ def _mask(g):
     return m
 
 
-def _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq0, du, dv, dw):
+def _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq0, du, dv, dw, scheme="weno"):
     """Oracle state and tangent after nsteps, stage states ghost-filled."""
+    tlm = _tlm(ora, scheme)
     q, dq = q0.copy(), dq0.copy()
     for _ in range(nsteps):
         if integrator == "ssprk3":
             cur, dcur = q, dq
             for c0, c1 in ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0)):
-                xs, dxs = ora.tlm_weno_tendency(cur, dcur, u, v, w, du, dv, dw)
+                xs, dxs = tlm(cur, dcur, u, v, w, du, dv, dw)
                 nxt = c0 * q + c1 * (cur + dt * xs)
                 dnxt = c0 * dq + c1 * (dcur + dt * dxs)
                 ora.fill_scalar(nxt)
@@ -102,11 +108,11 @@ def _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq0, du, dv, dw
             q, dq = cur, dcur
         else:
             # integ.h form: q1 = q + dt a1; q+ = q + dt/2 (a1 + a2)
-            a1, da1 = ora.tlm_weno_tendency(q, dq, u, v, w, du, dv, dw)
+            a1, da1 = tlm(q, dq, u, v, w, du, dv, dw)
             q1, dq1 = q + dt * a1, dq + dt * da1
             ora.fill_scalar(q1)
             ora.fill_scalar(dq1)
-            a2, da2 = ora.tlm_weno_tendency(q1, dq1, u, v, w, du, dv, dw)
+            a2, da2 = tlm(q1, dq1, u, v, w, du, dv, dw)
             qn, dqn = q + 0.5 * dt * (a1 + a2), dq + 0.5 * dt * (da1 + da2)
             ora.fill_scalar(qn)
             ora.fill_scalar(dqn)
@@ -114,19 +120,20 @@ def _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq0, du, dv, dw
     return q, dq
 
 
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
 @pytest.mark.parametrize("integrator", ["ssprk3", "heun"])
 @pytest.mark.parametrize("checkpoint", [0, 2])
-def test_adjoint_advance_dot_product(cuda, integrator, checkpoint):
-    print(f"case adjoint_advance {integrator} checkpoint={checkpoint}", flush=True)
+def test_adjoint_advance_dot_product(cuda, integrator, checkpoint, scheme):
+    print(f"case adjoint_advance {scheme} {integrator} checkpoint={checkpoint}", flush=True)
     g, u, v, w, q0 = _case(True, "lognormal", seed=81)
     dt = cases.cfl_dt(g, u, v, w, 0.5)
     nsteps = 5
     rng = np.random.default_rng(83)
     dq, du, dv, dw = _dirs(g, rng, np.abs(q0).mean())
     lam = _interior_only(g, rng.standard_normal(g.shape))
-    ora = Oracle(g, scal_adv="weno")
-    qT, dqT = _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq, du, dv, dw)
-    with Advection(g, scal_adv="weno") as a:
+    ora = Oracle(g, scal_adv=scheme)
+    qT, dqT = _tlm_trajectory(ora, g, integrator, nsteps, dt, u, v, w, q0, dq, du, dv, dw, scheme)
+    with Advection(g, scal_adv=scheme) as a:
         U, V, W, Q0, L = _dev(cuda, u, v, w, q0, lam)
         # the forward itself agrees with the oracle trajectory
         Q = Q0.clone()

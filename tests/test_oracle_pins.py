@@ -892,23 +892,29 @@ def _perturb(g, rng, scale):
     return cases.fill_periodic(g, d)
 
 
-def test_tlm_value_is_the_forward_tendency():
+def _tlm(ora, scheme):
+    return ora.tlm_weno5_tendency if scheme == "weno5" else ora.tlm_weno_tendency
+
+
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
+def test_tlm_value_is_the_forward_tendency(scheme):
     g, u, v, w, q = case3d(seed=51)
-    ora = Oracle(g, scal_adv="weno")
+    ora = Oracle(g, scal_adv=scheme)
     ref = np.zeros(g.shape)
     ora.scalar_tendency(0.0, u, v, w, q, ref)
     z = np.zeros(g.shape)
-    xs, _ = ora.tlm_weno_tendency(q, z, u, v, w, z, z, z)
+    xs, _ = _tlm(ora, scheme)(q, z, u, v, w, z, z, z)
     I = g.interior()
     assert np.abs(xs[I] - ref[I]).max() <= 1e-13 * np.abs(ref[I]).max()
 
 
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
 @pytest.mark.parametrize("wrt", ["q", "uvw"])
-def test_tlm_matches_central_differences(wrt):
+def test_tlm_matches_central_differences(wrt, scheme):
     """Directional derivative of the oracle tendency by central differences
     (step 1e-6 of the data scale) against the tangent-linear model."""
     g, u, v, w, q = case3d(seed=52)
-    ora = Oracle(g, scal_adv="weno")
+    ora = Oracle(g, scal_adv=scheme)
     rng = np.random.default_rng(53)
     z = np.zeros(g.shape)
     if wrt == "q":
@@ -917,7 +923,7 @@ def test_tlm_matches_central_differences(wrt):
         dq = z
         du, dv, dw = (_perturb(g, rng, 1.0) for _ in range(3))
         dw[0] = 0.0
-    _, tl = ora.tlm_weno_tendency(q, dq, u, v, w, du, dv, dw)
+    _, tl = _tlm(ora, scheme)(q, dq, u, v, w, du, dv, dw)
     h = 1e-6
 
     def T(s):
