@@ -171,7 +171,7 @@ def fused(args):
             for h, p in zip(host, prec)]
     fields = [f.clone() for f in init]
     log(f"fused case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
-    scheme = args.scheme if args.scheme in ("weno", "muscl") else "weno"
+    scheme = args.scheme if args.scheme in ("weno", "muscl", "weno5") else "weno"
     adv = Advection(g, scal_adv=scheme, muscllimiter=args.limiter)
     adv.set_tuning(group32=args.group32, kchunk=args.kchunk, tile_y=args.tile_y)
     code = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
@@ -222,8 +222,9 @@ def fused(args):
     dom = max(per_kind, key=lambda k: per_kind[k]["avg_launch_us"])
     stage_us = sum(v["avg_launch_us"] for v in per_kind.values())
     stage_bytes_all = sum(kind_bytes(idx) for idx in groups.values())
-    kernel_names = {"f32": "fused_kernel<0,64,4,float> (15 FP32 species)",
-                    "f64": "fused_kernel<0,64,8,double> (theta, qv)",
+    sid = SCHEME_ID[scheme]
+    kernel_names = {"f32": f"fused_kernel<{sid},64,4,float> (15 FP32 species)",
+                    "f64": f"fused_kernel<{sid},64,8,double> (theta, qv)",
                     "momentum": "momentum_batch_kernel<32,8> (3 FP32 momenta)"}
     cpu = None
     cpu_steps = args.cpu_steps if args.cpu_steps is not None else 1
@@ -302,7 +303,8 @@ def adjoint(args):
     mask[I] = 1.0
     TOP = torch.from_numpy(top * mask).to(dev)
     log(f"adjoint case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
-    adv = Advection(g, scal_adv="weno")
+    scheme = args.scheme if args.scheme in ("weno", "weno5") else "weno"
+    adv = Advection(g, scal_adv=scheme)
     adv.set_stream(torch.cuda.current_stream(dev).cuda_stream)
     code, ck = _lib.INT_SSPRK3, args.checkpoint
     nsteps = args.steps if args.steps is not None else 200
@@ -404,16 +406,17 @@ def adjoint(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded turbulent wind, log-normal scalar)",
-        "config": {"workload": f"trajectory adjoint {maxx}x{max(maxy, 1)}x{nz}, weno, ssprk3, {nsteps} steps, "
+        "config": {"workload": f"trajectory adjoint {maxx}x{max(maxy, 1)}x{nz}, {scheme}, ssprk3, {nsteps} steps, "
                                f"checkpoint every {ck or nsteps} steps, J = sum(top half rho0 q(T)^2), "
                                f"gradient wrt (q0, u, v, w)",
                    "cells": ncell, "forward_s": t_fwd, "adjoint_s": t_adj, "dt_s": dt,
                    "forward_recording_s": t_rec, "adjoint_self_contained_s": t_adj_self,
                    "self_contained_ratio": t_adj_self / t_fwd,
                    "device_arrays": 3 + nseg + 3 * kk},
-        "roofline": {"bound": "hbm", "kernel": "adjoint_kernel<64,8> (one stage adjoint)", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": HBM_TRAFFIC.get(("adjoint", size, "weno", "ssprk3", 1)),
+        "roofline": {"bound": "hbm", "kernel": ("adjoint_kernel<32,8,weno5>" if scheme == "weno5" else
+                                                "adjoint_kernel<64,8>") + " (one stage adjoint)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": HBM_TRAFFIC.get(("adjoint", size, scheme, "ssprk3", 1)),
                      "alg_bytes_per_launch": adj_bytes, "avg_launch_us": adj_kernel_s * 1e6},
         "gradient_norms": {"q0_lognormal": q_grad_norm, "q0_theta": theta_grad_norm},
         "cpu_baseline": None,
@@ -665,7 +668,7 @@ def plain(args):
             del vq
 
     cpu = None
-    cpu_steps = cpu_steps if cpu_steps is not None else 6
+    cpu_steps = args.cpu_steps if args.cpu_steps is not None else 6
     if cpu_steps > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {cpu_steps} steps with {threads} threads")

@@ -313,28 +313,54 @@ fused_kernel(FusedArgs A)
         const int col = (il - 1) + nx * (jl - 1);
         Slots<TX, TY, H> sl;
         sl.init(tid, i0, j0, A);
-        // the chunk's face velocities and column w(k+1), read once for every field
+        // the chunk's face velocities and column w(k+1), read once for every
+        // field: the source array and plane offset of each of this thread's
+        // staging slots are fixed for the item, and all the chunk's loads are
+        // in flight together (issued before the barrier that waits for the
+        // previous item's fields) before they go to LDS
         {
+            constexpr int VS = (I::NV + I::NT - 1) / I::NT;
             const size_t plane = (size_t)nx * ny;
-            __syncthreads();  // the previous item's fields are done with vel_lds
-            for (int kk = 0; kk < nk; ++kk) {
-                const int k = k0 + kk;
-                const size_t kb = (size_t)(k - 1) * plane;
-                for (int e = tid; e < I::NV; e += I::NT) {
-                    float vel = 0.0f;
-                    if (e < I::NF) {
-                        int base, step;
-                        size_t vcol;
-                        const int kind = face_slot_geom<TX, TY, H>(e, i0, j0, A, base, step, vcol);
-                        vel = kind == 1 ? A.u[kb + vcol] : A.v[kb + vcol];
-                    } else if (k < nz) {
-                        const int c = e - I::NF, cx = c % TX, cy = c / TX;
-                        const int ci = wrap(i0 + cx, nx, A.mx), cj = I::THREE_D ? wrap(j0 + cy, ny, A.my) : 1;
-                        vel = A.w[kb + plane + (size_t)(ci - 1) + (size_t)nx * (cj - 1)];
-                    }
-                    vel_lds[kk * I::NV + e] = vel;
+            const float *vp[VS];
+            int voff[VS];
+            bool vw[VS];
+#pragma unroll
+            for (int s = 0; s < VS; ++s) {
+                const int e = tid + s * I::NT;
+                vp[s] = A.u;
+                voff[s] = -1;
+                vw[s] = false;
+                if (e < I::NF) {
+                    int base, step;
+                    size_t vcol;
+                    const int kind = face_slot_geom<TX, TY, H>(e, i0, j0, A, base, step, vcol);
+                    vp[s] = kind == 1 ? A.u : A.v;
+                    voff[s] = (int)vcol;
+                } else if (e < I::NV) {
+                    const int c = e - I::NF, cx = c % TX, cy = c / TX;
+                    const int ci = wrap(i0 + cx, nx, A.mx), cj = I::THREE_D ? wrap(j0 + cy, ny, A.my) : 1;
+                    vp[s] = A.w;  // w(k+1) at level k
+                    voff[s] = (ci - 1) + nx * (cj - 1);
+                    vw[s] = true;
                 }
             }
+            float vb[FUSED_KC][VS];
+#pragma unroll
+            for (int kk = 0; kk < FUSED_KC; ++kk) {
+                const int k = k0 + kk;
+#pragma unroll
+                for (int s = 0; s < VS; ++s) {
+                    vb[kk][s] = 0.0f;
+                    if (kk < nk && voff[s] >= 0 && (!vw[s] || k < nz))
+                        vb[kk][s] = vp[s][(size_t)(vw[s] ? k : k - 1) * plane + (size_t)voff[s]];
+                }
+            }
+            __syncthreads();  // the previous item's fields are done with vel_lds
+#pragma unroll
+            for (int kk = 0; kk < FUSED_KC; ++kk)
+#pragma unroll
+                for (int s = 0; s < VS; ++s)
+                    if (kk < nk && tid + s * I::NT < I::NV) vel_lds[kk * I::NV + tid + s * I::NT] = vb[kk][s];
             __syncthreads();
         }
         if constexpr (sizeof(T) == 8) {

@@ -214,32 +214,55 @@ __global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__
         const int col = (wrap(i, A.nx, A.mx) - 1) + A.nx * (wrap(j, A.ny, A.my) - 1);
         SlotsR sl;
         sl.init(G, tid, i0, j0, A);
-        // the chunk's face velocities and column w(k+1), read once for every field
+        // the chunk's face velocities and column w(k+1), read once for every
+        // field: each staging slot's source array and plane offset are fixed
+        // for the item, and all the chunk's loads are in flight together
+        // (issued before the barrier that waits for the previous item's
+        // fields) before they go to LDS
         {
+            constexpr int VS = (NV_MAX + NT - 1) / NT;
             const int nx = A.nx, ny = A.ny;
             const size_t plane = (size_t)nx * ny;
-            __syncthreads();  // the previous item's fields are done with vel_lds
-            for (int kk = 0; kk < nk; ++kk) {
-                const int k = k0 + kk;
-                const size_t kb = (size_t)(k - 1) * plane;
-                for (int e = tid; e < G.NV; e += NT) {
-                    float vel = 0.0f;
-                    if (e < G.NXF) {
-                        const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);
-                        vel = A.u[kb + (size_t)(wrap(i0 + fl, nx, A.mx) - 1) +
-                                  (size_t)nx * (wrap(j0 + r, ny, A.my) - 1)];
-                    } else if (e < G.NF) {
-                        const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;
-                        vel = A.v[kb + (size_t)(wrap(i0 + c, nx, A.mx) - 1) +
-                                  (size_t)nx * (wrap(j0 + fr, ny, A.my) - 1)];
-                    } else if (k < A.nz) {
-                        const int c = e - G.NF, cx = c % G.TX, cy = c / G.TX;
-                        vel = A.w[kb + plane + (size_t)(wrap(i0 + cx, nx, A.mx) - 1) +
-                                  (size_t)nx * (wrap(j0 + cy, ny, A.my) - 1)];
-                    }
-                    vel_lds[kk * G.NV + e] = vel;
+            const float *vp[VS];
+            int voff[VS];
+            bool vw[VS];
+#pragma unroll
+            for (int s = 0; s < VS; ++s) {
+                const int e = tid + s * NT;
+                vp[s] = A.u;
+                voff[s] = -1;
+                vw[s] = false;
+                if (e < G.NXF) {
+                    const int r = e / (G.TX + 1), fl = e - r * (G.TX + 1);
+                    voff[s] = (wrap(i0 + fl, nx, A.mx) - 1) + nx * (wrap(j0 + r, ny, A.my) - 1);
+                } else if (e < G.NF) {
+                    const int e2 = e - G.NXF, fr = e2 / G.TX, c = e2 - fr * G.TX;
+                    vp[s] = A.v;
+                    voff[s] = (wrap(i0 + c, nx, A.mx) - 1) + nx * (wrap(j0 + fr, ny, A.my) - 1);
+                } else if (e < G.NV) {
+                    const int c = e - G.NF, cx = c % G.TX, cy = c / G.TX;
+                    vp[s] = A.w;  // w(k+1) at level k
+                    voff[s] = (wrap(i0 + cx, nx, A.mx) - 1) + nx * (wrap(j0 + cy, ny, A.my) - 1);
+                    vw[s] = true;
                 }
             }
+            float vb[KC_MAX][VS];
+#pragma unroll
+            for (int kk = 0; kk < KC_MAX; ++kk) {
+                const int k = k0 + kk;
+#pragma unroll
+                for (int s = 0; s < VS; ++s) {
+                    vb[kk][s] = 0.0f;
+                    if (kk < nk && voff[s] >= 0 && (!vw[s] || k < A.nz))
+                        vb[kk][s] = vp[s][(size_t)(vw[s] ? k : k - 1) * plane + (size_t)voff[s]];
+                }
+            }
+            __syncthreads();  // the previous item's fields are done with vel_lds
+#pragma unroll
+            for (int kk = 0; kk < KC_MAX; ++kk)
+#pragma unroll
+                for (int s = 0; s < VS; ++s)
+                    if (kk < nk && tid + s * NT < G.NV) vel_lds[kk * G.NV + tid + s * NT] = vb[kk][s];
             __syncthreads();
         }
 #pragma unroll 1
