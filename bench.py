@@ -363,7 +363,22 @@ def adjoint(args):
     t_adj_self = time.perf_counter() - ts
     if not torch.equal(qb, qb2):
         raise RuntimeError("recorded and self-contained adjoints differ")
-    del qb2, z3
+    # self-contained with the whole trajectory held in HBM (checkpoint 0: one
+    # segment, every stage state of the run stored; 42 GB at 256x256x128 and
+    # 200 SSP-RK3 steps): one forward replay and the backward sweep, no
+    # checkpoint sweep
+    qb3 = (2.0 * TOP * qT).contiguous()
+    for z in z3:
+        z.zero_()
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    adv.adjoint_advance(code, nsteps, dt, U, V, W, Q0, qb3, *z3, checkpoint=0)
+    torch.cuda.synchronize()
+    t_adj_full = time.perf_counter() - ts
+    if not torch.equal(qb, qb3):
+        raise RuntimeError("whole-trajectory and checkpointed adjoints differ")
+    full_bytes = (4 + 3 * nsteps) * Q0.numel() * 8  # 3 adjoint arrays, q0, the stage states
+    del qb2, qb3, z3
     # the stage-adjoint kernel alone, timed with HIP events on the library stream
     lam = (2.0 * TOP * qT).contiguous()
     o = [torch.zeros_like(Q0) for _ in range(4)]
@@ -412,6 +427,9 @@ def adjoint(args):
                    "cells": ncell, "forward_s": t_fwd, "adjoint_s": t_adj, "dt_s": dt,
                    "forward_recording_s": t_rec, "adjoint_self_contained_s": t_adj_self,
                    "self_contained_ratio": t_adj_self / t_fwd,
+                   "adjoint_whole_trajectory_s": t_adj_full,
+                   "whole_trajectory_ratio": t_adj_full / t_fwd,
+                   "whole_trajectory_bytes": full_bytes,
                    "device_arrays": 3 + nseg + 3 * kk},
         "roofline": {"bound": "hbm", "kernel": ("adjoint_kernel<32,8,weno5>" if scheme == "weno5" else
                                                 "adjoint_kernel<64,8>") + " (one stage adjoint)",

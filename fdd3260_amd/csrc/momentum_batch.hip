@@ -12,11 +12,13 @@
 // this one is checked against it with tolerances.
 //
 // Column-marching form: a workgroup owns a TX x TY tile of columns and a
-// chunk of levels.  Every operand the stencils of level k read (the three
-// momenta and the three advecting velocities at levels k-2..k+2, columns
-// within two cells of the tile) sits in a five-level LDS ring of FP32
-// tiles with a 2-cell halo; each level is loaded once, coalesced, and
-// serves five output levels.  One work item per workgroup.
+// chunk of levels.  The operands the stencils of level k read with
+// horizontal offsets (the momenta at k..k+2, the horizontal velocities at
+// k-2..k+1, w at k-1..k+2, columns within two cells of the tile) sit in
+// LDS rings of FP32 tiles with a 2-cell halo, each level loaded once,
+// coalesced; the values read only in the thread's own column (the momenta
+// at k-2, k-1, w at k-2) are carried in registers.  36 KB of LDS per
+// 256-thread group: four groups per CU.  One work item per workgroup.
 #include "mimadv_internal.hpp"
 
 namespace mimadv {
@@ -163,14 +165,21 @@ __device__ __forceinline__ float w_z(int k, int nz, const float *mz, const MomF 
 }
 
 template <int TX, int TY>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) void momentum_batch_kernel(
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) void momentum_batch_kernel(
     GridDev g, MomF c, MomBatchArgs a, int kchunk, int ntx, int nwork)
 {
     constexpr bool THREE_D = TY > 1;
     constexpr int NT = TX * TY;
     constexpr int H = 2, LW = TX + 2 * H, LH = THREE_D ? TY + 2 * H : 1, PL = LW * LH;
-    constexpr int NR = 5, TS = (PL + NT - 1) / NT;
-    __shared__ float ring[6][NR][PL];  // mu, mv, mw, ut, vt, wt
+    constexpr int TS = (PL + NT - 1) / NT;
+    // LDS rings of FP32 tiles with a 2-cell halo, each holding the levels
+    // its stencils read with horizontal offsets: the momenta at k..k+2, the
+    // horizontal velocities at k-2..k+1, w at k-1..k+2.  The stencils read
+    // the momenta at k-2, k-1 and w at k-2 only in the thread's own column:
+    // those stay in registers.
+    __shared__ float ringm[3][3][PL];  // mu, mv, mw
+    __shared__ float ringh[2][4][PL];  // ut, vt
+    __shared__ float ringw[4][PL];     // wt
 
     const int nx = g.nx, ny = g.ny, nz = g.nz, mx = nx - 5, my = ny - 5;
     const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
@@ -193,6 +202,8 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
         const int k0 = 1 + kc * kchunk, k1 = min(nz, k0 + kchunk - 1);
         const int i = i0 + tx, j = j0 + ty;
         const bool valid = i <= nx - 2 && (!THREE_D || j <= ny - 2);
+        // ring position of this thread's own column
+        const int own = (THREE_D ? (ty + H) * LW : 0) + tx + H;
         // plane offset of ring entry s of this thread (ghost columns read as
         // the caller filled them; positions past the array, the padding of a
         // partial tile, take a periodic image that feeds no interior cell)
@@ -209,37 +220,69 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
             }
             return (ii - 1) + nx * (jj - 1);
         };
+        // level km of the momenta and w, level kh of the horizontal
+        // velocities (levels outside 1..nz read as zero)
         float pre[6][TS];
-        auto fetch = [&](int kk) {
-            const bool in = kk >= 1 && kk <= nz;
-            const size_t kb = in ? sz * (kk - 1) : 0;
+        auto fetch = [&](int km, int kh) {
+            const bool inm = km >= 1 && km <= nz, inh = kh >= 1 && kh <= nz;
+            const size_t kbm = inm ? sz * (km - 1) : 0, kbh = inh ? sz * (kh - 1) : 0;
 #pragma unroll
             for (int s = 0; s < TS; ++s) {
                 const int off = t_off(s);
 #pragma unroll
-                for (int n = 0; n < 6; ++n) pre[n][s] = (in && off >= 0) ? src[n][kb + off] : 0.0f;
+                for (int n = 0; n < 6; ++n) {
+                    const bool h = n == 3 || n == 4;
+                    pre[n][s] = (off >= 0 && (h ? inh : inm)) ? src[n][(h ? kbh : kbm) + off] : 0.0f;
+                }
             }
         };
-        auto store = [&](int kk) {
-            const int slot = (kk + 2 * NR) % NR;
+        auto store = [&](int km, int kh, bool with_h) {
+            const int sm = (km + 6) % 3;
 #pragma unroll
             for (int s = 0; s < TS; ++s) {
-                if (tid + s * NT >= PL) continue;
+                const int e = tid + s * NT;
+                if (e >= PL) continue;
 #pragma unroll
-                for (int n = 0; n < 6; ++n) ring[n][slot][tid + s * NT] = pre[n][s];
+                for (int n = 0; n < 3; ++n) ringm[n][sm][e] = pre[n][s];
+                ringw[km & 3][e] = pre[5][s];
+                if (with_h) {
+                    ringh[0][kh & 3][e] = pre[3][s];
+                    ringh[1][kh & 3][e] = pre[4][s];
+                }
             }
         };
+        // fill: levels k0-2..k0 of every ring, then the own-column registers
+        // of k0-2, k0-1 (their ring slots are reused), then the momenta and w
+        // at k0+1; the momenta and w at k0+2 and the horizontal velocities at
+        // k0+1 are requested for the first level
+        float mk2_0, mk2_1, mk2_2, mk1_0, mk1_1, mk1_2, wkm2;
 #pragma unroll 1
-        for (int kk = k0 - 2; kk <= k0 + 1; ++kk) {
-            fetch(kk);
-            store(kk);
+        for (int kk = k0 - 2; kk <= k0; ++kk) {
+            fetch(kk, kk);
+            store(kk, kk, true);
         }
-        fetch(k0 + 2);
+        __syncthreads();
+        {
+            const int a2 = (k0 - 2 + 6) % 3, a1 = (k0 - 1 + 6) % 3;
+            mk2_0 = ringm[0][a2][own];
+            mk2_1 = ringm[1][a2][own];
+            mk2_2 = ringm[2][a2][own];
+            mk1_0 = ringm[0][a1][own];
+            mk1_1 = ringm[1][a1][own];
+            mk1_2 = ringm[2][a1][own];
+        }
+        wkm2 = ringw[(k0 - 2) & 3][own];
+        __syncthreads();
+        fetch(k0 + 1, k0 + 1);
+        store(k0 + 1, k0 + 1, false);
+        fetch(k0 + 2, k0 + 1);
 #pragma unroll 1
         for (int k = k0; k <= k1; ++k) {
-            store(k + 2);
+            store(k + 2, k + 1, true);
             __syncthreads();
-            if (k < k1) fetch(k + 3);
+            if (k < k1) fetch(k + 3, k + 2);
+            // ring slot of the momenta at level k
+            const int s0 = k % 3;
             if (valid) {
                 const size_t kb = sz * (k - 1);
                 const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
@@ -249,11 +292,22 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                     for (int n = 0; n < 3; ++n)
                         if (n != 1 || THREE_D) m0v[n] = a.m0[n][id];
                 }
-                // ring element of operand n at column offsets (di, dj) from
-                // this thread's column and level kk (k-2 <= kk <= k+2)
-                auto R = [&](int n, int di, int dj, int kk) {
-                    const int e = ((kk + 2 * NR) % NR) * PL + (THREE_D ? (ty + H + dj) * LW : 0) + (tx + H + di);
-                    return ring[n][0][e];
+                // operand n (a literal) at column offsets (di, dj) from this
+                // thread's column and level kk (k-2 <= kk <= k+2); the levels
+                // kept in registers are read only in the own column
+                auto R = [&](int n, int di, int dj, int kk) __attribute__((always_inline)) -> float {
+                    const int e = own + (THREE_D ? dj * LW : 0) + di;
+                    const int dk = kk - k;
+                    if (n < 3) {
+                        const float r2 = n == 0 ? mk2_0 : (n == 1 ? mk2_1 : mk2_2);
+                        const float r1 = n == 0 ? mk1_0 : (n == 1 ? mk1_1 : mk1_2);
+                        if (dk < 0) return dk == -2 ? r2 : r1;
+                        const int sl = s0 + dk;  // dk in 0..2
+                        return ringm[n][sl >= 3 ? sl - 3 : sl][e];
+                    }
+                    if (n < 5) return ringh[n - 3][kk & 3][e];
+                    if (dk == -2) return wkm2;
+                    return ringw[kk & 3][e];
                 };
                 float L[3];
                 // ---- u (uadv_fd): x self, y cross, z cross
@@ -297,7 +351,7 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                     if (a.epi == 0) {
                         qv = (float)Ln;
                     } else {
-                        const double m = (double)R(n, 0, 0, k);
+                        const double m = (double)ringm[n][s0][own];
                         const double qn = a.c0 != 0.0 ? a.c0 * (double)m0v[n] + a.c1 * (m + a.dt * Ln)
                                                       : a.c1 * (m + a.dt * Ln);
                         qv = (float)qn;
@@ -321,7 +375,15 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(3, 8))) 
                     }
                 }
             }
-            // the next level's store overwrites the ring slot of level k-2
+            // own-column registers for level k+1; the next level's stores
+            // overwrite the slots of the momenta at k and of w at k-1
+            mk2_0 = mk1_0;
+            mk2_1 = mk1_1;
+            mk2_2 = mk1_2;
+            mk1_0 = ringm[0][s0][own];
+            mk1_1 = ringm[1][s0][own];
+            mk1_2 = ringm[2][s0][own];
+            wkm2 = ringw[(k - 1) & 3][own];
             __syncthreads();
         }
     }
