@@ -47,6 +47,7 @@ HBM_PEAK_GBS = 8000.0
 HBM_TRAFFIC = {
     ("plain", "256,256,128", "weno", "heun", 2): 551.2e6,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1078.2e6,
+    ("fused", "f32", "512,512,160", "weno", "ssprk3", 20): 11.787e9,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
 
@@ -173,7 +174,8 @@ def fused(args):
     log(f"fused case built in {time.time() - t0:.1f}s, dt = {dt:.4f}s")
     scheme = args.scheme if args.scheme in ("weno", "muscl", "weno5") else "weno"
     adv = Advection(g, scal_adv=scheme, muscllimiter=args.limiter)
-    adv.set_tuning(group32=args.group32, kchunk=args.kchunk, tile_y=args.tile_y)
+    adv.set_tuning(group32=args.group32, kchunk=args.kchunk, blocks_per_cu=args.blocks_per_cu, tile_y=args.tile_y,
+                   tile_x=args.tile_x)
     code = _lib.INT_HEUN if integ == "heun" else _lib.INT_SSPRK3
     nst = 2 if integ == "heun" else 3
     graph = not args.no_graph
@@ -224,7 +226,7 @@ def fused(args):
     stage_bytes_all = sum(kind_bytes(idx) for idx in groups.values())
     sid = SCHEME_ID[scheme]
     kernel_names = {"f32": f"fused_kernel<{sid},64,4,float> (15 FP32 species)",
-                    "f64": f"fused_kernel<{sid},64,8,double> (theta, qv)",
+                    "f64": f"fused_kernel<{sid},64,4,double> (theta, qv)",
                     "momentum": "momentum_batch_kernel<32,8> (3 FP32 momenta)"}
     cpu = None
     cpu_steps = args.cpu_steps if args.cpu_steps is not None else 1
@@ -571,6 +573,10 @@ def main():
     ap.add_argument("--group32", type=int, default=0)
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of the column tile (2, 4, 8)")
+    ap.add_argument("--tile-x", type=int, default=0, help="fused workload: columns of the tile (64, or 32 with "
+                                                       "--tile-y 8)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="fused workload: persistent workgroups per CU (0: the occupancy limit)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="plain workload: skip the scheme variants")
     ap.add_argument("--copy-probe", type=int, default=0,

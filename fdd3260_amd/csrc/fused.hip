@@ -272,14 +272,14 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 // to four waves, and 96 reach five without spills.  The other instances are
 // bound by LDS or need their registers (WENO5's six-value stencils among
 // them: held to 96 VGPRs it spills).
-template <int SCHEME, int TY, class T>
+template <int SCHEME, int TX, int TY, class T>
 constexpr int fused_min_waves()
 {
-    return (SCHEME != 3 && TY == 4 && sizeof(T) == 4) ? 5 : 1;
+    return (SCHEME != 3 && TX * TY == 256 && TY > 1 && sizeof(T) == 4) ? 5 : 1;
 }
 
 template <int SCHEME, int TX, int TY, class T>
-__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(fused_min_waves<SCHEME, TY, T>(), 8))) void
+__global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(fused_min_waves<SCHEME, TX, TY, T>(), 8))) void
 fused_kernel(FusedArgs A)
 {
     constexpr int H = stencil_h<SCHEME>();
@@ -406,8 +406,11 @@ template <int SCHEME, class T>
 hipError_t launch_prec(const FusedArgs &A, hipStream_t s, int bpc)
 {
     if (!A.three_d) return launch_tiles<SCHEME, 64, 1, T>(A, s, bpc);
-    // default rows: 4 for FP32 (five waves per SIMD), 8 for FP64
-    const int ty = A.tile_y > 0 ? A.tile_y : (sizeof(T) == 4 ? 4 : 8);
+    if (A.tile_x == 32) return launch_tiles<SCHEME, 32, 8, T>(A, s, bpc);
+    // default rows: 4 (FP32: five groups of four waves per CU; FP64: four
+    // groups, as many waves as two 64x8 groups with half the work held at
+    // each barrier)
+    const int ty = A.tile_y > 0 ? A.tile_y : 4;
     switch (ty) {
     case 2: return launch_tiles<SCHEME, 64, 2, T>(A, s, bpc);
     case 4: return launch_tiles<SCHEME, 64, 4, T>(A, s, bpc);

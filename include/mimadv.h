@@ -273,8 +273,8 @@ typedef struct mimadv_tuning {
     int group32;        /* FP32 fields per launch (default and maximum 18) */
     int kchunk;         /* levels per work item (default and maximum 8) */
     int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
-    int tile_y;         /* rows of the 64-wide column tile in 3D: 2, 4 or 8 (default 4 for the
-                           FP32 launch, 8 for the FP64 launch) */
+    int tile_y;         /* rows of the column tile in 3D: 2, 4 or 8 (default 4) */
+    int tile_x;         /* columns of the tile: 64 (default), or 32 with tile_y 8 in 3D */
 } mimadv_tuning;
 int mimadv_set_tuning(mimadv_ctx *ctx, const mimadv_tuning *t);
 
