@@ -109,7 +109,8 @@ class Opts(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("group32", C.c_int), ("kchunk", C.c_int), ("blocks_per_cu", C.c_int), ("tile_y", C.c_int)]
+    _fields_ = [("group32", C.c_int), ("kchunk", C.c_int), ("blocks_per_cu", C.c_int), ("tile_y", C.c_int),
+                ("tile_x", C.c_int)]
 
 
 _lib = None

@@ -244,8 +244,8 @@ class Advection:
         self._check(rc, "mimadv_advance")
 
     # -- batched mixed-precision path ---------------------------------------
-    def set_tuning(self, group32=0, kchunk=0, blocks_per_cu=0, tile_y=0):
-        t = _lib.Tuning(int(group32), int(kchunk), int(blocks_per_cu), int(tile_y))
+    def set_tuning(self, group32=0, kchunk=0, blocks_per_cu=0, tile_y=0, tile_x=0):
+        t = _lib.Tuning(int(group32), int(kchunk), int(blocks_per_cu), int(tile_y), int(tile_x))
         self._check(self.lib.mimadv_set_tuning(self._h, C.byref(t)), "mimadv_set_tuning")
 
     def _batch_fields(self, fields):

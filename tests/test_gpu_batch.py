@@ -399,8 +399,8 @@ def test_batch_graph_follows_option_changes(cuda, scheme):
 
 @pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
 def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
-    """Every public tile instance (64x2, 64x4, 64x8 for both precisions) and
-    level chunks of 8 and 5 on nz = 17, where the last chunk is ragged
+    """Every public tile instance (64x2, 64x4, 64x8 and 32x8 for both
+    precisions) and level chunks of 8 and 5 on nz = 17, where the last chunk is ragged
     (17 = 2*8 + 1 = 3*5 + 2): each z face is evaluated by one code site
     whatever the chunking, so all shapes give the same bits; the default
     shape is held to the oracle."""
@@ -417,14 +417,16 @@ def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
         du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
         fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
                                                                for q in f32]
-        for ty in (0, 2, 4, 8):
+        for tx, ty in ((0, 0), (0, 2), (0, 4), (0, 8), (32, 8)):
             for kc in (0, 5):
-                a.set_tuning(tile_y=ty, kchunk=kc)
+                a.set_tuning(tile_x=tx, tile_y=ty, kchunk=kc)
                 outs = [torch.zeros_like(f) for f in fields]
                 a.batch_tendency(du, dv, dw, fields, outs)
                 a.stream_sync()
-                results[(ty, kc)] = [o.cpu() for o in outs]
-    base = results[(0, 0)]
+                results[(tx, ty, kc)] = [o.cpu() for o in outs]
+        with pytest.raises(_lib.MimadvError, match="tile_x"):
+            a.set_tuning(tile_x=32, tile_y=4)
+    base = results[(0, 0, 0)]
     for n, (o, r) in enumerate(zip(base, refs)):
         got = o.double().numpy()
         if n < len(f64):
@@ -433,7 +435,7 @@ def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
             assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
     for key, outs in results.items():
         for n, (x, y) in enumerate(zip(outs, base)):
-            assert torch.equal(x, y), f"tile_y/kchunk {key}: field {n} differs from the default shape"
+            assert torch.equal(x, y), f"tile_x/tile_y/kchunk {key}: field {n} differs from the default shape"
 
 
 @pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
