@@ -370,3 +370,54 @@ def test_adjoint_autograd_matches_central_differences(cuda):
                 assert fd is not None, f"direction {n}: every step straddles a jump"
                 print(f"  direction {n}: eps {eps:g} fd {fd:.12e} adjoint {ad:.12e}", flush=True)
                 assert abs(fd - ad) <= 1e-5 * abs(ad), (n, eps, fd, ad)
+
+
+def test_central_differences_64_directions(cuda):
+    """Central differences of J = sum over the top half of rho0*q(T)^2 after
+    6 SSP-RK3 steps on smooth data, along 64 seeded directions that perturb
+    q0, u, v and w together, against the adjoint gradient.  Each direction
+    takes the largest step of eps = 1e-6 .. 1e-9 (times the direction, whose
+    q part is scaled to the data) whose one-sided differences agree to 1e-4
+    of the derivative (no jump of the scheme inside [-eps, eps]), and holds
+    the central difference to 1e-5 of it."""
+    print("case adjoint FD 64 directions", flush=True)
+    g, u, v, w, q0 = _case(True, "gaussian", seed=121, maxx=28, maxy=24, nz=20)
+    dt = cases.cfl_dt(g, u, v, w, 0.5)
+    nsteps = 6
+    rho0 = 1.2 * np.exp(-g.zc() / 8000.0)
+    top = np.zeros(g.shape)
+    top[g.nz // 2:] = rho0[g.nz // 2:, None, None]
+    top = _interior_only(g, top)
+    rng = np.random.default_rng(122)
+    with Advection(g, scal_adv="weno") as a:
+        a.set_stream(torch.cuda.current_stream(cuda).cuda_stream)
+        U, V, W, Q0, TOP = _dev(cuda, u, v, w, q0, top)
+
+        def QT(q0_, u_, v_, w_):
+            q = q0_.clone()
+            a.advance(_lib.INT_SSPRK3, nsteps, dt, u_, v_, w_, [q])
+            return q
+
+        def dJ(qa, qb):
+            return float((TOP * (qa - qb) * (qa + qb)).sum())
+
+        q0T = QT(Q0, U, V, W)
+        qb = (2.0 * TOP * q0T).contiguous()
+        ub, vb, wb = (torch.zeros_like(Q0) for _ in range(3))
+        a.adjoint_advance(_lib.INT_SSPRK3, nsteps, dt, U, V, W, Q0, qb, ub, vb, wb, checkpoint=3)
+        grads = (qb, ub, vb, wb)
+        used = {}
+        for n in range(64):
+            D = _dev(cuda, *_dirs(g, rng, float(np.abs(q0).mean())))
+            ad = float(sum((gr * y).sum() for gr, y in zip(grads, D)))
+            fd = None
+            for eps in (1e-6, 1e-7, 1e-8, 1e-9):
+                qp = QT(*(x + eps * y for x, y in zip((Q0, U, V, W), D)))
+                qm = QT(*(x - eps * y for x, y in zip((Q0, U, V, W), D)))
+                if abs(dJ(qp, q0T) - dJ(q0T, qm)) / eps <= 1e-4 * abs(ad):
+                    fd = dJ(qp, qm) / (2 * eps)
+                    break
+            assert fd is not None, f"direction {n}: every step straddles a jump"
+            used[eps] = used.get(eps, 0) + 1
+            assert abs(fd - ad) <= 1e-5 * abs(ad), (n, eps, fd, ad)
+        print(f"  steps taken: {used}", flush=True)
