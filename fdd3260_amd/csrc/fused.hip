@@ -55,6 +55,22 @@ struct Item {
 // every field (the chunk length is capped at this).
 constexpr int FUSED_KC = 8;
 
+// The FP32 256-thread tiles of the WENO and MUSCL schemes: six-level chunks
+// take 25 KB of LDS, so six groups of four waves fit a CU, and the kernel
+// is held to 80 VGPRs (a few values of the per-item velocity staging spill;
+// the level loop does not).  The FP64 and WENO5 instances are bound by
+// their registers and keep eight-level chunks.
+template <int SCHEME, int TX, int TY, class T>
+constexpr bool fused_six_waves()
+{
+    return SCHEME != 3 && TX * TY == 256 && TY > 1 && sizeof(T) == 4;
+}
+template <int SCHEME, int TX, int TY, class T>
+constexpr int fused_kc()
+{
+    return fused_six_waves<SCHEME, TX, TY, T>() ? 6 : FUSED_KC;
+}
+
 // Face e of the flattened list [x faces row-major | y faces] of a tile:
 // kind (1 x, 2 y), LDS offset/step of its stencil and the column of its
 // velocity (u(f+1) of x face f + 1/2, v(g+1) of y face g + 1/2).
@@ -267,15 +283,13 @@ __device__ __forceinline__ void march_field(const FusedArgs &A, const T *__restr
 // T: the precision of every field of the launch (FP64 and FP32 members of
 // a batch go to two launches, so that the FP32 kernel keeps the register
 // budget of FP32 arithmetic).
-// Waves per SIMD the register allocation must allow.  The FP32 64x4 tile
-// takes 32 KB of LDS, so five groups fit a CU; at 97 VGPRs it would be held
-// to four waves, and 96 reach five without spills.  The other instances are
-// bound by LDS or need their registers (WENO5's six-value stencils among
-// them: held to 96 VGPRs it spills).
+// Waves per SIMD the register allocation must allow: six for the FP32
+// 256-thread tiles (see fused_kc).  The other instances are bound by LDS or
+// need their registers (WENO5's six-value stencils among them).
 template <int SCHEME, int TX, int TY, class T>
 constexpr int fused_min_waves()
 {
-    return (SCHEME != 3 && TX * TY == 256 && TY > 1 && sizeof(T) == 4) ? 5 : 1;
+    return fused_six_waves<SCHEME, TX, TY, T>() ? 6 : 1;
 }
 
 template <int SCHEME, int TX, int TY, class T>
@@ -286,7 +300,8 @@ fused_kernel(FusedArgs A)
     using I = Item<TX, TY, H>;
     __shared__ T tile_lds[I::LH * I::LW];
     __shared__ T face_lds[I::FS * I::NT];
-    __shared__ float vel_lds[FUSED_KC * I::NV];
+    constexpr int KC = fused_kc<SCHEME, TX, TY, T>();
+    __shared__ float vel_lds[KC * I::NV];
 
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -344,9 +359,9 @@ fused_kernel(FusedArgs A)
                     vw[s] = true;
                 }
             }
-            float vb[FUSED_KC][VS];
+            float vb[KC][VS];
 #pragma unroll
-            for (int kk = 0; kk < FUSED_KC; ++kk) {
+            for (int kk = 0; kk < KC; ++kk) {
                 const int k = k0 + kk;
 #pragma unroll
                 for (int s = 0; s < VS; ++s) {
@@ -357,7 +372,7 @@ fused_kernel(FusedArgs A)
             }
             __syncthreads();  // the previous item's fields are done with vel_lds
 #pragma unroll
-            for (int kk = 0; kk < FUSED_KC; ++kk)
+            for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
                 for (int s = 0; s < VS; ++s)
                     if (kk < nk && tid + s * I::NT < I::NV) vel_lds[kk * I::NV + tid + s * I::NT] = vb[kk][s];
@@ -385,7 +400,8 @@ hipError_t launch_tiles(const FusedArgs &a0, hipStream_t s, int blocks_per_cu)
     A.nty = TY > 1 ? (A.ny - 5 + TY - 1) / TY : 1;
     const int tiles = A.ntx * A.nty;
     A.xcd_remap = (tiles % 8 == 0) ? 1 : 0;
-    A.kchunk = A.kchunk <= 0 ? FUSED_KC : min(A.kchunk, FUSED_KC);
+    constexpr int KC = fused_kc<SCHEME, TX, TY, T>();
+    A.kchunk = A.kchunk <= 0 ? KC : min(A.kchunk, KC);
     A.nwork = tiles * ((A.nz + A.kchunk - 1) / A.kchunk);
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||

@@ -271,7 +271,8 @@ int mimadv_advection_1d(mimadv_ctx *ctx, const double *w0, const double *dens, i
 /* Tuning of the batched launches; 0 selects the default of each field. */
 typedef struct mimadv_tuning {
     int group32;        /* FP32 fields per launch (default and maximum 18) */
-    int kchunk;         /* levels per work item (default and maximum 8) */
+    int kchunk;         /* levels per work item (default and maximum 8; 6 for FP32 WENO/MUSCL
+                           launches with 256-thread tiles) */
     int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
     int tile_y;         /* rows of the column tile in 3D: 2, 4 or 8 (default 4) */
     int tile_x;         /* columns of the tile: 64 (default), or 32 with tile_y 8 in 3D */
