@@ -447,8 +447,8 @@ def adjoint(args):
 def multi(args):
     """Uneven independent subdomains: 48 periodic subdomains with seeded
     sizes (MAXX, MAXY in [48, 192], nz in [48, 96]), 4-16 FP32 species each
-    (log-normal), wind RMS in [1, 4] m/s, scheme weno or muscl (MC) per
-    subdomain, SSP-RK3, one launch per stage for the whole batch with tile
+    (log-normal), wind RMS in [1, 4] m/s, scheme weno (weno5 with --scheme
+    weno5) or muscl (MC) per subdomain, SSP-RK3, one launch per stage for the whole batch with tile
     shapes from the autotuner.  Also timed: the same batch as one batched
     launch per subdomain and stage."""
     import numpy as np
@@ -461,7 +461,8 @@ def multi(args):
     t0 = time.time()
     dev = torch.device("cuda:0")
     advs, vels, fields, dts, cells, nspec = [], [], [], [], [], []
-    for d in cases.config4_draw(seed=4, ndom=ndom):
+    weno = "weno5" if args.scheme == "weno5" else "weno"
+    for d in cases.config4_draw(seed=4, ndom=ndom, weno=weno):
         g = gm.stretched(d["maxx"], d["maxy"], d["nz"])
         u, v, w = cases.turbulent_wind(g, seed=d["wind_seed"], rms=d["rms"])
         dts.append(cases.cfl_dt(g, u, v, w, 0.5))
@@ -520,7 +521,7 @@ def multi(args):
         "dtype": "f32",
         "data": "synthetic (seeded subdomain sizes, turbulent winds, log-normal species)",
         "config": {"workload": f"{ndom} uneven periodic subdomains, MAXX/MAXY 48-192, nz 48-96, 4-16 FP32 species, "
-                               f"weno/muscl, ssprk3, one launch per stage",
+                               f"{weno}/muscl, ssprk3, one launch per stage",
                    "cells": sum(cells), "species_cells": sum(c * n for c, n in zip(cells, nspec)),
                    "autotune_s": t_at, "tile_choices": shapes,
                    "per_subdomain_launches_updates_per_s": upd / wall_seq,
@@ -605,12 +606,20 @@ def main():
             # the other benchmark workloads, each measured the same way, with
             # their own default sizes and steps
             sub = {}
-            for name, fn in (("plain_fp64_256x256x128", plain), ("adjoint_200_steps", adjoint),
-                             ("multi_48_subdomains", multi)):
+            # and the WENO5 variants of the batch, the uneven subdomains and
+            # the adjoint (the scheme those workloads are specified with;
+            # the reference scheme 'weno' is the headline)
+            for name, fn, scheme in (("plain_fp64_256x256x128", plain, "weno"),
+                                     ("adjoint_200_steps", adjoint, "weno"),
+                                     ("multi_48_subdomains", multi, "weno"),
+                                     ("fused_weno5_512x512x160", fused, "weno5"),
+                                     ("adjoint_weno5_200_steps", adjoint, "weno5"),
+                                     ("multi_weno5_48_subdomains", multi, "weno5")):
                 sa = argparse.Namespace(**vars(args))
                 sa.size = None
                 sa.steps = None
                 sa.integrator = None
+                sa.scheme = scheme
                 sa.cpu_steps = 0 if name != "plain_fp64_256x256x128" else None
                 log(f"sub-record {name}")
                 r = fn(sa)

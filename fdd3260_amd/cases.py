@@ -148,18 +148,18 @@ def config3_fields(g: Grid, u, v, w, seed=5, nspec=15):
     return out, prec
 
 
-def config4_draw(seed=4, ndom=48):
+def config4_draw(seed=4, ndom=48, weno="weno"):
     """Seeded draw of the uneven-subdomain batch: per subdomain MAXX, MAXY in
-    [48, 192], nz in [48, 96], scheme 'weno' or 'muscl' (even odds), 4-16
-    species, wind RMS in [1, 4] m/s, and per species the log-normal mu in
-    [-20, -6] (sigma 1.5).  Returns a list of dicts; field seeds are
-    1000 + d (wind) and 2000 + 20 d + s (species)."""
+    [48, 192], nz in [48, 96], scheme `weno` ('weno' or 'weno5') or 'muscl'
+    (even odds), 4-16 species, wind RMS in [1, 4] m/s, and per species the
+    log-normal mu in [-20, -6] (sigma 1.5).  Returns a list of dicts; field
+    seeds are 1000 + d (wind) and 2000 + 20 d + s (species)."""
     rng = np.random.default_rng(seed)
     doms = []
     for d in range(ndom):
         mx, my = (int(x) for x in rng.integers(48, 193, 2))
         nz = int(rng.integers(48, 97))
-        scheme = "weno" if rng.random() < 0.5 else "muscl"
+        scheme = weno if rng.random() < 0.5 else "muscl"
         ns = int(rng.integers(4, 17))
         rms = float(rng.uniform(1.0, 4.0))
         mus = [float(rng.uniform(-20, -6)) for _ in range(ns)]

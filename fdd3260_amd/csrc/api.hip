@@ -1812,7 +1812,7 @@ int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int 
     for (int d = 0; d < m->ndom; ++d) {
         if (tile_x) tile_x[d] = m->tile_x[d];
         if (tile_y) tile_y[d] = MULTI_NT / m->tile_x[d];
-        if (kchunk) kchunk[d] = m->kchunk[d] > 0 ? std::min(m->kchunk[d], 8) : 8;
+        if (kchunk) kchunk[d] = m->kchunk[d] > 0 ? std::min(m->kchunk[d], MULTI_KC) : MULTI_KC;
     }
     return MIMADV_OK;
 }
@@ -1822,7 +1822,7 @@ int mimadv_multi_autotune(mimadv_multi *m, int reps)
     if (!m) return MIMADV_ERR_ARG;
     if (reps < 1) reps = 1;
     static const int cand_x[3] = {64, 32, 16};
-    static const int cand_k[2] = {4, 8};
+    static const int cand_k[2] = {4, MULTI_KC};
     // A candidate is timed at saturation: the subdomain's work list is
     // repeated until the launch holds at least `target` work items (the
     // copies write the same values to the same scratch stage array), so a

@@ -43,6 +43,7 @@ hipError_t launch_fused(int scheme, int n64, int n32, const FusedArgs &A, hipStr
 // subdomain (FP32 fields only, 3D, tile_x * tile_y = 256, ntx/nty/kchunk/
 // work_begin filled in by multi_prepare), nwork the total work count.
 constexpr int MULTI_NT = 256;
+constexpr int MULTI_KC = 6;  // levels of a multi-domain work item (maximum and default)
 bool multi_tile_ok(int tile_x, int tile_y);
 void multi_prepare(FusedArgs &A, int work_begin);  // sets ntx, nty, nwork; kchunk clamped
 hipError_t launch_multi(const FusedArgs *d_table, int ndom, int nwork, hipStream_t s);

@@ -29,7 +29,7 @@ constexpr int NT = MULTI_NT;
 constexpr int FS = 3;                  // 2*NT + TX + TY faces per tile
 constexpr int TS = 3;                  // (TX+4)*(TY+4) tile values
 constexpr int TILE_MAX = 70 * 10;      // largest (TX+2H)*(TY+2H) of the shapes (64x4, H = 3)
-constexpr int KC_MAX = 8;              // levels of a work item (velocities staged in LDS)
+constexpr int KC_MAX = MULTI_KC;       // levels of a work item (velocities staged in LDS)
 constexpr int NV_MAX = 2 * NT + 68 + NT;  // face velocities and column w of one level
 
 struct Geom {
@@ -183,7 +183,10 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
     }
 }
 
-__global__ __launch_bounds__(NT) void multi_kernel(const FusedArgs *__restrict__ table, int ndom, int nwork)
+// Six-level work items take 25 KB of LDS; the register budget is held to
+// five waves per SIMD (96 VGPRs; three values are reloaded per level).
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5, 8))) void multi_kernel(
+    const FusedArgs *__restrict__ table, int ndom, int nwork)
 {
     __shared__ float tile_lds[TILE_MAX];
     __shared__ float face_lds[FS * NT];

@@ -337,11 +337,11 @@ int mimadv_multi_create(int ndom, mimadv_ctx *const *ctx, const float *const *u,
 void mimadv_multi_destroy(mimadv_multi *m);
 const char *mimadv_multi_last_error(const mimadv_multi *m);
 /* Tile shape and level chunk per subdomain (arrays of ndom; kchunk 0 =
- * default 8, the maximum: a work item stages its levels' velocities in LDS).  Without a call, a cost model picks the shape that wastes the
+ * default 6, the maximum: a work item stages its levels' velocities in LDS).  Without a call, a cost model picks the shape that wastes the
  * fewest face evaluations. */
 int mimadv_multi_set_tiles(mimadv_multi *m, const int *tile_x, const int *kchunk);
 int mimadv_multi_get_tiles(const mimadv_multi *m, int *tile_x, int *tile_y, int *kchunk);
-/* Autotuner: times every candidate (tile shape x level chunk in 4, 8)
+/* Autotuner: times every candidate (tile shape x level chunk in 4, 6)
  * of every subdomain with launches of one stage of that subdomain alone,
  * its work list repeated until the launch fills the GPU (so a candidate is
  * judged by its cost per work item at saturation), reps times each; keeps
