@@ -535,9 +535,10 @@ hipError_t launch_adjoint_stage(const AdjArgs &a0, hipStream_t s)
     if (A.kchunk <= 0) A.kchunk = 16;
     if (A.kchunk > A.nz) A.kchunk = A.nz;
     if (A.weno5) {
-        // 6-cell stencils: a 32x8 tile keeps the face contributions within
-        // two groups' worth of LDS per CU
-        if (A.ny > 1) return launch_tile<32, 8, true>(A, s);
+        // 6-cell stencils: a 16x16 tile has the fewest faces per cell
+        // (672 for 256 cells), so its face contributions, tiles and slot
+        // offsets take 49.7 KB of LDS and three groups share a CU
+        if (A.ny > 1) return launch_tile<16, 16, true>(A, s);
         return launch_tile<64, 1, true>(A, s);
     }
     if (A.ny > 1) return launch_tile<64, 8, false>(A, s);
