@@ -58,6 +58,49 @@ __device__ __forceinline__ double quot(double n, double d, double r)
     return fma(fma(-d, q, n), r, q);
 }
 
+// n / 6, correctly rounded, from the correctly rounded reciprocal y of 6:
+// q = n y is within an ulp of n/6 and q + (n - 6 q) y rounds to n / 6
+// (Markstein's corrected quotient; the residual is exact in an fma) when
+// n, q and the residual are normal.  Zero keeps its sign; operands outside
+// [2^-960, 2^960] and non-finite ones take the division.
+__device__ __forceinline__ double div6(double n)
+{
+    constexpr double y = 1.0 / 6.0;
+    const double q = n * y;
+    const double an = fabs(n);
+    if (an >= 0x1p-960 && an <= 0x1p960) return fma(fma(-6.0, q, n), y, q);
+    if (n == 0.0) return q;
+    return n / 6.0;
+}
+
+// n / d, d > 0, with the refined reciprocal inside the range the steps are
+// exact for (see refined_rcp); zero numerators keep their sign.
+__device__ __forceinline__ double div_pos(double n, double d)
+{
+    const double an = fabs(n);
+    if (d >= 0x1p-400 && d <= 0x1p400 && an >= 0x1p-400 && an <= 0x1p400) return quot(n, d, refined_rcp(d));
+    if (n == 0.0 && d >= 0x1p-400 && d <= 0x1p400) return n * refined_rcp(d);
+    return n / d;
+}
+
+// WENO5-JS reconstruction at the right face of c from a..e (the oracle's
+// weno5_left), bit for bit: the same operation order, with the divisions as
+// div6 / div_pos.
+__device__ __forceinline__ double weno5_left_x(double a, double b, double c, double d, double e)
+{
+    const double eps = 1e-6;
+    const double q0 = div6(2.0 * a - 7.0 * b + 11.0 * c);
+    const double q1 = div6(-b + 5.0 * c + 2.0 * d);
+    const double q2 = div6(2.0 * c + 5.0 * d - e);
+    const double b0 = 13.0 / 12.0 * (a - 2.0 * b + c) * (a - 2.0 * b + c) + 0.25 * (a - 4.0 * b + 3.0 * c) * (a - 4.0 * b + 3.0 * c);
+    const double b1 = 13.0 / 12.0 * (b - 2.0 * c + d) * (b - 2.0 * c + d) + 0.25 * (b - d) * (b - d);
+    const double b2 = 13.0 / 12.0 * (c - 2.0 * d + e) * (c - 2.0 * d + e) + 0.25 * (3.0 * c - 4.0 * d + e) * (3.0 * c - 4.0 * d + e);
+    const double a0 = div_pos(0.1, (eps + b0) * (eps + b0));
+    const double a1 = div_pos(0.6, (eps + b1) * (eps + b1));
+    const double a2 = div_pos(0.3, (eps + b2) * (eps + b2));
+    return div_pos(a0 * q0 + a1 * q1 + a2 * q2, a0 + a1 + a2);
+}
+
 // Reference face flux for the upwind side only.  nonlin: weights may
 // switch (x/y always, z for faces 3..nz-3).
 __device__ __forceinline__ double weno_face_up(double a, double b, double c, double d, double uu, bool nonlin)
@@ -159,10 +202,10 @@ __device__ __forceinline__ double xy_face(const double *t, int b, int st, double
         else return lw_hface<true>(1, d1, sp, Q, U);
     } else {
         if (uu > 0.0) {
-            const double fl = weno5_left(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], t[b + 4 * st]);
+            const double fl = weno5_left_x(t[b], t[b + st], t[b + 2 * st], t[b + 3 * st], t[b + 4 * st]);
             return 0.5 * fl * (uu + fabs(uu));
         }
-        const double fr = weno5_left(t[b + 5 * st], t[b + 4 * st], t[b + 3 * st], t[b + 2 * st], t[b + st]);
+        const double fr = weno5_left_x(t[b + 5 * st], t[b + 4 * st], t[b + 3 * st], t[b + 2 * st], t[b + st]);
         return 0.5 * fr * (uu - fabs(uu));
     }
 }
@@ -328,9 +371,9 @@ __device__ __forceinline__ void march_body(const GridDev &g, const SchemeParams 
                     // WENO5-JS where the 5-cell upwind stencil lies in the
                     // column, the reference face elsewhere
                     if (wc > 0.0 && k >= 3 && k <= nz - 2)
-                        fzt = 0.5 * weno5_left(zm1, z0, z1, z2, z3) * (wc + fabs(wc));
+                        fzt = 0.5 * weno5_left_x(zm1, z0, z1, z2, z3) * (wc + fabs(wc));
                     else if (wc < 0.0 && k >= 2 && k <= nz - 3)
-                        fzt = 0.5 * weno5_left(z4, z3, z2, z1, z0) * (wc - fabs(wc));
+                        fzt = 0.5 * weno5_left_x(z4, z3, z2, z1, z0) * (wc - fabs(wc));
                     else
                         fzt = zface(k, nz, sp.weno_bottom, k == 1 ? z1 : z0, z1, z2, z3, wc);
                 } else if constexpr (S == 0) {
