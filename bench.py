@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 HBM_TRAFFIC = {
     ("plain", "256,256,128", "weno", "heun", 2): 551.2e6,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1078.2e6,
-    ("fused", "f32", "512,512,160", "weno", "ssprk3", 20): 12.018e9,
+    ("fused", "f32", "512,512,160", "weno", "ssprk3", 20): 13.823e9,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
 
@@ -225,8 +225,10 @@ def fused(args):
     stage_us = sum(v["avg_launch_us"] for v in per_kind.values())
     stage_bytes_all = sum(kind_bytes(idx) for idx in groups.values())
     sid = SCHEME_ID[scheme]
-    kernel_names = {"f32": f"fused_kernel<{sid},64,4,float> (15 FP32 species)",
-                    "f64": f"fused_kernel<{sid},64,4,double> (theta, qv)",
+    tile = ("32,8" if args.tile_x == 32 or (args.tile_x == 0 and args.tile_y == 0)
+            else f"64,{args.tile_y or 4}")
+    kernel_names = {"f32": f"fused_kernel<{sid},{tile},float> (15 FP32 species)",
+                    "f64": f"fused_kernel<{sid},{tile},double> (theta, qv)",
                     "momentum": "momentum_batch_kernel<32,8> (3 FP32 momenta)"}
     cpu = None
     cpu_steps = args.cpu_steps if args.cpu_steps is not None else 1
@@ -573,7 +575,8 @@ def main():
     ap.add_argument("--checkpoint", type=int, default=20, help="adjoint workload: steps between checkpoints")
     ap.add_argument("--group32", type=int, default=0)
     ap.add_argument("--kchunk", type=int, default=0)
-    ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of the column tile (2, 4, 8)")
+    ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of a 64-wide column tile (2, 4, 8; "
+                                                       "default: 32x8 tiles)")
     ap.add_argument("--tile-x", type=int, default=0, help="fused workload: columns of the tile (64, or 32 with "
                                                        "--tile-y 8)")
     ap.add_argument("--blocks-per-cu", type=int, default=0,

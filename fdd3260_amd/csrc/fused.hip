@@ -422,10 +422,9 @@ template <int SCHEME, class T>
 hipError_t launch_prec(const FusedArgs &A, hipStream_t s, int bpc)
 {
     if (!A.three_d) return launch_tiles<SCHEME, 64, 1, T>(A, s, bpc);
-    if (A.tile_x == 32) return launch_tiles<SCHEME, 32, 8, T>(A, s, bpc);
-    // default rows: 4 (FP32: five groups of four waves per CU; FP64: four
-    // groups, as many waves as two 64x8 groups with half the work held at
-    // each barrier)
+    // default 32x8: the fewest x/y faces per cell of the 256-thread tiles
+    // (552 for 256 cells against 580 for 64x4) and the smallest halo tile
+    if (A.tile_x == 32 || (A.tile_x == 0 && A.tile_y == 0)) return launch_tiles<SCHEME, 32, 8, T>(A, s, bpc);
     const int ty = A.tile_y > 0 ? A.tile_y : 4;
     switch (ty) {
     case 2: return launch_tiles<SCHEME, 64, 2, T>(A, s, bpc);
