@@ -226,7 +226,7 @@ def fused(args):
     stage_bytes_all = sum(kind_bytes(idx) for idx in groups.values())
     sid = SCHEME_ID[scheme]
     tile = ("32,8" if args.tile_x == 32 or (args.tile_x == 0 and args.tile_y == 0)
-            else f"64,{args.tile_y or 4}")
+            else "16,16" if args.tile_x == 16 else f"64,{args.tile_y or 4}")
     kernel_names = {"f32": f"fused_kernel<{sid},{tile},float> (15 FP32 species)",
                     "f64": f"fused_kernel<{sid},{tile},double> (theta, qv)",
                     "momentum": "momentum_batch_kernel<32,8> (3 FP32 momenta)"}
@@ -577,7 +577,7 @@ def main():
     ap.add_argument("--kchunk", type=int, default=0)
     ap.add_argument("--tile-y", type=int, default=0, help="fused workload: rows of a 64-wide column tile (2, 4, 8; "
                                                        "default: 32x8 tiles)")
-    ap.add_argument("--tile-x", type=int, default=0, help="fused workload: columns of the tile (64, or 32 with "
+    ap.add_argument("--tile-x", type=int, default=0, help="fused workload: columns of the tile (64, 16, or 32 with "
                                                        "--tile-y 8)")
     ap.add_argument("--blocks-per-cu", type=int, default=0,
                     help="fused workload: persistent workgroups per CU (0: the occupancy limit)")

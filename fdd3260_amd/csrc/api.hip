@@ -702,8 +702,9 @@ int mimadv_set_tuning(mimadv_ctx *c, const mimadv_tuning *t)
         return fail(c, MIMADV_ERR_ARG, "tuning values out of range");
     if (t->tile_y != 0 && t->tile_y != 2 && t->tile_y != 4 && t->tile_y != 8)
         return fail(c, MIMADV_ERR_ARG, "tile_y must be 2, 4 or 8");
-    if (t->tile_x != 0 && t->tile_x != 64 && !(t->tile_x == 32 && (t->tile_y == 0 || t->tile_y == 8)))
-        return fail(c, MIMADV_ERR_ARG, "tile_x must be 64, or 32 with tile_y 8");
+    if (t->tile_x != 0 && t->tile_x != 64 && !(t->tile_x == 32 && (t->tile_y == 0 || t->tile_y == 8)) &&
+        !(t->tile_x == 16 && t->tile_y == 0))
+        return fail(c, MIMADV_ERR_ARG, "tile_x must be 64, 32 with tile_y 8, or 16 with tile_y 0");
     drop_graph(c);
     c->tuning = *t;
     return MIMADV_OK;

@@ -425,6 +425,7 @@ hipError_t launch_prec(const FusedArgs &A, hipStream_t s, int bpc)
     // default 32x8: the fewest x/y faces per cell of the 256-thread tiles
     // (552 for 256 cells against 580 for 64x4) and the smallest halo tile
     if (A.tile_x == 32 || (A.tile_x == 0 && A.tile_y == 0)) return launch_tiles<SCHEME, 32, 8, T>(A, s, bpc);
+    if (A.tile_x == 16) return launch_tiles<SCHEME, 16, 16, T>(A, s, bpc);
     const int ty = A.tile_y > 0 ? A.tile_y : 4;
     switch (ty) {
     case 2: return launch_tiles<SCHEME, 64, 2, T>(A, s, bpc);

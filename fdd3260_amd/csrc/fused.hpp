@@ -27,8 +27,9 @@ struct FusedArgs {
     int weno_bottom;
     int ntx, nty, kchunk, nwork, xcd_remap;
     int tile_y;  // rows of the tile in 3D (2, 4, 8; 0: see tile_x)
-    int tile_x;  // tile width: batched launches 64 or 32 (with tile_y 8 or
-                 // 0); both 0: 32x8; multi-domain launches tile_x * tile_y = 256
+    int tile_x;  // tile width: batched launches 64, 32 (with tile_y 8 or 0) or
+                 // 16 (16x16); both 0: 32x8; multi-domain launches
+                 // tile_x * tile_y = 256
     // multi-domain launches only
     int scheme;        // 0 WENO, 2 MUSCL
     int work_begin;    // first work id of this subdomain in the launch

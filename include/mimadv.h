@@ -275,9 +275,9 @@ typedef struct mimadv_tuning {
                            launches with 256-thread tiles) */
     int blocks_per_cu;  /* persistent grid size per CU (default: occupancy) */
     int tile_y;         /* rows of the column tile in 3D: 2, 4 or 8 */
-    int tile_x;         /* columns of the tile: 64, or 32 with tile_y 8 (or 0).  Both 0:
-                           32x8 tiles in 3D; tile_x 0 with tile_y set: 64 columns; tile_x
-                           64 with tile_y 0: 64x4 */
+    int tile_x;         /* columns of the tile: 64, 32 with tile_y 8 (or 0), or 16 with
+                           tile_y 0 (16x16).  Both 0: 32x8 tiles in 3D; tile_x 0 with tile_y
+                           set: 64 columns; tile_x 64 with tile_y 0: 64x4 */
 } mimadv_tuning;
 int mimadv_set_tuning(mimadv_ctx *ctx, const mimadv_tuning *t);
 

@@ -399,8 +399,8 @@ def test_batch_graph_follows_option_changes(cuda, scheme):
 
 @pytest.mark.parametrize("scheme", ["weno", "muscl", "weno5"])
 def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
-    """Every public tile instance (32x8, the default, and 64x2, 64x4, 64x8 for
-    both precisions) and level chunks of the default and 5 on nz = 17, where the last chunk is ragged
+    """Every public tile instance (32x8, the default, and 64x2, 64x4, 64x8,
+    16x16 for both precisions) and level chunks of the default and 5 on nz = 17, where the last chunk is ragged
     (17 = 2*8 + 1 = 3*5 + 2): each z face is evaluated by one code site
     whatever the chunking, so all shapes give the same bits; the default
     shape is held to the oracle."""
@@ -417,7 +417,7 @@ def test_batch_tile_shapes_ragged_chunks(cuda, scheme):
         du, dv, dw = (torch.from_numpy(x.astype(np.float32)).to(cuda) for x in (u, v, w))
         fields = [torch.from_numpy(q).to(cuda) for q in f64] + [torch.from_numpy(q.astype(np.float32)).to(cuda)
                                                                for q in f32]
-        for tx, ty in ((0, 0), (0, 2), (0, 4), (0, 8), (32, 8), (64, 0)):
+        for tx, ty in ((0, 0), (0, 2), (0, 4), (0, 8), (32, 8), (64, 0), (16, 0)):
             for kc in (0, 5):
                 a.set_tuning(tile_x=tx, tile_y=ty, kchunk=kc)
                 outs = [torch.zeros_like(f) for f in fields]
