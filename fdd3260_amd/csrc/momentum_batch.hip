@@ -276,22 +276,30 @@ __global__ __launch_bounds__(TX *TY) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         fetch(k0 + 1, k0 + 1);
         store(k0 + 1, k0 + 1, false);
         fetch(k0 + 2, k0 + 1);
+        // step-start momenta of the thread's cell, requested one level ahead
+        const bool need_m0 = valid && a.epi != 0 && a.c0 != 0.0;
+        float m0n[3] = {0.0f, 0.0f, 0.0f};
+        auto fetch_m0 = [&](int kk) {
+            const size_t id = (size_t)(i - 1) + sy * (j - 1) + sz * (kk - 1);
+#pragma unroll
+            for (int n = 0; n < 3; ++n)
+                if (n != 1 || THREE_D) m0n[n] = a.m0[n][id];
+        };
+        if (need_m0) fetch_m0(k0);
 #pragma unroll 1
         for (int k = k0; k <= k1; ++k) {
             store(k + 2, k + 1, true);
+            const float m0v[3] = {m0n[0], m0n[1], m0n[2]};
             __syncthreads();
-            if (k < k1) fetch(k + 3, k + 2);
+            if (k < k1) {
+                fetch(k + 3, k + 2);
+                if (need_m0) fetch_m0(k + 1);
+            }
             // ring slot of the momenta at level k
             const int s0 = k % 3;
             if (valid) {
                 const size_t kb = sz * (k - 1);
                 const size_t id = (size_t)(i - 1) + sy * (j - 1) + kb;
-                float m0v[3] = {0.0f, 0.0f, 0.0f};
-                if (a.epi != 0 && a.c0 != 0.0) {
-#pragma unroll
-                    for (int n = 0; n < 3; ++n)
-                        if (n != 1 || THREE_D) m0v[n] = a.m0[n][id];
-                }
                 // operand n (a literal) at column offsets (di, dj) from this
                 // thread's column and level kk (k-2 <= kk <= k+2); the levels
                 // kept in registers are read only in the own column
