@@ -443,6 +443,7 @@ def adjoint(args):
         "gradient_norms": {"q0_lognormal": q_grad_norm, "q0_theta": theta_grad_norm},
         "cpu_baseline": None,
     }
+    adv.close()  # releases the trajectory storage before the next workload
     return out
 
 
