@@ -208,13 +208,16 @@ def test_adjoint_dot_product_full_size(cuda):
         print(f"  direction {i} checked", flush=True)
 
 
-def test_adjoint_checkpoint_interval_is_bit_invariant(cuda):
-    print("case adjoint checkpoint invariance", flush=True)
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
+def test_adjoint_checkpoint_interval_is_bit_invariant(cuda, scheme):
+    """Checkpoint 0 (the whole trajectory stored), 1, 3 and 7 give the same
+    gradient bit for bit: the replays recompute identical states."""
+    print(f"case adjoint checkpoint invariance {scheme}", flush=True)
     g, u, v, w, q0 = _case(True, "gaussian", seed=91, nz=20)
     dt = cases.cfl_dt(g, u, v, w, 0.5)
     lam = _interior_only(g, np.random.default_rng(92).standard_normal(g.shape))
     outs = []
-    with Advection(g, scal_adv="weno") as a:
+    with Advection(g, scal_adv=scheme) as a:
         U, V, W, Q0, L = _dev(cuda, u, v, w, q0, lam)
         for ck in (0, 1, 3, 7):
             qb = L.clone()
@@ -260,13 +263,14 @@ def test_recorded_forward_and_adjoint_match(cuda):
                 assert torch.equal(x, y)
 
 
-def test_adjoint_kchunk_is_invariant(cuda):
+@pytest.mark.parametrize("scheme", ["weno", "weno5"])
+def test_adjoint_kchunk_is_invariant(cuda, scheme):
     """Level chunking of the adjoint kernel changes only the work split."""
-    print("case adjoint kchunk invariance", flush=True)
+    print(f"case adjoint kchunk invariance {scheme}", flush=True)
     g, u, v, w, q = _case(True, "gaussian", seed=95, nz=23)
     lam = _interior_only(g, np.random.default_rng(96).standard_normal(g.shape))
     outs = []
-    with Advection(g, scal_adv="weno") as a:
+    with Advection(g, scal_adv=scheme) as a:
         U, V, W, Q, L = _dev(cuda, u, v, w, q, lam)
         for kc in (0, 1, 5, 23):
             a.set_tuning(kchunk=kc)
