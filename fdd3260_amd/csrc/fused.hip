@@ -138,20 +138,6 @@ struct Slots {
     }
 };
 
-// Element at a 32-bit byte offset from a wave-uniform base pointer: the
-// global load/store takes the base from scalar registers and the offset
-// from one vector register, with no 64-bit vector address arithmetic.
-template <class T>
-__device__ __forceinline__ T load_at(const T *base, uint32_t off)
-{
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-}
-template <class T>
-__device__ __forceinline__ void store_at(T *base, uint32_t off, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
-}
-
 // One field marched up the item's levels.  Per level: the prefetched tile
 // goes to LDS, the next level's tile, face velocities, window value and
 // step-start value are requested, every thread evaluates its face slots

@@ -252,6 +252,20 @@ __device__ __forceinline__ T zface(int f, int nz, const T *zw, T ww, const Fused
     }
 }
 
+// Element at a 32-bit byte offset from a wave-uniform base pointer: the
+// global load/store takes the base from scalar registers and the offset
+// from one vector register, with no 64-bit vector address arithmetic.
+template <class T>
+__device__ __forceinline__ T load_at(const T *base, uint32_t off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ void store_at(T *base, uint32_t off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
 __device__ __forceinline__ int wrap(int i, int n, int m)
 {
     // 1-based index i mapped into [1, n]; outside positions take their

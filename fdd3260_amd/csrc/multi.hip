@@ -96,11 +96,16 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
     constexpr int H = stencil_h<SCHEME>();
     const int kstart = k0 > 1 ? -1 : 0;
     const int ks = k0 + kstart;
+    // byte offsets in a level plane: this thread's column, its tile slots
+    const uint32_t colb = (uint32_t)col * 4u;
+    uint32_t tb[TS];
+#pragma unroll
+    for (int s = 0; s < TS; ++s) tb[s] = (uint32_t)sl.t_off[s] * 4u;
     // window q(ks-H+1..ks+H), levels below 1 read as q(1)
     float zw[2 * H];
 #pragma unroll
-    for (int m = 0; m < 2 * H; ++m) zw[m] = q[col + lev(ks - H + 1 + m)];
-    const float wz0 = ks < nz ? A.w[col + lev(ks + 1)] : 0.0f;
+    for (int m = 0; m < 2 * H; ++m) zw[m] = load_at(q + lev(ks - H + 1 + m), colb);
+    const float wz0 = ks < nz ? load_at(A.w + lev(ks + 1), colb) : 0.0f;
     float fzb = 0.0f;
     float tv[TS], q0v = 0.0f, zn = 0.0f;
     double d1v = 0.0;
@@ -110,9 +115,9 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
         const float *qk = q + kb;
 #pragma unroll
         for (int s = 0; s < TS; ++s)
-            if (sl.t_off[s] >= 0) tv[s] = qk[sl.t_off[s]];
-        if (need_q0 && valid) q0v = q0[col + kb];
-        zn = q[col + lev(k + H)];
+            if (sl.t_off[s] >= 0) tv[s] = load_at(qk, tb[s]);
+        if (need_q0 && valid) q0v = load_at(q0 + kb, colb);
+        zn = load_at(q + lev(k + H), colb);
         d1v = A.dz1w[k - 1];
     };
     const int ex = ty * (G.TX + 1) + tx, ey = G.NXF + ty * G.TX + tx;
@@ -150,19 +155,19 @@ __device__ __forceinline__ void march_multi(const FusedArgs &A, const Geom &G, c
         }
         __syncthreads();
         if (valid) {
-            const size_t id = col + (size_t)(k - 1) * plane;
+            float *outk = out + (size_t)(k - 1) * plane;
             double div = (double(lf[ex + 1]) - double(lf[ex])) * A.dx1;
             div += (double(lf[ey + G.TX]) - double(lf[ey])) * A.dy1;
             div += (double(fzt) - double(fzb)) * dz1w_k;
             const double a = -div;
             if (A.epi == 0) {
-                out[id] = float(a);
+                store_at(outk, colb, float(a));
             } else {
                 double qn;
                 if (A.c0 != 0.0) qn = A.c0 * double(q0c) + A.c1 * (double(zw[H - 1]) + A.dt * a);
                 else qn = A.c1 * (double(zw[H - 1]) + A.dt * a);
                 const float qv = float(qn);
-                out[id] = qv;
+                store_at(outk, colb, qv);
                 if (A.fill) {
                     const int i = i0 + tx, j = j0 + ty;
                     int ix = 0, jy = 0;
