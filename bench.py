@@ -47,7 +47,7 @@ HBM_PEAK_GBS = 8000.0
 HBM_TRAFFIC = {
     ("plain", "256,256,128", "weno", "heun", 2): 551.2e6,
     ("adjoint", "256,256,128", "weno", "ssprk3", 1): 1078.2e6,
-    ("fused", "f32", "512,512,160", "weno", "ssprk3", 20): 13.823e9,
+    ("fused", "f32", "512,512,160", "weno", "ssprk3", 20): 13.542e9,
 }
 SCHEME_ID = {"weno": 0, "ppm": 1, "muscl": 2, "weno5": 3, "quick": 4, "lw": 5}
 
