@@ -568,7 +568,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="steps of the CPU-baseline sample (0: skip; default 1 fused, 6 plain)")
     ap.add_argument("--no-subrecords", action="store_true",
-                    help="fused workload: skip the plain, adjoint and multi-domain sub-records")
+                    help="fused workload: skip the sub-records (plain, adjoint, multi-domain, and the "
+                         "WENO5 batch, adjoint and multi-domain)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: all host cores (at most 16)")
     ap.add_argument("--workload", default="fused", choices=["plain", "fused", "adjoint", "multi"])
     ap.add_argument("--ndom", type=int, default=48, help="multi workload: number of subdomains")
