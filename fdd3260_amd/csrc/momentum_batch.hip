@@ -420,7 +420,9 @@ hipError_t launch_momentum_batch(const GridDev &g, int mom_ord, int lavisc, cons
     c.dy2 = (float)(1.0 / (2.0 * g.dy));
     c.dy3 = (float)(1.0 / (3.0 * g.dy));
     c.lavisc = lavisc;
-    const int kchunk = g.nz < 16 ? g.nz : 16;
+    // 32-level chunks: the ring fill of a work item (its first four levels)
+    // is paid once per 32 levels (4 % faster than 16-level chunks)
+    const int kchunk = g.nz < 32 ? g.nz : 32;
     const int nchunk = (g.nz + kchunk - 1) / kchunk;
     if (g.ny > 1) {
         constexpr int TX = 32, TY = 8;

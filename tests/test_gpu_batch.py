@@ -216,14 +216,15 @@ def test_batch_momentum_tendency(cuda, mom_ord, lavisc):
                 assert _rel_rms(got, r, g) <= 1e-5, f"fp32 field {n}"
 
 
-@pytest.mark.parametrize("nz", [24, 17, 9])
+@pytest.mark.parametrize("nz", [40, 33, 9])
 @pytest.mark.parametrize("mom_ord,lavisc", [(4, True), (2, False)])
 def test_batch_momentum_matches_fp64_kernel(cuda, nz, mom_ord, lavisc):
     """The batched FP32 momentum kernel against the FP64 kernel of
     mimadv_momentum_tendency (itself bit-identical to the oracle's
-    qnadv_uvw) on FP32-representable inputs.  Columns march in chunks of 16
-    levels: nz = 17 leaves a last chunk of one level, nz = 9 a single chunk,
-    so every vertical closure line (k = 1, 2, nz-1, nz) meets chunk edges.
+    qnadv_uvw) on FP32-representable inputs.  Columns march in chunks of 32
+    levels: nz = 40 makes two chunks, nz = 33 leaves a last chunk of one
+    level, nz = 9 a single chunk, so every vertical closure line (k = 1, 2,
+    nz-1, nz) meets chunk edges.
     Held to 2e-6 of the field maximum at every cell; the batched path takes
     the no-flux floor and lid only."""
     print(f"case batch_momentum_vs_fp64 nz={nz} ord={mom_ord} lavisc={lavisc}", flush=True)
