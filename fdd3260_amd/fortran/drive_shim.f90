@@ -9,7 +9,12 @@
 !   output (stream): xs after advection_x; xs after advection_x, _y, _z in
 !          sequence; xs of advection_ac_arrays; ua va wa of qnadv_uvw_arrays;
 !          xs of advection_ac_1d_arrays with w0(k) = -1e-3*k and dref =
-!          den0(k) = 1.2 - 0.01*k (the conservative anelastic build)
+!          den0(k) = 1.2 - 0.01*k (the conservative anelastic build);
+!          with that density (avden0 = den0): the momenta of get_momentum_arrays
+!          on u v w, get_advective_arrays of those momenta (u0shift 1.25,
+!          v0shift -0.5), get_velocities_arrays of the momenta, and q after
+!          get_conserved_array then after its inverse (get_scalars), with
+!          dens(i,j,k) = den0(k) + 1e-3*mod(i+2j, 5)
 !
 program drive_shim
   use mimadv_advection
@@ -18,9 +23,9 @@ program drive_shim
   logical :: lim
   real(8) :: dx, dy, dz, dt0
   real(8), allocatable :: fdz0(:), dzw(:), dz1w(:), dz1(:), w0(:), den0(:)
-  integer :: k
+  integer :: i, j, k
   real(8), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:), q(:,:,:), mu(:,:,:), mv(:,:,:), mw(:,:,:)
-  real(8), allocatable :: xs(:,:,:), dummy(:,:,:), ua(:,:,:), va(:,:,:), wa(:,:,:)
+  real(8), allocatable :: xs(:,:,:), dummy(:,:,:), ua(:,:,:), va(:,:,:), wa(:,:,:), dens(:,:,:)
   character(len=512) :: fin, fout
   character(len=5) :: sname
   character(len=10) :: lname
@@ -86,6 +91,28 @@ program drive_shim
   call mimadv_set_density(den0, den0)
   xs = 0.d0
   call advection_ac_1d_arrays(w0, q, xs)
+  write(11) xs
+  ua = u
+  va = v
+  wa = w
+  call get_momentum_arrays(ua, va, wa)
+  write(11) ua, va, wa
+  call get_advective_arrays(1.25d0, -0.5d0, ua, va, wa, mu, mv, mw)
+  write(11) mu, mv, mw
+  call get_velocities_arrays(ua, va, wa)
+  write(11) ua, va, wa
+  allocate(dens(nx,ny,nz))
+  do k = 1, nz
+    do j = 1, ny
+      do i = 1, nx
+        dens(i,j,k) = den0(k) + 1.d-3 * mod(i + 2 * j, 5)
+      enddo
+    enddo
+  enddo
+  xs = q
+  call get_conserved_array(dens, xs, .false.)
+  write(11) xs
+  call get_conserved_array(dens, xs, .true.)
   write(11) xs
   close(11)
   call mimadv_finalize()
